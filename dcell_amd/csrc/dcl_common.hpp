@@ -1,0 +1,303 @@
+// dcl_common.hpp -- shared device helpers of the gfx950 level-set kernels.
+//
+// Coordinates (band nodes, departure points, particles) are global grid-index
+// doubles; values (phi, velocity, weights, sums) use the storage type T.
+// In-window offsets are 32-bit: the host rejects windows whose element count
+// (times dof) reaches 2^31.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dcell_lsm.h"
+
+namespace dcl {
+
+// window geometry as the kernels see it
+struct KWin {
+  int px, py, pz;
+  int nx, ny, nz;
+};
+
+struct KVel {
+  KWin w;
+  int dof, udof;
+};
+
+__host__ inline KWin kwin(const dcl_window& w) {
+  KWin k;
+  k.px = w.p.x; k.py = w.p.y; k.pz = w.p.z;
+  k.nx = w.n.x; k.ny = w.n.y; k.nz = w.n.z < 1 ? 1 : w.n.z;
+  return k;
+}
+
+__device__ __forceinline__ double clampd(double v, double lo, double hi) {
+  return fmin(fmax(v, lo), hi);  // NaN -> lo: a bad coordinate never leaves the window
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
+
+// linear split of a global coordinate: local base index in [0, n-2], fraction in [0, 1]
+__device__ __forceinline__ int lin_split(double s, int p, int n, double& f) {
+  s = clampd(s, (double)p, (double)(p + n - 1));
+  double fl = floor(s);
+  int b = (int)fl;
+  if (b > p + n - 2) b = p + n - 2;
+  f = s - (double)b;
+  return b - p;
+}
+
+// cubic split: local base index in [1, n-3] (stencil base-1 .. base+2)
+__device__ __forceinline__ int cub_split(double s, int p, int n, double& f) {
+  s = clampd(s, (double)(p + 1), (double)(p + n - 2));
+  double fl = floor(s);
+  int b = (int)fl;
+  if (b > p + n - 3) b = p + n - 3;
+  f = s - (double)b;
+  return b - p;
+}
+
+// Catmull-Rom (Keys a = -1/2) weights of the stencil offsets -1, 0, 1, 2
+template <typename T>
+__device__ __forceinline__ void cr_weights(T t, T w[4]) {
+  const T t2 = t * t, t3 = t2 * t;
+  w[0] = T(0.5) * (T(2) * t2 - t3 - t);
+  w[1] = T(0.5) * (T(3) * t3 - T(5) * t2 + T(2));
+  w[2] = T(0.5) * (T(4) * t2 + t - T(3) * t3);
+  w[3] = T(0.5) * (t3 - t2);
+}
+
+template <typename T>
+__device__ __forceinline__ void cr_dweights(T t, T dw[4]) {
+  const T t2 = t * t;
+  dw[0] = T(0.5) * (T(4) * t - T(3) * t2 - T(1));
+  dw[1] = T(0.5) * (T(9) * t2 - T(10) * t);
+  dw[2] = T(0.5) * (T(8) * t + T(1) - T(9) * t2);
+  dw[3] = T(0.5) * (T(3) * t2 - T(2) * t);
+}
+
+// Velocity at an integer node (the first stage of every backtrace).  With
+// the fractions 1/2 along the staggered axis and 0 elsewhere, the eight terms
+// of InterpolateVelocity3D (Common/InterpolateVelocity.c:35-67) reduce
+// exactly to the mean of the two faces around the node; the node is clamped
+// into [vp, vp+vn-2] first.
+template <typename T, int DIM>
+__device__ __forceinline__ void node_vel(const T* __restrict__ vel, const KVel& v, int X, int Y,
+                                         int Z, T V[DIM]) {
+  const int x = clampi(X, v.w.px, v.w.px + v.w.nx - 2) - v.w.px;
+  const int y = clampi(Y, v.w.py, v.w.py + v.w.ny - 2) - v.w.py;
+  const int sx = v.dof, sy = v.w.nx * v.dof;
+  if (DIM == 3) {
+    const int z = clampi(Z, v.w.pz, v.w.pz + v.w.nz - 2) - v.w.pz;
+    const int sz = v.w.ny * sy;
+    const T* b = vel + (z * v.w.ny + y) * sy + x * sx + v.udof;
+    V[0] = T(0.5) * b[0] + T(0.5) * b[sx];
+    V[1] = T(0.5) * b[1] + T(0.5) * b[sy + 1];
+    V[2] = T(0.5) * b[2] + T(0.5) * b[sz + 2];
+  } else {
+    const T* b = vel + y * sy + x * sx + v.udof;
+    V[0] = T(0.5) * b[0] + T(0.5) * b[sx];
+    V[1] = T(0.5) * b[1] + T(0.5) * b[sy + 1];
+  }
+}
+
+// MAC-staggered trilinear velocity at an arbitrary point: InterpolateVelocity3D
+// / 2D (Common/InterpolateVelocity.c:7-67), loop order k, j, i.  The
+// fraction split is done in double; weights and sums in T.
+template <typename T, int DIM>
+__device__ __forceinline__ void mac_vel(const T* __restrict__ vel, const KVel& v,
+                                        const double Xin[DIM], T V[DIM]) {
+  double X[3];
+  X[0] = clampd(Xin[0], (double)v.w.px, (double)(v.w.px + v.w.nx - 2));
+  X[1] = clampd(Xin[1], (double)v.w.py, (double)(v.w.py + v.w.ny - 2));
+  if (DIM == 3) X[2] = clampd(Xin[2], (double)v.w.pz, (double)(v.w.pz + v.w.nz - 2));
+  const int sx = v.dof, sy = v.w.nx * v.dof, sz = v.w.ny * sy;
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) {
+    const double hx = c == 0 ? 0.5 : 0.0, hy = c == 1 ? 0.5 : 0.0, hz = c == 2 ? 0.5 : 0.0;
+    const double fx = floor(X[0] + hx), fy = floor(X[1] + hy);
+    const double ax = X[0] - fx + hx, ay = X[1] - fy + hy;
+    const T wx1 = (T)ax, wx0 = (T)(1.0 - ax), wy1 = (T)ay, wy0 = (T)(1.0 - ay);
+    const int xs = (int)fx - v.w.px, ys = (int)fy - v.w.py;
+    if (DIM == 3) {
+      const double fz = floor(X[2] + hz);
+      const double az = X[2] - fz + hz;
+      const T wz1 = (T)az, wz0 = (T)(1.0 - az);
+      const int zs = (int)fz - v.w.pz;
+      const T* b = vel + zs * sz + ys * sy + xs * sx + v.udof + c;
+      T sum = T(0);
+      sum += wx0 * wy0 * wz0 * b[0];
+      sum += wx1 * wy0 * wz0 * b[sx];
+      sum += wx0 * wy1 * wz0 * b[sy];
+      sum += wx1 * wy1 * wz0 * b[sy + sx];
+      sum += wx0 * wy0 * wz1 * b[sz];
+      sum += wx1 * wy0 * wz1 * b[sz + sx];
+      sum += wx0 * wy1 * wz1 * b[sz + sy];
+      sum += wx1 * wy1 * wz1 * b[sz + sy + sx];
+      V[c] = sum;
+    } else {
+      const T* b = vel + ys * sy + xs * sx + v.udof + c;
+      T sum = T(0);
+      sum += wx0 * wy0 * b[0];
+      sum += wx1 * wy0 * b[sx];
+      sum += wx0 * wy1 * b[sy];
+      sum += wx1 * wy1 * b[sy + sx];
+      V[c] = sum;
+    }
+  }
+}
+
+// Bilinear3D with GridFunction3D_Identity (Common/Grid3D.c:4-31), order i, j, k
+template <typename T>
+__device__ __forceinline__ T trilinear(const T* __restrict__ phi, const KWin& w, const double S[3]) {
+  double fx, fy, fz;
+  const int xs = lin_split(S[0], w.px, w.nx, fx);
+  const int ys = lin_split(S[1], w.py, w.ny, fy);
+  const int zs = lin_split(S[2], w.pz, w.nz, fz);
+  const T sx = (T)fx, sy = (T)fy, sz = (T)fz;
+  const T ox = (T)(1.0 - fx), oy = (T)(1.0 - fy), oz = (T)(1.0 - fz);
+  const int Y = w.nx, Z = w.nx * w.ny;
+  const T* b = phi + zs * Z + ys * Y + xs;
+  T sum = T(0);
+  sum += b[0] * ox * oy * oz;
+  sum += b[Z] * ox * oy * sz;
+  sum += b[Y] * ox * sy * oz;
+  sum += b[Y + Z] * ox * sy * sz;
+  sum += b[1] * sx * oy * oz;
+  sum += b[1 + Z] * sx * oy * sz;
+  sum += b[1 + Y] * sx * sy * oz;
+  sum += b[1 + Y + Z] * sx * sy * sz;
+  return sum;
+}
+
+// Bilinear2D (Common/Grid.c:190-212): the (2x-1) form, x outer, y inner, /4
+template <typename T>
+__device__ __forceinline__ T bilinear_ref2d(const T* __restrict__ phi, const KWin& w, const double S[2]) {
+  double fx, fy;
+  const int xi = lin_split(S[0], w.px, w.nx, fx);
+  const int yi = lin_split(S[1], w.py, w.ny, fy);
+  const T xf = (T)(2. * fx - 1.), yf = (T)(2. * fy - 1.);
+  const T xb0 = T(1) - xf, xb1 = T(1) + xf, yb0 = T(1) - yf, yb1 = T(1) + yf;
+  const T* b = phi + yi * w.nx + xi;
+  T sum = T(0);
+  sum += b[0] * xb0 * yb0;
+  sum += b[w.nx] * xb0 * yb1;
+  sum += b[1] * xb1 * yb0;
+  sum += b[1 + w.nx] * xb1 * yb1;
+  return sum / T(4);
+}
+
+// GridInterpolate3D / 2D (Common/Grid.c:353-411), order k, j, i
+template <typename T, int DIM>
+__device__ __forceinline__ T grid_interp(const T* __restrict__ phi, const KWin& w, const double X[DIM]) {
+  double fx, fy, fz = 0;
+  const int xs = lin_split(X[0], w.px, w.nx, fx);
+  const int ys = lin_split(X[1], w.py, w.ny, fy);
+  const T wx1 = (T)fx, wx0 = (T)(1.0 - fx), wy1 = (T)fy, wy0 = (T)(1.0 - fy);
+  const int Y = w.nx;
+  if (DIM == 3) {
+    const int zs = lin_split(X[2], w.pz, w.nz, fz);
+    const T wz1 = (T)fz, wz0 = (T)(1.0 - fz);
+    const int Z = w.nx * w.ny;
+    const T* b = phi + zs * Z + ys * Y + xs;
+    T v = T(0);
+    v += wx0 * wy0 * wz0 * b[0];
+    v += wx1 * wy0 * wz0 * b[1];
+    v += wx0 * wy1 * wz0 * b[Y];
+    v += wx1 * wy1 * wz0 * b[Y + 1];
+    v += wx0 * wy0 * wz1 * b[Z];
+    v += wx1 * wy0 * wz1 * b[Z + 1];
+    v += wx0 * wy1 * wz1 * b[Z + Y];
+    v += wx1 * wy1 * wz1 * b[Z + Y + 1];
+    return v;
+  } else {
+    const T* b = phi + ys * Y + xs;
+    T v = T(0);
+    v += wx0 * wy0 * b[0];
+    v += wx1 * wy0 * b[1];
+    v += wx0 * wy1 * b[Y];
+    v += wx1 * wy1 * b[Y + 1];
+    return v;
+  }
+}
+
+// Catmull-Rom tricubic gather over the 4x4x4 stencil around S
+template <typename T>
+__device__ __forceinline__ T tricubic(const T* __restrict__ phi, const KWin& w, const double S[3]) {
+  double tx, ty, tz;
+  const int bx = cub_split(S[0], w.px, w.nx, tx);
+  const int by = cub_split(S[1], w.py, w.ny, ty);
+  const int bz = cub_split(S[2], w.pz, w.nz, tz);
+  T wx[4], wy[4], wz[4];
+  cr_weights<T>((T)tx, wx);
+  cr_weights<T>((T)ty, wy);
+  cr_weights<T>((T)tz, wz);
+  const int Y = w.nx, Z = w.nx * w.ny;
+  const T* b = phi + (bz - 1) * Z + (by - 1) * Y + (bx - 1);
+  T sum = T(0);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    T sy = T(0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const T* r = b + k * Z + j * Y;
+      T sx = T(0);
+      sx += wx[0] * r[0];
+      sx += wx[1] * r[1];
+      sx += wx[2] * r[2];
+      sx += wx[3] * r[3];
+      sy += wy[j] * sx;
+    }
+    sum += wz[k] * sy;
+  }
+  return sum;
+}
+
+template <typename T>
+__device__ __forceinline__ T bicubic(const T* __restrict__ phi, const KWin& w, const double S[2]) {
+  double tx, ty;
+  const int bx = cub_split(S[0], w.px, w.nx, tx);
+  const int by = cub_split(S[1], w.py, w.ny, ty);
+  T wx[4], wy[4];
+  cr_weights<T>((T)tx, wx);
+  cr_weights<T>((T)ty, wy);
+  const int Y = w.nx;
+  const T* b = phi + (by - 1) * Y + (bx - 1);
+  T sum = T(0);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const T* r = b + j * Y;
+    T sx = T(0);
+    sx += wx[0] * r[0];
+    sx += wx[1] * r[1];
+    sx += wx[2] * r[2];
+    sx += wx[3] * r[3];
+    sum += wy[j] * sx;
+  }
+  return sum;
+}
+
+// Ordered-integer keys: max/min on the key is max/min on the float value, so
+// float extrema can use the integer atomics (deterministic, order-free).
+__device__ __forceinline__ unsigned long long okey(double v) {
+  unsigned long long b = __double_as_longlong(v);
+  return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double okey_val(unsigned long long k) {
+  unsigned long long b = (k & 0x8000000000000000ull) ? (k & 0x7fffffffffffffffull) : ~k;
+  return __longlong_as_double(b);
+}
+__device__ __forceinline__ unsigned int okey(float v) {
+  unsigned int b = __float_as_uint(v);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float okey_val(unsigned int k) {
+  unsigned int b = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+  return __uint_as_float(b);
+}
+
+template <typename T> struct KeyOf;
+template <> struct KeyOf<double> { typedef unsigned long long type; };
+template <> struct KeyOf<float> { typedef unsigned int type; };
+
+}  // namespace dcl

@@ -1,0 +1,50 @@
+// dcl_host.hpp -- host-side helpers shared by the C-ABI translation units:
+// the per-thread error message, HIP error mapping and argument checks.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include "dcell_lsm.h"
+#include "dcl_common.hpp"
+
+namespace dcl {
+
+int set_error(int code, const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// window element count (times dof) must fit the kernels' 32-bit offsets
+inline int64_t win_nodes(const dcl_window& w) {
+  const int64_t nz = w.n.z < 2 ? 1 : w.n.z;
+  return (int64_t)w.n.x * w.n.y * nz;
+}
+
+int check_window(const dcl_window* w, const char* name, int min_n, int dof = 1);
+int check_velocity(const dcl_velocity* v, int dim);
+int check_precision(int precision);
+
+inline unsigned grid1d(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+}  // namespace dcl
+
+#define DCL_HIP(call)                                                          \
+  do {                                                                         \
+    hipError_t e__ = (call);                                                   \
+    if (e__ != hipSuccess) return dcl::hip_fail(e__, #call, __FILE__, __LINE__); \
+  } while (0)
+
+#define DCL_LAUNCHED(name)                                                     \
+  do {                                                                         \
+    hipError_t e__ = hipGetLastError();                                        \
+    if (e__ != hipSuccess) return dcl::hip_fail(e__, name, __FILE__, __LINE__); \
+  } while (0)
+
+#define DCL_TRY(call)          \
+  do {                         \
+    int rc__ = (call);         \
+    if (rc__) return rc__;     \
+  } while (0)

@@ -1,0 +1,114 @@
+// runtime.hip -- error reporting, argument checks and the memory helpers of
+// the C ABI (dcell_lsm.h).
+#include <stdio.h>
+#include <string.h>
+
+#include "dcl_host.hpp"
+
+namespace dcl {
+
+static thread_local char g_msg[512] = "";
+
+int set_error(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_msg, sizeof(g_msg), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what, const char* file, int line) {
+  return set_error(DCL_ERR_LIB, "%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+}
+
+int check_window(const dcl_window* w, const char* name, int min_n, int dof) {
+  if (!w) return set_error(DCL_ERR_ARG_NULL, "%s window is NULL", name);
+  const bool two = w->n.z < 2;
+  if (w->n.x < min_n || w->n.y < min_n || (!two && w->n.z < min_n))
+    return set_error(DCL_ERR_ARG_SIZ, "%s window %dx%dx%d: every axis needs at least %d nodes",
+                     name, w->n.x, w->n.y, w->n.z, min_n);
+  if (win_nodes(*w) * dof >= (int64_t)1 << 31)
+    return set_error(DCL_ERR_ARG_SIZ, "%s window has %lld elements; the limit is 2^31 - 1", name,
+                     (long long)(win_nodes(*w) * dof));
+  if (!(w->d.x > 0) || !(w->d.y > 0) || (!two && !(w->d.z > 0)))
+    return set_error(DCL_ERR_ARG_WRONG, "%s window spacing must be positive", name);
+  return 0;
+}
+
+int check_velocity(const dcl_velocity* v, int dim) {
+  if (!v) return set_error(DCL_ERR_ARG_NULL, "velocity is NULL");
+  if (!v->data) return set_error(DCL_ERR_ARG_NULL, "velocity data is NULL");
+  if ((v->win.n.z < 2 ? 2 : 3) != dim)
+    return set_error(DCL_ERR_ARG_WRONG, "velocity window is %dD, phi is %dD",
+                     v->win.n.z < 2 ? 2 : 3, dim);
+  if (v->udof < 0 || v->udof + dim > v->dof)
+    return set_error(DCL_ERR_ARG_WRONG, "velocity dof %d cannot hold %d components from udof %d",
+                     v->dof, dim, v->udof);
+  return check_window(&v->win, "velocity", 2, v->dof);
+}
+
+int check_precision(int precision) {
+  if (precision != DCL_F64 && precision != DCL_F32)
+    return set_error(DCL_ERR_ARG_WRONG, "precision %d is neither DCL_F64 nor DCL_F32", precision);
+  return 0;
+}
+
+}  // namespace dcl
+
+using namespace dcl;
+
+extern "C" {
+
+int dcl_abi_version(void) { return DCL_ABI_VERSION; }
+
+const char* dcl_last_error(void) { return g_msg; }
+
+int dcl_device_count(int* count) {
+  if (!count) return set_error(DCL_ERR_ARG_NULL, "count is NULL");
+  DCL_HIP(hipGetDeviceCount(count));
+  return 0;
+}
+
+int dcl_malloc(void** ptr, size_t bytes) {
+  if (!ptr) return set_error(DCL_ERR_ARG_NULL, "ptr is NULL");
+  *ptr = nullptr;
+  if (bytes == 0) return 0;
+  hipError_t e = hipMalloc(ptr, bytes);
+  if (e != hipSuccess) {
+    *ptr = nullptr;
+    (void)hipGetLastError();
+    return set_error(DCL_ERR_MEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  }
+  return 0;
+}
+
+int dcl_free(void* ptr) {
+  if (ptr) DCL_HIP(hipFree(ptr));
+  return 0;
+}
+
+int dcl_memcpy_htod(void* dst, const void* src, size_t bytes, void* stream) {
+  if (!bytes) return 0;
+  DCL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, as_stream(stream)));
+  return 0;
+}
+
+int dcl_memcpy_dtoh(void* dst, const void* src, size_t bytes, void* stream) {
+  if (!bytes) return 0;
+  DCL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, as_stream(stream)));
+  DCL_HIP(hipStreamSynchronize(as_stream(stream)));
+  return 0;
+}
+
+int dcl_memcpy_dtod(void* dst, const void* src, size_t bytes, void* stream) {
+  if (!bytes) return 0;
+  DCL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, as_stream(stream)));
+  return 0;
+}
+
+int dcl_stream_synchronize(void* stream) {
+  DCL_HIP(hipStreamSynchronize(as_stream(stream)));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,198 @@
+// sl_advect.hip -- semi-Lagrangian advection of phi on gfx950.
+//
+// Replaces the band loop of LevelSetAdvectSL_3D / _2D
+// (LevelSetMethod/AdvectSL.c:23-89): per node, the MAC velocity at the node
+// (InterpolateVelocity3D, Common/InterpolateVelocity.c:35-67), the departure
+// point (Euler, AdvectSL.c:80-82, or the RK2 midpoint rule of
+// ParticleLS3D.c:25-32 run backwards) and a gather of phi0 there (Bilinear3D,
+// Common/Grid3D.c:4-31, or the Catmull-Rom tricubic of TricubicInverse.h).
+//
+// Dense windows: one thread per node, 64x4 node tiles so each wavefront reads
+// a contiguous x-run of phi0 and velocity; blocks are numbered so that the
+// eight XCDs each take a contiguous run of z-slabs (their L2s then hold the
+// slabs their gathers revisit).  Band lists: one thread per band node.
+#include "dcl_host.hpp"
+
+namespace dcl {
+
+struct SLP {
+  KWin w;
+  KVel v;
+  double c[3];   // dt / dh per axis
+  double ch[3];  // dt / (2 dh) per axis
+};
+
+template <typename T, int DIM, int INTERP, int BT>
+__device__ __forceinline__ T sl_point(const T* __restrict__ phi0, const T* __restrict__ vel,
+                                      const SLP& P, int X, int Y, int Z) {
+  T V[DIM];
+  node_vel<T, DIM>(vel, P.v, X, Y, Z, V);
+  const double Xd[3] = {(double)X, (double)Y, (double)Z};
+  double S[3] = {0, 0, 0};
+  if (BT == DCL_RK2) {
+    double Xm[3] = {0, 0, 0};
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) Xm[a] = Xd[a] - (double)V[a] * P.ch[a];
+    T V2[DIM];
+    mac_vel<T, DIM>(vel, P.v, Xm, V2);
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) S[a] = Xd[a] - (double)V2[a] * P.c[a];
+  } else {
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) S[a] = Xd[a] - (double)V[a] * P.c[a];
+  }
+  if (INTERP == DCL_CUBIC) return DIM == 3 ? tricubic<T>(phi0, P.w, S) : bicubic<T>(phi0, P.w, S);
+  return DIM == 3 ? trilinear<T>(phi0, P.w, S) : bilinear_ref2d<T>(phi0, P.w, S);
+}
+
+// Map the hardware block id so that consecutive ids on one XCD (ids are dealt
+// round-robin over the 8 XCDs) walk consecutive tiles.
+__device__ __forceinline__ unsigned xcd_remap(unsigned hw, unsigned nblocks) {
+  const unsigned per = nblocks / 8u;
+  if (per == 0 || hw >= per * 8u) return hw;
+  return (hw % 8u) * per + hw / 8u;
+}
+
+template <typename T, int DIM, int INTERP, int BT>
+__global__ __launch_bounds__(256) void sl_dense(const T* __restrict__ phi0, T* __restrict__ phi,
+                                                const T* __restrict__ vel, SLP P, unsigned tx,
+                                                unsigned ty) {
+  const unsigned nb = tx * ty * gridDim.y;
+  const unsigned lb = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, nb);
+  const unsigned bx = lb % tx, by = (lb / tx) % ty, bz = lb / (tx * ty);
+  const int i = bx * 64 + threadIdx.x;
+  const int j = by * 4 + threadIdx.y;
+  const int k = bz;
+  if (i >= P.w.nx || j >= P.w.ny) return;
+  const int id = (k * P.w.ny + j) * P.w.nx + i;
+  const bool in = i >= 2 && i <= P.w.nx - 3 && j >= 2 && j <= P.w.ny - 3 &&
+                  (DIM == 2 || (k >= 2 && k <= P.w.nz - 3));
+  if (!in) {
+    phi[id] = phi0[id];
+    return;
+  }
+  phi[id] = sl_point<T, DIM, INTERP, BT>(phi0, vel, P, P.w.px + i, P.w.py + j, P.w.pz + k);
+}
+
+template <typename T, int DIM, int INTERP, int BT, bool COMPACT>
+__global__ __launch_bounds__(256) void sl_band(const T* __restrict__ phi0, T* __restrict__ out,
+                                               const T* __restrict__ vel, SLP P,
+                                               const dcl_icoor* __restrict__ band, int64_t n) {
+  const int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (b >= n) return;
+  const dcl_icoor q = band[b];
+  const T v = sl_point<T, DIM, INTERP, BT>(phi0, vel, P, q.x, q.y, DIM == 3 ? q.z : P.w.pz);
+  if (COMPACT) {
+    out[b] = v;
+  } else {
+    const int x = q.x - P.w.px, y = q.y - P.w.py, z = DIM == 3 ? q.z - P.w.pz : 0;
+    if (x < 0 || x >= P.w.nx || y < 0 || y >= P.w.ny || z < 0 || z >= P.w.nz) return;
+    out[(z * P.w.ny + y) * P.w.nx + x] = v;
+  }
+}
+
+template <typename T, int DIM, int INTERP, int BT>
+static int launch_sl(const void* phi0, void* phi, const void* vel, const SLP& P,
+                     const dcl_icoor* band, int64_t band_len, void* band_values,
+                     hipStream_t s) {
+  const T* p0 = static_cast<const T*>(phi0);
+  const T* v = static_cast<const T*>(vel);
+  if (!band) {
+    const unsigned tx = (P.w.nx + 63) / 64, ty = (P.w.ny + 3) / 4;
+    dim3 grid(tx * ty, DIM == 3 ? P.w.nz : 1);
+    hipLaunchKernelGGL((sl_dense<T, DIM, INTERP, BT>), grid, dim3(64, 4), 0, s, p0,
+                       static_cast<T*>(phi), v, P, tx, ty);
+    DCL_LAUNCHED("sl_dense");
+    return 0;
+  }
+  if (band_len == 0) return 0;
+  const unsigned g = grid1d(band_len, 256);
+  if (band_values)
+    hipLaunchKernelGGL((sl_band<T, DIM, INTERP, BT, true>), dim3(g), dim3(256), 0, s, p0,
+                       static_cast<T*>(band_values), v, P, band, band_len);
+  else
+    hipLaunchKernelGGL((sl_band<T, DIM, INTERP, BT, false>), dim3(g), dim3(256), 0, s, p0,
+                       static_cast<T*>(phi), v, P, band, band_len);
+  DCL_LAUNCHED("sl_band");
+  return 0;
+}
+
+template <typename T>
+static int dispatch_sl(int dim, int interp, int bt, const void* phi0, void* phi, const void* vel,
+                       const SLP& P, const dcl_icoor* band, int64_t n, void* bv, hipStream_t s) {
+#define DCL_SL(D, I, B) \
+  if (dim == D && interp == I && bt == B) return launch_sl<T, D, I, B>(phi0, phi, vel, P, band, n, bv, s);
+  DCL_SL(3, DCL_LINEAR, DCL_EULER)
+  DCL_SL(3, DCL_LINEAR, DCL_RK2)
+  DCL_SL(3, DCL_CUBIC, DCL_EULER)
+  DCL_SL(3, DCL_CUBIC, DCL_RK2)
+  DCL_SL(2, DCL_LINEAR, DCL_EULER)
+  DCL_SL(2, DCL_LINEAR, DCL_RK2)
+  DCL_SL(2, DCL_CUBIC, DCL_EULER)
+  DCL_SL(2, DCL_CUBIC, DCL_RK2)
+#undef DCL_SL
+  return set_error(DCL_ERR_ARG_WRONG, "interp %d / backtrace %d not supported", interp, bt);
+}
+
+int sl_params(const dcl_window* win, const dcl_velocity* vel, double dt, SLP& P) {
+  P.w = kwin(*win);
+  P.v.w = kwin(vel->win);
+  P.v.dof = vel->dof;
+  P.v.udof = vel->udof;
+  const double d[3] = {win->d.x, win->d.y, win->d.z};
+  for (int a = 0; a < 3; ++a) {
+    P.c[a] = d[a] > 0 ? dt / d[a] : 0.0;
+    P.ch[a] = d[a] > 0 ? dt / (2.0 * d[a]) : 0.0;
+  }
+  return 0;
+}
+
+int sl_launch(const void* phi0, void* phi, const dcl_window* win, const dcl_velocity* vel,
+              const dcl_icoor* band, int64_t band_len, void* band_values, double dt,
+              int precision, int interp, int backtrace, hipStream_t s) {
+  const int dim = win->n.z < 2 ? 2 : 3;
+  SLP P;
+  sl_params(win, vel, dt, P);
+  if (precision == DCL_F64)
+    return dispatch_sl<double>(dim, interp, backtrace, phi0, phi, vel->data, P, band, band_len,
+                               band_values, s);
+  return dispatch_sl<float>(dim, interp, backtrace, phi0, phi, vel->data, P, band, band_len,
+                            band_values, s);
+}
+
+int check_sl_args(const void* phi0, void* phi, const dcl_window* win, const dcl_velocity* vel,
+                  const dcl_icoor* band, int64_t band_len, void* band_values, int precision,
+                  int interp, int backtrace) {
+  DCL_TRY(check_precision(precision));
+  if (!phi0) return set_error(DCL_ERR_ARG_NULL, "phi0 is NULL");
+  const int min_n = band ? 2 : 5;
+  DCL_TRY(check_window(win, "phi", min_n));
+  const int dim = win->n.z < 2 ? 2 : 3;
+  DCL_TRY(check_velocity(vel, dim));
+  if (interp != DCL_LINEAR && interp != DCL_CUBIC)
+    return set_error(DCL_ERR_ARG_WRONG, "interp %d is neither DCL_LINEAR nor DCL_CUBIC", interp);
+  if (backtrace != DCL_EULER && backtrace != DCL_RK2)
+    return set_error(DCL_ERR_ARG_WRONG, "backtrace %d is neither DCL_EULER nor DCL_RK2", backtrace);
+  if (interp == DCL_CUBIC && (win->n.x < 4 || win->n.y < 4 || (dim == 3 && win->n.z < 4)))
+    return set_error(DCL_ERR_ARG_SIZ, "cubic interpolation needs 4 nodes per axis");
+  if (band && band_len < 0) return set_error(DCL_ERR_ARG_SIZ, "band_len %lld < 0", (long long)band_len);
+  if (!band || !band_values) {
+    if (!phi) return set_error(DCL_ERR_ARG_NULL, "phi is NULL");
+    if (phi == phi0) return set_error(DCL_ERR_ARG_WRONG, "phi and phi0 must be distinct buffers");
+  }
+  return 0;
+}
+
+}  // namespace dcl
+
+using namespace dcl;
+
+extern "C" int dcl_sl_advect(const void* phi0, void* phi, const dcl_window* win,
+                             const dcl_velocity* vel, const dcl_icoor* band, int64_t band_len,
+                             void* band_values, double dt, int precision, int interp,
+                             int backtrace, void* stream) {
+  DCL_TRY(check_sl_args(phi0, phi, win, vel, band, band_len, band_values, precision, interp,
+                        backtrace));
+  return sl_launch(phi0, phi, win, vel, band, band_len, band_values, dt, precision, interp,
+                   backtrace, as_stream(stream));
+}

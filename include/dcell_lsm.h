@@ -1,0 +1,224 @@
+/*
+ * dcell_lsm.h -- C ABI of the MI355X (gfx950) level-set update of DCell.
+ *
+ * Plain C: raw device pointers, sizes and a stream handle (a hipStream_t
+ * passed as void*, NULL = the default stream).  Every entry point returns 0 on
+ * success or one of the DCL_ERR_* codes below, which take PETSc's numeric
+ * values so a DCell caller can CHKERRQ() them; dcl_last_error() gives the
+ * message of the last failure on the calling thread.  Entry points only
+ * enqueue work on the stream unless they document a host result.
+ *
+ * Layout (the reference's own, Common/Grid.c:69-93): a window is a dense box
+ * of nodes with global offset p and size n; scalars are row-major, x fastest,
+ * v[((z-p.z)*n.y + (y-p.y))*n.x + (x-p.x)]; vector grids are array-of-structs
+ * [z][y][x][dof].  n.z < 2 means 2D (Common/Grid.c:21).  Positions (band
+ * nodes, departure points, particles) are global grid-index coordinates
+ * (LevelSetMethod/AdvectSL.c:74-82).  Velocity is MAC-staggered: component c
+ * of node i sits half a cell below i along axis c (Common/Common.h:26-30,
+ * Common/InterpolateVelocity.c:45-50).
+ *
+ * Precision: DCL_F64 stores and computes in double.  DCL_F32 stores phi,
+ * velocity and curvature in float and computes weights and sums in float;
+ * departure coordinates and their index/fraction split stay in double.
+ *
+ * Boundary policy (the reference reads outside the window where a departure
+ * point leaves it; these clamps are no-ops wherever it does not):
+ *   velocity sample points are clamped into [vp, vp+vn-2] per axis;
+ *   linear phi samples into [p, p+n-1] with base index <= p+n-2;
+ *   cubic phi samples into [p+1, p+n-2] with base index <= p+n-3;
+ *   dense SL updates nodes in [p+2, p+n-3] and carries the two outer layers
+ *   over from phi0; curvature is 0 where its stencil leaves the window;
+ *   particle-correction corner writes outside the window are dropped.
+ *
+ * Calls that share device buffers must be ordered by the caller (same
+ * stream).  The library keeps no global state besides the per-thread error
+ * message and the objects created by the *_create functions.
+ */
+#ifndef DCELL_LSM_H
+#define DCELL_LSM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DCL_ABI_VERSION 1
+
+/* return codes (PETSc's values for the same conditions) */
+#define DCL_SUCCESS 0
+#define DCL_ERR_MEM 55       /* device or host allocation failed */
+#define DCL_ERR_SUP 56       /* unsupported combination of options */
+#define DCL_ERR_ARG_SIZ 60   /* a size or count is out of range */
+#define DCL_ERR_ARG_WRONG 62 /* an option value is invalid */
+#define DCL_ERR_LIB 76       /* the HIP runtime reported an error */
+#define DCL_ERR_ARG_NULL 85  /* a required pointer is NULL */
+
+/* iCoor (Common/Common.h:32-34) */
+typedef struct { int32_t x, y, z; } dcl_icoor;
+/* Coor (Common/Common.h:28-30) */
+typedef struct { double x, y, z; } dcl_coor;
+/* the geometry fields of struct _Grid (Common/Grid.h:11-13) */
+typedef struct {
+  dcl_icoor p; /* global index of the first node */
+  dcl_icoor n; /* nodes per axis; n.z < 2 is a 2D window */
+  dcl_coor d;  /* spacing in units of length */
+} dcl_window;
+
+/* struct _Particle (LevelSetMethod/LSM_private.h:45-48): position in global
+ * index units and a signed radius in grid units; 32 bytes */
+typedef struct { double x, y, z, radius; } dcl_particle;
+/* the DCL_F32 form of a particle; 16 bytes */
+typedef struct { float x, y, z, radius; } dcl_particle_f32;
+
+/* a velocity grid: the reference's velgrid (LevelSetMethod/Advect.c:12-22) */
+typedef struct {
+  const void *data; /* device, [z][y][x][dof] of the precision in use */
+  dcl_window win;
+  int32_t dof;      /* components stored per node: 2 (2D), 3 (3D), 4 fluid [p,u,v,w] */
+  int32_t udof;     /* index of u within a node: 0 for a velgrid, 1 for the fluid (U_FACE) */
+} dcl_velocity;
+
+enum { DCL_F64 = 0, DCL_F32 = 1 };
+enum { DCL_LINEAR = 0, DCL_CUBIC = 1 };
+enum { DCL_EULER = 0, DCL_RK2 = 1 };
+enum { DCL_CURV_STD = 0, DCL_CURV_LSS = 1 };
+/* reproduce the reference's corner base A.z = floor(X.y) (ParticleLS3D.c:76) */
+enum { DCL_PLS_REF_AZ = 1 };
+
+/* ---------------------------------------------------------------- runtime */
+int dcl_abi_version(void);
+const char *dcl_last_error(void);
+int dcl_device_count(int *count);
+int dcl_malloc(void **ptr, size_t bytes);
+int dcl_free(void *ptr);
+int dcl_memcpy_htod(void *dst, const void *src, size_t bytes, void *stream);
+int dcl_memcpy_dtoh(void *dst, const void *src, size_t bytes, void *stream);
+int dcl_memcpy_dtod(void *dst, const void *src, size_t bytes, void *stream);
+int dcl_stream_synchronize(void *stream);
+
+/* ------------------------------------------------------------- SL advect */
+/* One semi-Lagrangian step phi0 -> phi.
+ * Replaces the loop of LevelSetAdvectSL_3D / _2D (LevelSetMethod/AdvectSL.c:23-89),
+ * reached from LevelSetAdvectSL (AdvectSL.c:6-19).
+ *   band == NULL: dense; every node of the window is written (interior nodes
+ *     advected, the two outer layers copied from phi0).
+ *   band != NULL: the band_len nodes of band (global coordinates, the
+ *     reference's ls->band, LevelSetMethod.h:15) are advected; their values go
+ *     to phi at those nodes, or, when band_values != NULL, to band_values[b]
+ *     (phi is then not touched and may be NULL).
+ *   interp DCL_LINEAR is the reference's Bilinear3D / Bilinear2D gather
+ *     (Common/Grid3D.c:4-31, Common/Grid.c:190-212); DCL_CUBIC is the
+ *     Catmull-Rom tricubic / bicubic of LevelSetMethod/TricubicInverse.h +
+ *     TricubicInterpolation.h (OrthogonalProjection.c:48-79 in 2D).
+ *   backtrace DCL_EULER is the reference's S = X - V dt/dh (AdvectSL.c:80-82);
+ *     DCL_RK2 is the particle midpoint rule (ParticleLS3D.c:25-32) run
+ *     backwards: Xm = X - dt/2 V(X)/dh, S = X - dt V(Xm)/dh.
+ * vel->win may differ from *win (sample points are clamped into vel->win). */
+int dcl_sl_advect(const void *phi0, void *phi, const dcl_window *win,
+                  const dcl_velocity *vel, const dcl_icoor *band, int64_t band_len,
+                  void *band_values, double dt, int precision, int interp,
+                  int backtrace, void *stream);
+
+/* ------------------------------------------------------------ curvature */
+/* Curvature of phi at every node (band == NULL) or at band nodes.
+ * DCL_CURV_STD: GridFunction3D_Curv, 19 points (Common/Grid3D.c:83-108) in 3D;
+ *   GridFunction2D_Curv, 9 points (Common/Grid.c:231-254) in 2D; NaN -> 0.
+ * DCL_CURV_LSS (2D only): GridFunction2D_CurvLSS, the 9x9 least-squares
+ *   stencil of LevelSetMethod/Curvature.c:3-186, whose pxx and pyy are half the
+ *   second derivatives, as in the reference; no NaN guard.
+ * kappa is in 1/length: phi's units cancel, dh is physical. */
+int dcl_curvature(const void *phi, void *kappa, const dcl_window *win,
+                  const dcl_icoor *band, int64_t band_len, void *band_values,
+                  int precision, int stencil, void *stream);
+
+/* One dense level-set update: SL step phi0 -> phi (as dcl_sl_advect, dense)
+ * and the curvature of the new phi into kappa (as dcl_curvature,
+ * DCL_CURV_STD), fused where the shapes allow.  This is the unit measured as
+ * one grid-point update.  No counterpart in the reference, which samples
+ * curvature pointwise at interface nodes (IIM/SurfaceQuantities.c:111). */
+int dcl_levelset_step(const void *phi0, void *phi, void *kappa, const dcl_window *win,
+                      const dcl_velocity *vel, double dt, int precision, int interp,
+                      int backtrace, void *stream);
+
+/* ------------------------------------------------------ particle level set */
+/* RK2 midpoint particle advection, ParticleLS_AdvectParticles3D / 2D
+ * (LevelSetMethod/ParticleLS3D.c:6-36, ParticleLS2D.c:6-36).  particles is a
+ * device array of dcl_particle (DCL_F64) or dcl_particle_f32 (DCL_F32); dh is
+ * ls->phi->d; dim is 2 or 3. */
+int dcl_pls_advect(void *particles, int64_t count, const dcl_velocity *vel,
+                   dcl_coor dh, double dt, int dim, int precision, void *stream);
+
+/* Workspace of the escaped-particle correction: the phi+ / phi- extremum
+ * grids (kept at their neutral value between calls) and the list of touched
+ * nodes.  max_nodes = window nodes; max_events bounds 2^dim * escaped
+ * particles per call (the call fails with DCL_ERR_ARG_SIZ beyond it, leaving
+ * phi untouched). */
+typedef struct dcl_pls_ws_s *dcl_pls_ws;
+int dcl_pls_ws_create(int64_t max_nodes, int64_t max_events, int precision, void *stream,
+                      dcl_pls_ws *ws);
+int dcl_pls_ws_destroy(dcl_pls_ws ws);
+
+/* Escaped-particle correction, ParticleLS_ErrorCorrection3D / 2D
+ * (LevelSetMethod/ParticleLS3D.c:40-116, ParticleLS2D.c:40-115): particles on
+ * the wrong side of phi by more than their radius raise phi+ (or lower phi-)
+ * at the corners of their cell to s(s r - |X - a|); then phi = phi+ where
+ * |phi+| <= |phi-| else phi-, over the band (band != NULL) or over every node
+ * (band == NULL).  With band_values != NULL the merged band values are written
+ * there and phi is left as it was.  flags: DCL_PLS_REF_AZ. */
+int dcl_pls_correct(const void *particles, int64_t count, void *phi,
+                    const dcl_window *win, const dcl_icoor *band, int64_t band_len,
+                    void *band_values, int flags, int dim, int precision,
+                    dcl_pls_ws ws, void *stream);
+
+/* ParticleLS_AdjustRadii (LevelSetMethod/ParticleLS.c:158-185):
+ * r = s * clamp(s phi(X), rmin, rmax) with the reference's strict bounds. */
+int dcl_pls_adjust_radii(void *particles, int64_t count, const void *phi,
+                         const dcl_window *win, double rmin, double rmax, int dim,
+                         int precision, void *stream);
+
+/* ------------------------------------------------------------ reductions */
+/* max |V| over nodes, LevelSet_MaxVelocity (LevelSetMethod/Advect.c:70-111).
+ * nodes: device array read with a byte stride (12 for dcl_icoor; the size of
+ * IrregularNode to pass ls->irregularNodes data as it is).  Synchronous: the
+ * result lands in *max_out (host). */
+int dcl_max_velocity(const dcl_velocity *vel, const void *nodes, int64_t count,
+                     int64_t stride_bytes, int dim, int precision, double *max_out,
+                     void *stream);
+
+/* Nodes with |phi| < width in raster order (z, y, x), written to band_out
+ * (device, capacity entries).  Synchronous: *count_out (host) is the total
+ * number found, which may exceed capacity (then only capacity are written).
+ * Replaces the band list the reference rebuilds in the FMM
+ * (LevelSetMethod/FastMarching.c:33,126-129) for GPU-resident use. */
+int dcl_band_compact(const void *phi, const dcl_window *win, double width,
+                     int precision, dcl_icoor *band_out, int64_t capacity,
+                     int64_t *count_out, void *stream);
+
+/* Interface measure with LevelSetDiracDelta3D / 2D
+ * (LevelSetMethod/NormalDirection.c:36-57) at the nodes [p+1, p+n-2]:
+ * out[0] = sum delta dV (area; length in 2D), out[1] = sum delta kappa dV
+ * (kappa may be NULL), out[2] = out[1]/out[0], out[3] = nodes with
+ * |phi| <= eps.  Synchronous, out is host memory. */
+int dcl_interface_metrics(const void *phi, const void *kappa, const dcl_window *win,
+                          double eps, int precision, double out[4], void *stream);
+
+/* max|a-b|, sum (a-b)^2, max|b|, n over n elements, a and b of their own
+ * precisions.  Synchronous, out is host memory. */
+int dcl_error_norms(const void *a, int prec_a, const void *b, int prec_b, int64_t n,
+                    double out[4], void *stream);
+
+/* --------------------------------------------------------------- adjoint */
+/* Reverse-mode derivative of one dense 3D SL step (dcl_sl_advect, dense):
+ * given lam_next = dL/dphi, accumulate lam += dL/dphi0 and gvel += dL/dvel
+ * (gvel has the layout of vel->data).  Transposed gather with atomics in LDS
+ * and then HBM.  No counterpart in the reference.  Requires vel->win == *win. */
+int dcl_sl_adjoint(const void *phi0, const dcl_window *win, const dcl_velocity *vel,
+                   const void *lam_next, void *lam, void *gvel, double dt,
+                   int precision, int interp, int backtrace, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DCELL_LSM_H */

@@ -1,0 +1,755 @@
+/*
+ * lsm_oracle.c -- CPU FP64 restatement of DCell's level-set update hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path and the timed CPU
+ * baseline.  Nothing in dcell_amd/ links or calls this file.
+ *
+ * Every routine names the reference code it restates.  Loop orders and the
+ * expression forms of the reference are kept (summation order matters only at
+ * the 1e-16 level, but keeping them makes the restatement easy to audit).
+ * Built with -ffp-contract=off so no FMA contraction changes the rounding.
+ *
+ * Parity pins (tests/test_oracle.py): closed forms for the interpolants,
+ * exact shifts, quadratic-exact curvature, sphere/circle curvature limits,
+ * the least-squares fit coefficients, hand-built PLS cases, the reference's
+ * own TricubicInverse.h / TricubicInterpolation.h compiled from the reference
+ * tree (oracle/ref_tricubic.c), brute-force numpy re-derivations on tiny
+ * grids, and central finite differences for the adjoint.
+ */
+#include "lsm_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_PI 3.14159265358979323846
+
+static int g_threads = 1;
+
+void orc_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
+int orc_get_threads(void) { return g_threads; }
+
+static inline int is2d(const orc_win *w) { return w->nz < 2; }
+
+static inline int64_t sidx(const orc_win *w, int x, int y, int z) {
+  int zz = is2d(w) ? 0 : z - w->pz;
+  return ((int64_t)zz * w->ny + (y - w->py)) * (int64_t)w->nx + (x - w->px);
+}
+
+/* fmin/fmax: a NaN coordinate clamps to lo, as on the GPU */
+static inline double clampd(double v, double lo, double hi) { return fmin(fmax(v, lo), hi); }
+
+/* PetscSign / PetscMax / PetscMin / PetscAbs as PETSc defines them */
+static inline double psign(double a) { return a >= 0 ? (a == 0 ? 0 : 1) : -1; }
+static inline double pmax(double a, double b) { return a < b ? b : a; }
+static inline double pmin(double a, double b) { return a < b ? a : b; }
+
+/* Tensor1 (Common/Common.h:26-30): the half-cell shift of the u, v, w faces */
+static const double T1[3][3] = {{0.5, 0, 0}, {0, 0.5, 0}, {0, 0, 0.5}};
+
+/* ---------------------------------------------------------------------------
+ * MAC velocity interpolation -- InterpolateVelocity3D (Common/InterpolateVelocity.c:35-67)
+ * The sample point is clamped into [vp, vp+vn-2] first (boundary policy).
+ * ------------------------------------------------------------------------- */
+static inline double vel_at(const double *vel, const orc_win *vw, int vdof,
+                            int x, int y, int z, int c) {
+  return vel[sidx(vw, x, y, z) * vdof + c];
+}
+
+void orc_interp_vel3d(const double *vel, const orc_win *vw, int vdof, int udof,
+                      const double Xin[3], double V[3]) {
+  double X[3];
+  X[0] = clampd(Xin[0], vw->px, vw->px + vw->nx - 2);
+  X[1] = clampd(Xin[1], vw->py, vw->py + vw->ny - 2);
+  X[2] = clampd(Xin[2], vw->pz, vw->pz + vw->nz - 2);
+  for (int dof = 0; dof < 3; dof++) {
+    int xs = (int)floor(X[0] + T1[dof][0]);
+    int ys = (int)floor(X[1] + T1[dof][1]);
+    int zs = (int)floor(X[2] + T1[dof][2]);
+    double sx = X[0] - xs + T1[dof][0];
+    double sy = X[1] - ys + T1[dof][1];
+    double sz = X[2] - zs + T1[dof][2];
+    double sum = 0;
+    for (int k = 0; k < 2; ++k)
+      for (int j = 0; j < 2; ++j)
+        for (int i = 0; i < 2; ++i)
+          sum += ((1 - sx) * (1 - i) + i * sx) * ((1 - sy) * (1 - j) + j * sy) *
+                 ((1 - sz) * (1 - k) + k * sz) *
+                 vel_at(vel, vw, vdof, xs + i, ys + j, zs + k, dof + udof);
+    V[dof] = sum;
+  }
+}
+
+/* InterpolateVelocity2D (Common/InterpolateVelocity.c:7-33) */
+void orc_interp_vel2d(const double *vel, const orc_win *vw, int vdof, int udof,
+                      const double Xin[2], double V[2]) {
+  double X[2];
+  X[0] = clampd(Xin[0], vw->px, vw->px + vw->nx - 2);
+  X[1] = clampd(Xin[1], vw->py, vw->py + vw->ny - 2);
+  for (int dof = 0; dof < 2; dof++) {
+    int xs = (int)floor(X[0] + T1[dof][0]);
+    int ys = (int)floor(X[1] + T1[dof][1]);
+    double sx = X[0] - xs + T1[dof][0];
+    double sy = X[1] - ys + T1[dof][1];
+    double sum = 0;
+    for (int j = 0; j < 2; ++j)
+      for (int i = 0; i < 2; ++i)
+        sum += ((1 - sx) * (1 - i) + i * sx) * ((1 - sy) * (1 - j) + j * sy) *
+               vel_at(vel, vw, vdof, xs + i, ys + j, vw->pz, dof + udof);
+    V[dof] = sum;
+  }
+}
+
+/* linear split of one coordinate under the boundary policy */
+static inline int lin_split(double s, int p, int n, double *f) {
+  s = clampd(s, p, p + n - 1);
+  int b = (int)floor(s);
+  if (b > p + n - 2) b = p + n - 2;
+  *f = s - b;
+  return b;
+}
+
+static inline int cub_split(double s, int p, int n, double *f) {
+  s = clampd(s, p + 1, p + n - 2);
+  int b = (int)floor(s);
+  if (b > p + n - 3) b = p + n - 3;
+  *f = s - b;
+  return b;
+}
+
+/* Bilinear3D with GridFunction3D_Identity (Common/Grid3D.c:4-36), loop order i, j, k */
+double orc_bilinear3d(const double *phi, const orc_win *w, const double S[3]) {
+  double sx, sy, sz;
+  int xs = lin_split(S[0], w->px, w->nx, &sx);
+  int ys = lin_split(S[1], w->py, w->ny, &sy);
+  int zs = lin_split(S[2], w->pz, w->nz, &sz);
+  double sum = 0.0;
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j)
+      for (int k = 0; k < 2; ++k) {
+        double xb = i * sx + (1 - i) * (1 - sx);
+        double yb = j * sy + (1 - j) * (1 - sy);
+        double zb = k * sz + (1 - k) * (1 - sz);
+        sum += phi[sidx(w, xs + i, ys + j, zs + k)] * xb * yb * zb;
+      }
+  return sum;
+}
+
+/* Bilinear2D (Common/Grid.c:190-212): the (2x-1) form, loop x outer, y inner, /4 */
+double orc_bilinear2d(const double *phi, const orc_win *w, const double S[2]) {
+  double fx, fy;
+  int xi = lin_split(S[0], w->px, w->nx, &fx);
+  int yi = lin_split(S[1], w->py, w->ny, &fy);
+  double xf = (2. * fx - 1.);
+  double yf = (2. * fy - 1.);
+  double sum = 0.;
+  for (int k = 0; k < 2; ++k)
+    for (int l = 0; l < 2; ++l) {
+      double xb = 1. + (2. * k - 1.) * xf;
+      double yb = 1. + (2. * l - 1.) * yf;
+      sum += phi[sidx(w, xi + k, yi + l, w->pz)] * xb * yb;
+    }
+  sum /= 4.;
+  return sum;
+}
+
+/* GridInterpolate3D (Common/Grid.c:381-411), loop order k, j, i */
+double orc_gridinterp3d(const double *phi, const orc_win *w, const double X[3]) {
+  double sx, sy, sz;
+  int xs = lin_split(X[0], w->px, w->nx, &sx);
+  int ys = lin_split(X[1], w->py, w->ny, &sy);
+  int zs = lin_split(X[2], w->pz, w->nz, &sz);
+  double val = 0;
+  for (int k = 0; k < 2; ++k)
+    for (int j = 0; j < 2; ++j)
+      for (int i = 0; i < 2; ++i)
+        val += ((1 - sx) * (1 - i) + i * sx) * ((1 - sy) * (1 - j) + j * sy) *
+               ((1 - sz) * (1 - k) + k * sz) * phi[sidx(w, xs + i, ys + j, zs + k)];
+  return val;
+}
+
+/* GridInterpolate2D (Common/Grid.c:353-377) */
+double orc_gridinterp2d(const double *phi, const orc_win *w, const double X[2]) {
+  double sx, sy;
+  int xs = lin_split(X[0], w->px, w->nx, &sx);
+  int ys = lin_split(X[1], w->py, w->ny, &sy);
+  double val = 0;
+  for (int j = 0; j < 2; ++j)
+    for (int i = 0; i < 2; ++i)
+      val += ((1 - sx) * (1 - i) + i * sx) * ((1 - sy) * (1 - j) + j * sy) *
+             phi[sidx(w, xs + i, ys + j, w->pz)];
+  return val;
+}
+
+/* ---------------------------------------------------------------------------
+ * Catmull-Rom (Keys a = -1/2) cubic.  TricubicInverse.h + TricubicInterpolation.h
+ * (used at LevelSetMethod/OrthogonalProjection3D.c:27,48) build the 64
+ * coefficients of the tricubic whose node derivatives are central differences
+ * on the 4x4x4 stencil; that polynomial is the tensor product of these 1-D
+ * weights (tests/test_oracle.py checks it against the compiled headers).
+ * ------------------------------------------------------------------------- */
+void orc_cr_weights(double t, double w[4]) {
+  double t2 = t * t, t3 = t2 * t;
+  w[0] = 0.5 * (2.0 * t2 - t3 - t);
+  w[1] = 0.5 * (3.0 * t3 - 5.0 * t2 + 2.0);
+  w[2] = 0.5 * (4.0 * t2 + t - 3.0 * t3);
+  w[3] = 0.5 * (t3 - t2);
+}
+
+void orc_cr_dweights(double t, double dw[4]) {
+  double t2 = t * t;
+  dw[0] = 0.5 * (4.0 * t - 3.0 * t2 - 1.0);
+  dw[1] = 0.5 * (9.0 * t2 - 10.0 * t);
+  dw[2] = 0.5 * (8.0 * t + 1.0 - 9.0 * t2);
+  dw[3] = 0.5 * (3.0 * t2 - 2.0 * t);
+}
+
+double orc_tricubic(const double *phi, const orc_win *w, const double S[3]) {
+  double tx, ty, tz, wx[4], wy[4], wz[4];
+  int bx = cub_split(S[0], w->px, w->nx, &tx);
+  int by = cub_split(S[1], w->py, w->ny, &ty);
+  int bz = cub_split(S[2], w->pz, w->nz, &tz);
+  orc_cr_weights(tx, wx);
+  orc_cr_weights(ty, wy);
+  orc_cr_weights(tz, wz);
+  double sum = 0;
+  for (int k = 0; k < 4; ++k) {
+    double sy = 0;
+    for (int j = 0; j < 4; ++j) {
+      double sx = 0;
+      for (int i = 0; i < 4; ++i) sx += wx[i] * phi[sidx(w, bx - 1 + i, by - 1 + j, bz - 1 + k)];
+      sy += wy[j] * sx;
+    }
+    sum += wz[k] * sy;
+  }
+  return sum;
+}
+
+double orc_bicubic(const double *phi, const orc_win *w, const double S[2]) {
+  double tx, ty, wx[4], wy[4];
+  int bx = cub_split(S[0], w->px, w->nx, &tx);
+  int by = cub_split(S[1], w->py, w->ny, &ty);
+  orc_cr_weights(tx, wx);
+  orc_cr_weights(ty, wy);
+  double sum = 0;
+  for (int j = 0; j < 4; ++j) {
+    double sx = 0;
+    for (int i = 0; i < 4; ++i) sx += wx[i] * phi[sidx(w, bx - 1 + i, by - 1 + j, w->pz)];
+    sum += wy[j] * sx;
+  }
+  return sum;
+}
+
+void orc_cr_stencil3(const double *st, double x, double y, double z, double out[4]) {
+  double wx[4], wy[4], wz[4], dx[4], dy[4], dz[4];
+  orc_cr_weights(x, wx); orc_cr_dweights(x, dx);
+  orc_cr_weights(y, wy); orc_cr_dweights(y, dy);
+  orc_cr_weights(z, wz); orc_cr_dweights(z, dz);
+  double p = 0, px = 0, py = 0, pz = 0;
+  for (int k = 0; k < 4; ++k)
+    for (int j = 0; j < 4; ++j)
+      for (int i = 0; i < 4; ++i) {
+        double f = st[(k * 4 + j) * 4 + i];
+        p += wx[i] * wy[j] * wz[k] * f;
+        px += dx[i] * wy[j] * wz[k] * f;
+        py += wx[i] * dy[j] * wz[k] * f;
+        pz += wx[i] * wy[j] * dz[k] * f;
+      }
+  out[0] = p; out[1] = px; out[2] = py; out[3] = pz;
+}
+
+/* ---------------------------------------------------------------------------
+ * SL step -- LevelSetAdvectSL_3D / _2D (LevelSetMethod/AdvectSL.c:23-89).
+ * EULER: S = X - V*dt/dh (AdvectSL.c:80-82).
+ * RK2 (extension): the particle midpoint rule of ParticleLS3D.c:25-32 run
+ * backwards, Xm = X - dt/2*V(X)/dh, S = X - dt*V(Xm)/dh.
+ * LINEAR: Bilinear3D / Bilinear2D (AdvectSL.c:84, :49).  CUBIC: Catmull-Rom.
+ * ------------------------------------------------------------------------- */
+static double sl_point3(const double *phi0, const double *vel, const orc_win *w,
+                        const orc_win *vw, int vdof, int udof, double dt,
+                        int interp, int backtrace, int x, int y, int z) {
+  double X[3] = {x, y, z}, V[3], S[3];
+  const double dh[3] = {w->dx, w->dy, w->dz};
+  orc_interp_vel3d(vel, vw, vdof, udof, X, V);
+  if (backtrace == ORC_RK2) {
+    double Xm[3];
+    for (int a = 0; a < 3; a++) Xm[a] = X[a] - dt / 2 * V[a] / dh[a];
+    orc_interp_vel3d(vel, vw, vdof, udof, Xm, V);
+    for (int a = 0; a < 3; a++) S[a] = X[a] - dt * V[a] / dh[a];
+  } else {
+    for (int a = 0; a < 3; a++) S[a] = X[a] - V[a] * dt / dh[a];
+  }
+  return interp == ORC_CUBIC ? orc_tricubic(phi0, w, S) : orc_bilinear3d(phi0, w, S);
+}
+
+static double sl_point2(const double *phi0, const double *vel, const orc_win *w,
+                        const orc_win *vw, int vdof, int udof, double dt,
+                        int interp, int backtrace, int x, int y) {
+  double X[2] = {x, y}, V[2], S[2];
+  const double dh[2] = {w->dx, w->dy};
+  orc_interp_vel2d(vel, vw, vdof, udof, X, V);
+  if (backtrace == ORC_RK2) {
+    double Xm[2];
+    for (int a = 0; a < 2; a++) Xm[a] = X[a] - dt / 2 * V[a] / dh[a];
+    orc_interp_vel2d(vel, vw, vdof, udof, Xm, V);
+    for (int a = 0; a < 2; a++) S[a] = X[a] - dt * V[a] / dh[a];
+  } else {
+    for (int a = 0; a < 2; a++) S[a] = X[a] - V[a] * dt / dh[a];
+  }
+  return interp == ORC_CUBIC ? orc_bicubic(phi0, w, S) : orc_bilinear2d(phi0, w, S);
+}
+
+static inline int interior(int v, int p, int n, int m) { return v >= p + m && v <= p + n - 1 - m; }
+
+int orc_sl_advect(const double *phi0, double *phi, const double *vel,
+                  const orc_icoor *band, int64_t band_len, double *band_out,
+                  const orc_win *w, const orc_win *vw, int vdof, int udof,
+                  double dt, int interp, int backtrace) {
+  const int two = is2d(w);
+  if (band) {
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+    for (int64_t b = 0; b < band_len; ++b) {
+      double v = two ? sl_point2(phi0, vel, w, vw, vdof, udof, dt, interp, backtrace, band[b].x, band[b].y)
+                     : sl_point3(phi0, vel, w, vw, vdof, udof, dt, interp, backtrace, band[b].x, band[b].y, band[b].z);
+      if (band_out) band_out[b] = v;
+      else phi[sidx(w, band[b].x, band[b].y, band[b].z)] = v;
+    }
+    return 0;
+  }
+  const int nz = two ? 1 : w->nz;
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+  for (int kz = 0; kz < nz; ++kz) {
+    int z = w->pz + kz;
+    for (int y = w->py; y < w->py + w->ny; ++y)
+      for (int x = w->px; x < w->px + w->nx; ++x) {
+        int64_t id = sidx(w, x, y, z);
+        int in = interior(x, w->px, w->nx, 2) && interior(y, w->py, w->ny, 2) &&
+                 (two || interior(z, w->pz, w->nz, 2));
+        if (!in) { phi[id] = phi0[id]; continue; }
+        phi[id] = two ? sl_point2(phi0, vel, w, vw, vdof, udof, dt, interp, backtrace, x, y)
+                      : sl_point3(phi0, vel, w, vw, vdof, udof, dt, interp, backtrace, x, y, z);
+      }
+  }
+  return 0;
+}
+
+/* ---------------------------------------------------------------------------
+ * Curvature
+ * ------------------------------------------------------------------------- */
+#define P3(i, j, k) phi[sidx(w, (i), (j), (k))]
+#define P2(i, j) phi[sidx(w, (i), (j), w->pz)]
+
+/* GridFunction3D_Curv (Common/Grid3D.c:83-108) */
+double orc_curv3d_at(const double *phi, const orc_win *w, int i, int j, int k) {
+  if (!(interior(i, w->px, w->nx, 1) && interior(j, w->py, w->ny, 1) && interior(k, w->pz, w->nz, 1)))
+    return 0;
+  const double dx = w->dx, dy = w->dy, dz = w->dz;
+  double px = (P3(i + 1, j, k) - P3(i - 1, j, k)) / (2.0 * dx);
+  double py = (P3(i, j + 1, k) - P3(i, j - 1, k)) / (2.0 * dy);
+  double pz = (P3(i, j, k + 1) - P3(i, j, k - 1)) / (2.0 * dz);
+  double px2 = px * px, py2 = py * py, pz2 = pz * pz;
+  double ppp = px2 + py2 + pz2;
+  double pxx = (P3(i - 1, j, k) - 2. * P3(i, j, k) + P3(i + 1, j, k)) / (dx * dx);
+  double pyy = (P3(i, j - 1, k) - 2. * P3(i, j, k) + P3(i, j + 1, k)) / (dy * dy);
+  double pzz = (P3(i, j, k - 1) - 2. * P3(i, j, k) + P3(i, j, k + 1)) / (dz * dz);
+  double pxy = (P3(i - 1, j - 1, k) + P3(i + 1, j + 1, k) - P3(i - 1, j + 1, k) - P3(i + 1, j - 1, k)) / (4. * dx * dy);
+  double pxz = (P3(i - 1, j, k - 1) + P3(i + 1, j, k + 1) - P3(i - 1, j, k + 1) - P3(i + 1, j, k - 1)) / (4. * dx * dz);
+  double pyz = (P3(i, j - 1, k - 1) + P3(i, j + 1, k + 1) - P3(i, j - 1, k + 1) - P3(i, j + 1, k - 1)) / (4. * dy * dz);
+  double c = px2 * (pyy + pzz) + py2 * (pxx + pzz) + pz2 * (pxx + pyy);
+  c -= 2. * (px * py * pxy + px * pz * pxz + py * pz * pyz);
+  c /= sqrt(ppp * ppp * ppp);
+  if (c != c) c = 0;
+  return c;
+}
+
+/* GridFunction2D_Curv (Common/Grid.c:231-254) */
+double orc_curv2d_at(const double *phi, const orc_win *w, int i, int j) {
+  if (!(interior(i, w->px, w->nx, 1) && interior(j, w->py, w->ny, 1))) return 0;
+  const double dx = w->dx, dy = w->dy;
+  double px = (P2(i + 1, j) - P2(i - 1, j)) / (2. * dx);
+  double py = (P2(i, j + 1) - P2(i, j - 1)) / (2. * dy);
+  double px2 = px * px, py2 = py * py;
+  double pxx = (P2(i - 1, j) - 2. * P2(i, j) + P2(i + 1, j)) / (dx * dx);
+  double pyy = (P2(i, j - 1) - 2. * P2(i, j) + P2(i, j + 1)) / (dy * dy);
+  double pxy = (P2(i - 1, j - 1) + P2(i + 1, j + 1) - P2(i - 1, j + 1) - P2(i + 1, j - 1)) / (4. * dx * dy);
+  double k = (pxx * py2 - 2. * px * py * pxy + pyy * px2) / sqrt((px2 + py2) * (px2 + py2) * (px2 + py2));
+  if (k != k) k = 0;
+  return k;
+}
+
+/* GridFunction2D_CurvLSS (LevelSetMethod/Curvature.c:3-186).  The reference's
+ * generated sums are the least-squares fit of c0 + c1 a + c2 b + c3 a^2 + c4 ab
+ * + c5 b^2 over offsets a, b in [-4, 4]; it uses pxx = c3 and pyy = c5, which
+ * are half the second derivatives.  Over the symmetric 9x9 design the normal
+ * equations decouple:
+ *   c1 = sum a p / 540, c2 = sum b p / 540, c4 = sum a b p / 3600,
+ *   c3 = sum (a^2 - 20/3) p / 2772, c5 = sum (b^2 - 20/3) p / 2772.
+ * No NaN guard in the reference. */
+double orc_curv2d_lss_at(const double *phi, const orc_win *w, int i, int j) {
+  if (!(interior(i, w->px, w->nx, 4) && interior(j, w->py, w->ny, 4))) return 0;
+  const double dx = w->dx, dy = w->dy;
+  double s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
+  for (int b = -4; b <= 4; ++b)
+    for (int a = -4; a <= 4; ++a) {
+      double p = P2(i + a, j + b);
+      s1 += a * p;
+      s2 += b * p;
+      s3 += (a * a - 20.0 / 3.0) * p;
+      s4 += (double)(a * b) * p;
+      s5 += (b * b - 20.0 / 3.0) * p;
+    }
+  double px = s1 / 540.0 / dx;
+  double py = s2 / 540.0 / dy;
+  double pxx = s3 / 2772.0 / (dx * dx);
+  double pyy = s5 / 2772.0 / (dy * dy);
+  double pxy = s4 / 3600.0 / (dx * dy);
+  double px2 = px * px, py2 = py * py;
+  return (pxx * py2 - 2. * px * py * pxy + pyy * px2) / sqrt((px2 + py2) * (px2 + py2) * (px2 + py2));
+}
+
+static double curv_at(const double *phi, const orc_win *w, int stencil, int x, int y, int z) {
+  if (is2d(w)) return stencil == ORC_CURV_LSS ? orc_curv2d_lss_at(phi, w, x, y) : orc_curv2d_at(phi, w, x, y);
+  return orc_curv3d_at(phi, w, x, y, z);
+}
+
+int orc_curvature(const double *phi, double *kappa, const orc_icoor *band,
+                  int64_t band_len, double *band_out, const orc_win *w, int stencil) {
+  if (band) {
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+    for (int64_t b = 0; b < band_len; ++b) {
+      double v = curv_at(phi, w, stencil, band[b].x, band[b].y, band[b].z);
+      if (band_out) band_out[b] = v;
+      else kappa[sidx(w, band[b].x, band[b].y, band[b].z)] = v;
+    }
+    return 0;
+  }
+  const int nz = is2d(w) ? 1 : w->nz;
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+  for (int kz = 0; kz < nz; ++kz)
+    for (int y = w->py; y < w->py + w->ny; ++y)
+      for (int x = w->px; x < w->px + w->nx; ++x)
+        kappa[sidx(w, x, y, w->pz + kz)] = curv_at(phi, w, stencil, x, y, w->pz + kz);
+  return 0;
+}
+
+/* ---------------------------------------------------------------------------
+ * Particle level set
+ * ------------------------------------------------------------------------- */
+/* ParticleLS_AdvectParticles3D / 2D (ParticleLS3D.c:6-36, ParticleLS2D.c:6-36) */
+int orc_pls_advect(double *P, int64_t count, const double *vel, const orc_win *vw,
+                   int vdof, int udof, const double dh[3], double dt, int dim) {
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+  for (int64_t m = 0; m < count; ++m) {
+    double *p = P + 4 * m;
+    double V1[3], V2[3], X[3];
+    if (dim == 3) {
+      orc_interp_vel3d(vel, vw, vdof, udof, p, V1);
+      for (int a = 0; a < 3; a++) X[a] = p[a] + dt / 2 * V1[a] / dh[a];
+      orc_interp_vel3d(vel, vw, vdof, udof, X, V2);
+      for (int a = 0; a < 3; a++) p[a] += dt * V2[a] / dh[a];
+    } else {
+      orc_interp_vel2d(vel, vw, vdof, udof, p, V1);
+      for (int a = 0; a < 2; a++) X[a] = p[a] + dt / 2 * V1[a] / dh[a];
+      orc_interp_vel2d(vel, vw, vdof, udof, X, V2);
+      for (int a = 0; a < 2; a++) p[a] += dt * V2[a] / dh[a];
+    }
+  }
+  return 0;
+}
+
+static inline int in_win(const orc_win *w, int x, int y, int z, int dim) {
+  if (x < w->px || x >= w->px + w->nx || y < w->py || y >= w->py + w->ny) return 0;
+  if (dim == 3 && (z < w->pz || z >= w->pz + w->nz)) return 0;
+  return 1;
+}
+
+/* ParticleLS_ErrorCorrection3D / 2D (ParticleLS3D.c:40-116, ParticleLS2D.c:40-115).
+ * Serial over particles, like the reference (the max/min scatter is order-free). */
+int orc_pls_correct(const double *P, int64_t count, double *phi,
+                    const orc_icoor *band, int64_t band_len, const orc_win *w,
+                    int flags, int dim) {
+  int64_t size = (int64_t)w->nx * w->ny * (dim == 3 ? w->nz : 1);
+  double *pos = malloc(sizeof(double) * size), *neg = malloc(sizeof(double) * size);
+  if (!pos || !neg) { free(pos); free(neg); return 55; }
+  memcpy(pos, phi, sizeof(double) * size);
+  memcpy(neg, phi, sizeof(double) * size);
+  for (int64_t m = 0; m < count; ++m) {
+    const double *p = P + 4 * m;
+    double dist = dim == 3 ? orc_gridinterp3d(phi, w, p) : orc_gridinterp2d(phi, w, p);
+    double sign = psign(p[3]);
+    if (sign * dist < 0 && fabs(dist) > fabs(p[3])) {
+      int Ax = (int)floor(p[0]);
+      int Ay = (int)floor(p[1]);
+      int Az = dim == 3 ? (int)floor((flags & ORC_PLS_REF_AZ) ? p[1] : p[2]) : w->pz;
+      for (int k = 0; k < (dim == 3 ? 2 : 1); ++k)
+        for (int j = 0; j < 2; ++j)
+          for (int i = 0; i < 2; ++i) {
+            int ax = Ax + i, ay = Ay + j, az = Az + k;
+            if (!in_win(w, ax, ay, az, dim)) continue;
+            double r2 = (p[0] - ax) * (p[0] - ax) + (p[1] - ay) * (p[1] - ay);
+            if (dim == 3) r2 += (p[2] - az) * (p[2] - az);
+            double phi_p = sign * (sign * p[3] - sqrt(r2));
+            int64_t id = sidx(w, ax, ay, az);
+            if (sign > 0) pos[id] = pmax(phi_p, pos[id]);
+            else neg[id] = pmin(phi_p, neg[id]);
+          }
+    }
+  }
+  if (band) {
+    for (int64_t b = 0; b < band_len; ++b) {
+      int64_t id = sidx(w, band[b].x, band[b].y, band[b].z);
+      phi[id] = fabs(pos[id]) <= fabs(neg[id]) ? pos[id] : neg[id];
+    }
+  } else {
+    for (int64_t id = 0; id < size; ++id) phi[id] = fabs(pos[id]) <= fabs(neg[id]) ? pos[id] : neg[id];
+  }
+  free(pos);
+  free(neg);
+  return 0;
+}
+
+/* ParticleLS_AdjustRadii (ParticleLS.c:158-185) */
+int orc_pls_adjust_radii(double *P, int64_t count, const double *phi, const orc_win *w,
+                         double rmin, double rmax, int dim) {
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+  for (int64_t m = 0; m < count; ++m) {
+    double *p = P + 4 * m;
+    double dist = dim == 3 ? orc_gridinterp3d(phi, w, p) : orc_gridinterp2d(phi, w, p);
+    double sign = psign(p[3]);
+    double sphi = sign * dist;
+    if (rmin < sphi && sphi < rmax) p[3] = sign * sphi;
+    else if (sphi > rmax) p[3] = sign * rmax;
+    else if (sphi < rmin) p[3] = sign * rmin;
+  }
+  return 0;
+}
+
+/* ---------------------------------------------------------------------------
+ * Reductions
+ * ------------------------------------------------------------------------- */
+/* LevelSet_MaxVelocity (LevelSetMethod/Advect.c:70-111); nodes are read with a
+ * byte stride so IrregularNode arrays can be passed as they are. */
+double orc_max_velocity(const double *vel, const orc_win *vw, int vdof, int udof,
+                        const orc_icoor *nodes, int64_t count, int64_t stride, int dim) {
+  double maxVel = 0;
+  for (int64_t i = 0; i < count; ++i) {
+    const orc_icoor *n = (const orc_icoor *)((const char *)nodes + i * stride);
+    double X[3] = {n->x, n->y, n->z}, V[3];
+    double mag;
+    if (dim == 3) {
+      orc_interp_vel3d(vel, vw, vdof, udof, X, V);
+      mag = sqrt(V[0] * V[0] + V[1] * V[1] + V[2] * V[2]);
+    } else {
+      orc_interp_vel2d(vel, vw, vdof, udof, X, V);
+      mag = sqrt(V[0] * V[0] + V[1] * V[1]);
+    }
+    maxVel = mag > maxVel ? mag : maxVel;
+  }
+  return maxVel;
+}
+
+/* band of nodes with |phi| < width in raster order (z, y, x); returns the count
+ * (which may exceed capacity: only capacity entries are written) */
+int64_t orc_band_compact(const double *phi, const orc_win *w, double width,
+                         orc_icoor *out, int64_t capacity) {
+  int64_t c = 0;
+  const int nz = is2d(w) ? 1 : w->nz;
+  for (int kz = 0; kz < nz; ++kz)
+    for (int y = w->py; y < w->py + w->ny; ++y)
+      for (int x = w->px; x < w->px + w->nx; ++x)
+        if (fabs(phi[sidx(w, x, y, w->pz + kz)]) < width) {
+          if (c < capacity) { out[c].x = x; out[c].y = y; out[c].z = w->pz + kz; }
+          c++;
+        }
+  return c;
+}
+
+/* Interface area and delta-weighted curvature with LevelSetDiracDelta3D / 2D
+ * (LevelSetMethod/NormalDirection.c:36-57) evaluated at nodes, where the
+ * Bilinear samples reduce to node values.  out = {area, sum delta*kappa*dV,
+ * mean kappa, number of nodes with |phi| <= eps}. */
+void orc_interface_metrics(const double *phi, const double *kappa, const orc_win *w,
+                           double eps, double out[4]) {
+  double area = 0, kint = 0, cnt = 0;
+  const int two = is2d(w);
+  const int nz = two ? 1 : w->nz;
+  const double dV = two ? w->dx * w->dy : w->dx * w->dy * w->dz;
+  for (int kz = 0; kz < nz; ++kz) {
+    int z = w->pz + kz;
+    if (!two && !interior(z, w->pz, w->nz, 1)) continue;
+    for (int y = w->py + 1; y < w->py + w->ny - 1; ++y)
+      for (int x = w->px + 1; x < w->px + w->nx - 1; ++x) {
+        int64_t id = sidx(w, x, y, z);
+        double ph = phi[id];
+        if (ph < -eps || ph > eps) continue;
+        double px = (phi[sidx(w, x + 1, y, z)] - phi[sidx(w, x - 1, y, z)]) / (2.0 * w->dx);
+        double py = (phi[sidx(w, x, y + 1, z)] - phi[sidx(w, x, y - 1, z)]) / (2.0 * w->dy);
+        double g2 = px * px + py * py;
+        if (!two) {
+          double pz = (phi[sidx(w, x, y, z + 1)] - phi[sidx(w, x, y, z - 1)]) / (2.0 * w->dz);
+          g2 += pz * pz;
+        }
+        double delta = sqrt(g2) * (1 + cos(ORC_PI * ph / eps)) / (2 * eps);
+        area += delta * dV;
+        if (kappa) kint += delta * kappa[id] * dV;
+        cnt += 1;
+      }
+  }
+  out[0] = area;
+  out[1] = kint;
+  out[2] = area > 0 ? kint / area : 0;
+  out[3] = cnt;
+}
+
+void orc_error_norms(const double *a, const double *b, const uint8_t *mask, int64_t n,
+                     double out[4]) {
+  double mx = 0, ss = 0, mb = 0, cnt = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    if (mask && !mask[i]) continue;
+    double d = fabs(a[i] - b[i]);
+    mx = d > mx ? d : mx;
+    ss += d * d;
+    double ab = fabs(b[i]);
+    mb = ab > mb ? ab : mb;
+    cnt += 1;
+  }
+  out[0] = mx; out[1] = ss; out[2] = mb; out[3] = cnt;
+}
+
+/* ---------------------------------------------------------------------------
+ * Adjoint of the dense 3D SL step (extension).  Reverse-mode derivative of
+ * orc_sl_advect (dense, 3D) written out by hand; pinned by central finite
+ * differences in tests/test_oracle.py.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  int xs[3], ys[3], zs[3];  /* per component base corner */
+  double s[3][3];           /* per component fractions (x, y, z) */
+  int clamped[3];           /* per axis: sample point was clamped */
+} mac_info;
+
+static void mac_eval(const double *vel, const orc_win *vw, int vdof, int udof,
+                     const double Xin[3], double V[3], double dV[3][3], mac_info *mi) {
+  double X[3];
+  const int lo[3] = {vw->px, vw->py, vw->pz}, nn[3] = {vw->nx, vw->ny, vw->nz};
+  for (int a = 0; a < 3; a++) {
+    double hi = lo[a] + nn[a] - 2;
+    X[a] = clampd(Xin[a], lo[a], hi);
+    mi->clamped[a] = (Xin[a] < lo[a] || Xin[a] > hi);
+  }
+  for (int c = 0; c < 3; c++) {
+    int xs = (int)floor(X[0] + T1[c][0]);
+    int ys = (int)floor(X[1] + T1[c][1]);
+    int zs = (int)floor(X[2] + T1[c][2]);
+    double sx = X[0] - xs + T1[c][0], sy = X[1] - ys + T1[c][1], sz = X[2] - zs + T1[c][2];
+    mi->xs[c] = xs; mi->ys[c] = ys; mi->zs[c] = zs;
+    mi->s[c][0] = sx; mi->s[c][1] = sy; mi->s[c][2] = sz;
+    double v = 0, gx = 0, gy = 0, gz = 0;
+    for (int k = 0; k < 2; ++k)
+      for (int j = 0; j < 2; ++j)
+        for (int i = 0; i < 2; ++i) {
+          double f = vel_at(vel, vw, vdof, xs + i, ys + j, zs + k, c + udof);
+          double wx = i ? sx : 1 - sx, wy = j ? sy : 1 - sy, wz = k ? sz : 1 - sz;
+          double dx = i ? 1 : -1, dy = j ? 1 : -1, dz = k ? 1 : -1;
+          v += wx * wy * wz * f;
+          gx += dx * wy * wz * f;
+          gy += wx * dy * wz * f;
+          gz += wx * wy * dz * f;
+        }
+    V[c] = v;
+    if (dV) {
+      dV[c][0] = mi->clamped[0] ? 0 : gx;
+      dV[c][1] = mi->clamped[1] ? 0 : gy;
+      dV[c][2] = mi->clamped[2] ? 0 : gz;
+    }
+  }
+}
+
+static void mac_scatter(double *gvel, const orc_win *vw, int vdof, int udof,
+                        const mac_info *mi, const double H[3]) {
+  for (int c = 0; c < 3; c++) {
+    if (H[c] == 0) continue;
+    double sx = mi->s[c][0], sy = mi->s[c][1], sz = mi->s[c][2];
+    for (int k = 0; k < 2; ++k)
+      for (int j = 0; j < 2; ++j)
+        for (int i = 0; i < 2; ++i) {
+          double wx = i ? sx : 1 - sx, wy = j ? sy : 1 - sy, wz = k ? sz : 1 - sz;
+          gvel[sidx(vw, mi->xs[c] + i, mi->ys[c] + j, mi->zs[c] + k) * vdof + c + udof] += H[c] * wx * wy * wz;
+        }
+  }
+}
+
+int orc_sl_adjoint(const double *phi0, const double *vel, const double *lam_next,
+                   double *lam, double *gvel, const orc_win *w, const orc_win *vw,
+                   int vdof, int udof, double dt, int interp, int backtrace) {
+  if (is2d(w)) return 56;
+  const double dh[3] = {w->dx, w->dy, w->dz};
+  const int P[3] = {w->px, w->py, w->pz}, N[3] = {w->nx, w->ny, w->nz};
+  for (int z = w->pz; z < w->pz + w->nz; ++z)
+    for (int y = w->py; y < w->py + w->ny; ++y)
+      for (int x = w->px; x < w->px + w->nx; ++x) {
+        int64_t id = sidx(w, x, y, z);
+        double g = lam_next[id];
+        int in = interior(x, w->px, w->nx, 2) && interior(y, w->py, w->ny, 2) && interior(z, w->pz, w->nz, 2);
+        if (!in) { lam[id] += g; continue; }
+        if (g == 0) continue;
+        double X[3] = {x, y, z}, V1[3], V2[3], Xm[3], S[3], dV2[3][3];
+        mac_info m1, m2;
+        mac_eval(vel, vw, vdof, udof, X, V1, NULL, &m1);
+        const double *V = V1;
+        if (backtrace == ORC_RK2) {
+          for (int a = 0; a < 3; a++) Xm[a] = X[a] - dt / 2 * V1[a] / dh[a];
+          mac_eval(vel, vw, vdof, udof, Xm, V2, dV2, &m2);
+          for (int a = 0; a < 3; a++) S[a] = X[a] - dt * V2[a] / dh[a];
+          V = V2;
+        } else {
+          for (int a = 0; a < 3; a++) S[a] = X[a] - V1[a] * dt / dh[a];
+        }
+        (void)V;
+        /* split and weights */
+        int b[3], cl[3];
+        double t[3], wt[3][4], dwt[3][4];
+        int np = interp == ORC_CUBIC ? 4 : 2, off = interp == ORC_CUBIC ? -1 : 0;
+        for (int a = 0; a < 3; a++) {
+          double lo = interp == ORC_CUBIC ? P[a] + 1 : P[a];
+          double hi = interp == ORC_CUBIC ? P[a] + N[a] - 2 : P[a] + N[a] - 1;
+          cl[a] = (S[a] < lo || S[a] > hi);
+          b[a] = interp == ORC_CUBIC ? cub_split(S[a], P[a], N[a], &t[a]) : lin_split(S[a], P[a], N[a], &t[a]);
+          if (interp == ORC_CUBIC) {
+            orc_cr_weights(t[a], wt[a]);
+            orc_cr_dweights(t[a], dwt[a]);
+          } else {
+            wt[a][0] = 1 - t[a]; wt[a][1] = t[a];
+            dwt[a][0] = -1; dwt[a][1] = 1;
+          }
+        }
+        double G[3] = {0, 0, 0};
+        for (int k = 0; k < np; ++k)
+          for (int j = 0; j < np; ++j)
+            for (int i = 0; i < np; ++i) {
+              int64_t sid = sidx(w, b[0] + off + i, b[1] + off + j, b[2] + off + k);
+              lam[sid] += g * wt[0][i] * wt[1][j] * wt[2][k];
+              double f = phi0[sid];
+              G[0] += dwt[0][i] * wt[1][j] * wt[2][k] * f;
+              G[1] += wt[0][i] * dwt[1][j] * wt[2][k] * f;
+              G[2] += wt[0][i] * wt[1][j] * dwt[2][k] * f;
+            }
+        double H[3];
+        for (int a = 0; a < 3; a++) H[a] = cl[a] ? 0 : -(dt / dh[a]) * g * G[a];
+        double H1[3] = {0, 0, 0};
+        if (backtrace == ORC_RK2) {
+          mac_scatter(gvel, vw, vdof, udof, &m2, H);
+          for (int bb = 0; bb < 3; bb++) {
+            double K = 0;
+            for (int a = 0; a < 3; a++) K += H[a] * dV2[a][bb];
+            H1[bb] = -(dt / 2 / dh[bb]) * K;
+          }
+        } else {
+          for (int a = 0; a < 3; a++) H1[a] = H[a];
+        }
+        mac_scatter(gvel, vw, vdof, udof, &m1, H1);
+      }
+  return 0;
+}

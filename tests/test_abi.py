@@ -1,0 +1,49 @@
+"""The C ABI library loads and exports every symbol include/dcell_lsm.h declares;
+argument checks answer without touching a GPU."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from dcell_amd import _lib as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols(path):
+    text = open(path).read()
+    return sorted(set(re.findall(r"\b(dcl_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = L.lib()
+    declared = header_symbols(os.path.join(ROOT, "include", "dcell_lsm.h"))
+    assert declared, "no declarations parsed"
+    missing = [s for s in declared if not hasattr(lib, s)]
+    assert not missing, missing
+    assert sorted(L.EXPORTS) == declared
+
+
+def test_abi_version_and_argument_errors():
+    lib = L.lib()
+    assert lib.dcl_abi_version() == 1
+    rc = lib.dcl_sl_advect(None, None, None, None, None, 0, None, 0.1, 0, 0, 0, None)
+    assert rc == 85  # DCL_ERR_ARG_NULL
+    assert b"phi0" in lib.dcl_last_error()
+    w = L.Window(L.ICoor(0, 0, 0), L.ICoor(3, 3, 3), L.Coor(1, 1, 1))
+    dummy = C.c_void_p(16)
+    V = L.Velocity(dummy, w, 3, 0)
+    rc = lib.dcl_sl_advect(dummy, C.c_void_p(32), C.byref(w), C.byref(V), None, 0, None, 0.1, 0, 0, 0, None)
+    assert rc == 60  # dense windows need 5 nodes per axis
+    rc = lib.dcl_sl_advect(dummy, C.c_void_p(32), C.byref(w), C.byref(V), None, 0, None, 0.1, 7, 0, 0, None)
+    assert rc == 62  # unknown precision
+    rc = lib.dcl_curvature(dummy, C.c_void_p(32), C.byref(w), None, 0, None, 0, 1, None)
+    assert rc == 56  # LSS stencil is 2D only
+
+
+def test_missing_library_fails_loudly(monkeypatch):
+    monkeypatch.setattr(L, "_lib", None)
+    monkeypatch.setattr(L, "LIB_PATH", "/nonexistent/libdcell_lsm.so")
+    with pytest.raises(RuntimeError, match="no CPU fallback"):
+        L.lib()

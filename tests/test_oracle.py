@@ -1,0 +1,476 @@
+"""Pins of the CPU oracle (oracle/lsm_oracle.c) before it is trusted as the checker.
+
+Each check is chosen so that a plausible mistake in the restatement (a wrong
+half-cell shift, a transposed axis, a dropped term, a sign) fails it: closed
+forms the interpolants must reproduce, exact shifts, the reference's own
+tricubic fragments compiled from the reference tree, independent numpy
+derivations (least squares, brute-force trilinear), convergence to analytic
+curvature and area, hand-built particle cases and finite differences.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from oracle import Win
+from dcell_amd import configs
+
+
+def win3(n, p=(0, 0, 0), d=(1.0, 1.0, 1.0)):
+    return Win(p, (n, n, n) if np.isscalar(n) else n, d)
+
+
+def win2(n, p=(0, 0, 0), d=(1.0, 1.0, 1.0)):
+    return Win(p, (n, n, 1), d)
+
+
+# ----------------------------------------------------------------- cubic pins
+def test_tricubic_matches_reference_header():
+    """The reference's TricubicInverse.h + TricubicInterpolation.h, compiled from
+    the reference tree, against the separable Catmull-Rom restatement."""
+    if O.ref_tricubic() is None:
+        pytest.skip("reference tree was not available when oracle/_ref was built")
+    rng = np.random.default_rng(11)
+    worst = 0.0
+    for _ in range(300):
+        st = rng.normal(size=64) * rng.uniform(0.1, 10)
+        x, y, z = rng.random(3)
+        a = O.cr_stencil3(st, x, y, z)
+        b = O.ref_tricubic_eval(st, x, y, z)
+        worst = max(worst, np.max(np.abs(a - b)) / max(1.0, np.max(np.abs(st))))
+    assert worst < 1e-12, worst
+
+
+def test_cr_weights_closed_form():
+    for t in np.linspace(0, 1, 17):
+        w = O.cr_weights(t)
+        assert abs(w.sum() - 1) < 1e-15                      # partition of unity
+        xs = np.array([-1.0, 0.0, 1.0, 2.0])
+        assert abs((w * xs).sum() - t) < 1e-15               # reproduces linears
+        assert abs((w * xs ** 2).sum() - t * t) < 1e-14      # and quadratics
+        dw = O.cr_dweights(t)
+        e = 1e-6
+        num = (O.cr_weights(t + e) - O.cr_weights(t - e)) / (2 * e)
+        assert np.allclose(dw, num, atol=1e-8)
+    assert np.allclose(O.cr_weights(0.0), [0, 1, 0, 0])
+    assert np.allclose(O.cr_weights(1.0), [0, 0, 1, 0])
+    # not cubic-exact (Keys a = -1/2): documented behaviour of the reference interpolant
+    w = O.cr_weights(0.3)
+    assert abs((w * np.array([-1.0, 0, 1, 8])).sum() - 0.027) > 1e-2
+
+
+def test_tricubic_quadratic_exact_and_bicubic_reduction():
+    n = 9
+    w3 = win3(n, p=(-3, 2, 5))
+    k, j, i = np.meshgrid(np.arange(n) + 5, np.arange(n) + 2, np.arange(n) - 3, indexing="ij")
+    f = lambda x, y, z: 0.3 * x * x - 0.2 * x * y + 0.1 * z * z + 0.7 * y - 0.4 * z + 1.0  # noqa
+    phi = f(i, j, k).astype(float)
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        S = np.array([-3, 2, 5]) + 1 + rng.random(3) * (n - 3)
+        assert abs(O.cubic(phi, w3, S) - f(*S)) < 1e-12
+    # a field constant in z: the tricubic equals the bicubic of one slice
+    g = rng.normal(size=(n, n))
+    phi3 = np.broadcast_to(g, (n, n, n)).copy()
+    w2 = win2(n, p=(-3, 2, 0))
+    for _ in range(20):
+        S = np.array([-3, 2, 5]) + 1 + rng.random(3) * (n - 3)
+        assert abs(O.cubic(phi3, w3, S) - O.cubic(g, w2, S[:2])) < 1e-13
+
+
+# ----------------------------------------------------------- linear interpolants
+def test_bilinear3d_trilinear_exact_and_nodes():
+    n = 6
+    w = win3(n, p=(4, -2, 1))
+    k, j, i = np.meshgrid(np.arange(n) + 1, np.arange(n) - 2, np.arange(n) + 4, indexing="ij")
+    f = lambda x, y, z: 1.5 + 0.25 * x - 0.5 * y + 2 * z + 0.1 * x * y * z - 0.3 * y * z  # noqa
+    phi = f(i, j, k).astype(float)
+    rng = np.random.default_rng(4)
+    for _ in range(50):
+        S = np.array([4, -2, 1]) + rng.random(3) * (n - 1)
+        assert abs(O.bilinear(phi, w, S) - f(*S)) < 1e-12
+        assert abs(O.gridinterp(phi, w, S) - f(*S)) < 1e-12
+    assert O.bilinear(phi, w, np.array([6.0, 0.0, 3.0])) == phi[2, 2, 2]
+
+
+def test_bilinear2d_two_x_minus_one_form():
+    n = 7
+    w = win2(n, p=(-1, 3, 0))
+    rng = np.random.default_rng(5)
+    phi = rng.normal(size=(n, n))
+    for _ in range(40):
+        S = np.array([-1, 3]) + rng.random(2) * (n - 1)
+        i, j = int(np.floor(S[0])) + 1, int(np.floor(S[1])) - 3
+        i, j = min(i, n - 2), min(j, n - 2)
+        fx, fy = S[0] + 1 - i, S[1] - 3 - j
+        ref = ((1 - fx) * (1 - fy) * phi[j, i] + fx * (1 - fy) * phi[j, i + 1]
+               + (1 - fx) * fy * phi[j + 1, i] + fx * fy * phi[j + 1, i + 1])
+        assert abs(O.bilinear(phi, w, S) - ref) < 1e-14
+        assert abs(O.gridinterp(phi, w, S) - ref) < 1e-14
+
+
+def test_mac_velocity_half_cell_shift():
+    """u_i sits at x = i - 1/2: a velocity linear in its own axis is recovered
+    exactly at any point (a wrong shift is off by b/2)."""
+    n = 8
+    w = win3(n, p=(2, 0, -1))
+    k, j, i = np.meshgrid(np.arange(n) - 1, np.arange(n), np.arange(n) + 2, indexing="ij")
+    vel = np.empty((n, n, n, 3))
+    vel[..., 0] = 0.7 + 1.3 * (i - 0.5) + 0.2 * j - 0.1 * k
+    vel[..., 1] = -0.4 + 0.5 * i - 0.9 * (j - 0.5) + 0.3 * k
+    vel[..., 2] = 0.1 - 0.2 * i + 0.6 * j + 1.1 * (k - 0.5)
+    rng = np.random.default_rng(6)
+    for _ in range(40):
+        X = np.array([2, 0, -1]) + rng.random(3) * (n - 2)
+        x, y, z = X
+        V = O.interp_vel(vel, w, X)
+        assert np.allclose(V, [0.7 + 1.3 * x + 0.2 * y - 0.1 * z,
+                               -0.4 + 0.5 * x - 0.9 * y + 0.3 * z,
+                               0.1 - 0.2 * x + 0.6 * y + 1.1 * z], atol=1e-13)
+    # udof offset: the fluid's [p, u, v, w] layout
+    vel4 = np.concatenate([np.full(vel.shape[:-1] + (1,), 99.0), vel], axis=-1)
+    X = np.array([4.3, 3.1, 2.2])
+    assert np.allclose(O.interp_vel(vel4, w, X, udof=1), O.interp_vel(vel, w, X), atol=0)
+
+
+def test_mac_velocity_2d():
+    n = 6
+    w = win2(n)
+    J, I = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    vel = np.empty((n, n, 2))
+    vel[..., 0] = 2.0 + 0.5 * (I - 0.5) - 0.25 * J
+    vel[..., 1] = -1.0 + 0.75 * I + 0.125 * (J - 0.5)
+    X = np.array([2.3, 1.7])
+    V = O.interp_vel(vel, w, X)
+    assert np.allclose(V, [2.0 + 0.5 * 2.3 - 0.25 * 1.7, -1.0 + 0.75 * 2.3 + 0.125 * 1.7], atol=1e-14)
+
+
+# -------------------------------------------------------------------- SL step
+def _const_vel(n, V, dim=3):
+    shape = (n, n, n, 3) if dim == 3 else (n, n, 2)
+    vel = np.empty(shape)
+    vel[...] = V
+    return vel
+
+
+@pytest.mark.parametrize("interp", [O.LINEAR, O.CUBIC])
+@pytest.mark.parametrize("bt", [O.EULER, O.RK2])
+def test_sl_zero_velocity_is_identity(interp, bt):
+    n = 10
+    w = win3(n, d=(0.1, 0.1, 0.1))
+    phi0 = np.random.default_rng(7).normal(size=(n, n, n))
+    out = O.sl_advect(phi0, _const_vel(n, 0.0), w, 0.05, interp, bt)
+    assert np.array_equal(out, phi0)
+
+
+@pytest.mark.parametrize("interp", [O.LINEAR, O.CUBIC])
+def test_sl_integer_shift_exact(interp):
+    n = 12
+    h = 0.1
+    w = win3(n, d=(h, h, h))
+    phi0 = np.random.default_rng(8).normal(size=(n, n, n))
+    V = np.array([1.0, -1.0, 0.0])      # dt V / h = (1, -1, 0) cells
+    out = O.sl_advect(phi0, _const_vel(n, V), w, h, interp, O.EULER)
+    inner = (slice(3, n - 3),) * 3
+    shifted = np.roll(np.roll(phi0, 1, axis=2), -1, axis=1)
+    assert np.array_equal(out[inner], shifted[inner])
+    # RK2 equals Euler for a constant field
+    out2 = O.sl_advect(phi0, _const_vel(n, V), w, h, interp, O.RK2)
+    assert np.array_equal(out2[inner], out[inner])
+    # outer two layers carried over from phi0
+    assert np.array_equal(out[:2], phi0[:2]) and np.array_equal(out[:, :, -2:], phi0[:, :, -2:])
+
+
+def test_sl_affine_exact_linear_and_quadratic_exact_cubic():
+    n = 14
+    h = 0.05
+    w = win3(n, p=(3, -4, 7), d=(h, h, h))
+    k, j, i = np.meshgrid(np.arange(n) + 7, np.arange(n) - 4, np.arange(n) + 3, indexing="ij")
+    V = np.array([0.37, -0.21, 0.55])
+    dt = 0.08
+    S = lambda a, ax: a - V[ax] * dt / h  # noqa
+    aff = lambda x, y, z: 0.3 * x - 0.7 * y + 0.2 * z + 1  # noqa
+    quad = lambda x, y, z: 0.03 * x * x - 0.02 * x * y + 0.05 * z * z + 0.3 * y  # noqa
+    inner = (slice(2, n - 2),) * 3
+    out = O.sl_advect(aff(i, j, k).astype(float), _const_vel(n, V), w, dt, O.LINEAR, O.EULER)
+    assert np.allclose(out[inner], aff(S(i, 0), S(j, 1), S(k, 2))[inner], atol=1e-12)
+    out = O.sl_advect(quad(i, j, k).astype(float), _const_vel(n, V), w, dt, O.CUBIC, O.RK2)
+    assert np.allclose(out[inner], quad(S(i, 0), S(j, 1), S(k, 2))[inner], atol=1e-11)
+
+
+def test_sl_rk2_linear_shear_closed_form():
+    """u = a*y (x-velocity linear in y, exactly representable): RK2 S_x =
+    x - dt a (y - dt/2 v) / h with v = 0 -> same as Euler; with v = b*x too the
+    midpoint changes S in closed form."""
+    n = 16
+    h = 0.1
+    w = win3(n, d=(h, h, h))
+    k, j, i = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    a, b, dt = 0.3, -0.2, 0.05
+    vel = np.zeros((n, n, n, 3))
+    vel[..., 0] = a * j * h              # u(x, y) = a y (physical y = j h)
+    vel[..., 1] = b * (i * h)            # v(x) = b x, evaluated at the v face (x = i h)
+    phi0 = (0.4 * i - 0.3 * j + 0.2 * k).astype(float)
+    out = O.sl_advect(phi0, vel, w, dt, O.LINEAR, O.RK2)
+    X, Y = i * h, j * h
+    Xm, Ym = X - dt / 2 * a * Y, Y - dt / 2 * b * X
+    Sx, Sy = (X - dt * a * Ym) / h, (Y - dt * b * Xm) / h
+    ref = 0.4 * Sx - 0.3 * Sy + 0.2 * k
+    inner = (slice(3, n - 3),) * 3
+    assert np.allclose(out[inner], ref[inner], atol=1e-11)
+
+
+def test_sl_band_mode_matches_dense_and_is_order_free():
+    n = 12
+    h = 1.0 / (n - 1)
+    w = win3(n, p=(5, 6, 7), d=(h, h, h))
+    rng = np.random.default_rng(9)
+    phi0 = rng.normal(size=(n, n, n))
+    vel = rng.normal(size=(n, n, n, 3)) * 0.3
+    dense = O.sl_advect(phi0, vel, w, 0.7 * h, O.CUBIC, O.RK2)
+    zz, yy, xx = np.meshgrid(*(np.arange(2, n - 2),) * 3, indexing="ij")
+    band = np.stack([xx.ravel() + 5, yy.ravel() + 6, zz.ravel() + 7], axis=1).astype(np.int32)
+    perm = rng.permutation(len(band))
+    vals = O.sl_advect(phi0, vel, w, 0.7 * h, O.CUBIC, O.RK2, band=band[perm])
+    assert np.array_equal(vals, dense[2:n - 2, 2:n - 2, 2:n - 2].ravel()[perm])
+
+
+def test_sl_2d_rotation_converges():
+    """Quarter revolution of a circle under rigid rotation with the bicubic
+    RK2 step: the error shrinks at least 3x per grid doubling."""
+    errs = []
+    for n in (33, 65):
+        h = 1.0 / (n - 1)
+        w = win2(n, d=(h, h, h))
+        phi0 = configs.circle_sdf(n, (0.5, 0.7), 0.15)
+        vel = configs.rotation_velocity(n)
+        steps = 4 * (n - 1) // 8
+        dt = (math.pi / 2) / (2 * math.pi) / steps
+        phi = phi0
+        for _ in range(steps):
+            phi = O.sl_advect(phi, vel, w, dt, O.CUBIC, O.RK2)
+        exact = configs.circle_sdf(n, (0.3, 0.5), 0.15)
+        band = np.abs(exact) < 2
+        errs.append(np.max(np.abs(phi - exact)[band]) * h)
+    assert errs[1] < errs[0] / 3, errs
+
+
+# ------------------------------------------------------------------ curvature
+def test_curvature3d_quadratic_exact_and_sphere():
+    n = 9
+    d = (0.1, 0.2, 0.15)
+    w = Win((0, 0, 0), (n, n, n), d)
+    k, j, i = np.meshgrid(*(np.arange(n),) * 3, indexing="ij")
+    x, y, z = i * d[0], j * d[1], k * d[2]
+    c = dict(a=0.3, b=-0.7, cc=0.5, xx=0.4, yy=-0.2, zz=0.1, xy=0.3, xz=-0.25, yz=0.15)
+    phi = (c["a"] * x + c["b"] * y + c["cc"] * z + c["xx"] * x * x + c["yy"] * y * y
+           + c["zz"] * z * z + c["xy"] * x * y + c["xz"] * x * z + c["yz"] * y * z)
+    X, Y, Z = 4 * d[0], 3 * d[1], 5 * d[2]
+    px = c["a"] + 2 * c["xx"] * X + c["xy"] * Y + c["xz"] * Z
+    py = c["b"] + 2 * c["yy"] * Y + c["xy"] * X + c["yz"] * Z
+    pz = c["cc"] + 2 * c["zz"] * Z + c["xz"] * X + c["yz"] * Y
+    pxx, pyy, pzz = 2 * c["xx"], 2 * c["yy"], 2 * c["zz"]
+    pxy, pxz, pyz = c["xy"], c["xz"], c["yz"]
+    num = (px * px * (pyy + pzz) + py * py * (pxx + pzz) + pz * pz * (pxx + pyy)
+           - 2 * (px * py * pxy + px * pz * pxz + py * pz * pyz))
+    ref = num / (px * px + py * py + pz * pz) ** 1.5
+    assert abs(O.curv_at(phi, w, 4, 3, 5) - ref) < 1e-12 * max(1, abs(ref))
+    assert O.curv_at(phi, w, 0, 3, 5) == 0.0                        # stencil leaves the window
+    flat = np.ones((n, n, n))
+    assert O.curv_at(flat, w, 4, 4, 4) == 0.0                       # 0/0 -> 0
+    # sphere: kappa = 2/r, second order
+    errs = []
+    for m in (33, 65):
+        h = 1.0 / (m - 1)
+        ws = win3(m, d=(h, h, h))
+        phi = configs.sphere_sdf(m, (0.5, 0.5, 0.5), 0.3)
+        kap = O.curvature(phi, ws)
+        sel = np.abs(phi) < 0.5
+        errs.append(np.max(np.abs(kap[sel] - 2 / (0.3 + phi[sel] * h))))
+    assert errs[1] < errs[0] / 3 and errs[1] < 0.05, errs
+
+
+def test_curvature2d_quadratic_exact_and_circle():
+    n = 7
+    d = (0.1, 0.25, 1.0)
+    w = Win((2, 3, 0), (n, n, 1), d)
+    J, I = np.meshgrid(np.arange(n) + 3, np.arange(n) + 2, indexing="ij")
+    x, y = I * d[0], J * d[1]
+    phi = 0.5 * x - 0.3 * y + 0.7 * x * x + 0.2 * x * y - 0.4 * y * y
+    X, Y = 5 * d[0], 6 * d[1]
+    px, py = 0.5 + 1.4 * X + 0.2 * Y, -0.3 + 0.2 * X - 0.8 * Y
+    ref = (1.4 * py * py - 2 * px * py * 0.2 + (-0.8) * px * px) / (px * px + py * py) ** 1.5
+    assert abs(O.curv_at(phi, w, 5, 6) - ref) < 1e-12 * max(1, abs(ref))
+    m = 129
+    h = 1.0 / (m - 1)
+    phi = configs.circle_sdf(m, (0.5, 0.5), 0.25)
+    kap = O.curvature(phi, win2(m, d=(h, h, h)))
+    sel = np.abs(phi) < 0.5
+    assert np.max(np.abs(kap[sel] - 1 / (0.25 + phi[sel] * h))) < 0.02
+
+
+def test_curvature_lss_is_least_squares_with_half_second_derivative():
+    """An independent numpy least-squares fit over the 9x9 design gives the
+    same coefficients; on a quadratic the stencil returns pxx = c_xx (half the
+    true second derivative), the reference's quirk."""
+    rng = np.random.default_rng(12)
+    n = 13
+    d = (0.3, 0.2, 1.0)
+    w = win2(n, d=d)
+    phi = rng.normal(size=(n, n))
+    a, b = np.meshgrid(np.arange(-4, 5), np.arange(-4, 5), indexing="xy")
+    A = np.stack([np.ones(81), a.ravel(), b.ravel(), a.ravel() ** 2, (a * b).ravel(), b.ravel() ** 2], 1)
+    i, j = 6, 6
+    coef = np.linalg.lstsq(A, phi[j - 4:j + 5, i - 4:i + 5].ravel(), rcond=None)[0]
+    px, py = coef[1] / d[0], coef[2] / d[1]
+    pxx, pyy, pxy = coef[3] / d[0] ** 2, coef[5] / d[1] ** 2, coef[4] / (d[0] * d[1])
+    ref = (pxx * py * py - 2 * px * py * pxy + pyy * px * px) / (px * px + py * py) ** 1.5
+    assert abs(O.curv_at(phi, w, i, j, stencil=O.CURV_LSS) - ref) < 1e-10 * max(1, abs(ref))
+    # quadratic: pxx reported as half the true value
+    J, I = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    x, y = I * d[0], J * d[1]
+    q = 0.4 * x + 0.3 * y + 0.5 * x * x
+    k_lss = O.curv_at(q, w, 6, 6, stencil=O.CURV_LSS)
+    X = 6 * d[0]
+    px, py = 0.4 + 1.0 * X, 0.3
+    assert abs(k_lss - (0.5 * py * py) / (px * px + py * py) ** 1.5) < 1e-10
+    assert abs(O.curv_at(q, w, 6, 6) - (1.0 * py * py) / (px * px + py * py) ** 1.5) < 1e-10
+
+
+# ------------------------------------------------------------------------ PLS
+def test_pls_advect_closed_forms():
+    n = 10
+    h = 0.1
+    w = win3(n, d=(h, h, h))
+    P = np.array([[3.2, 4.7, 5.1, 0.3], [6.5, 2.25, 3.75, -0.2]])
+    V = np.array([0.4, -0.3, 0.2])
+    out = O.pls_advect(P, _const_vel(n, V), w, (h, h, h), 0.05)
+    assert np.allclose(out[:, :3], P[:, :3] + 0.05 * V / h, atol=1e-13)
+    assert np.array_equal(out[:, 3], P[:, 3])
+    # u = a*y (linear shear): midpoint rule in closed form
+    k, j, i = np.meshgrid(*(np.arange(n),) * 3, indexing="ij")
+    vel = np.zeros((n, n, n, 3))
+    vel[..., 0] = 0.5 * j * h
+    vel[..., 1] = -0.3 * i * h
+    dt = 0.1
+    out = O.pls_advect(P, vel, w, (h, h, h), dt)
+    x, y = P[:, 0], P[:, 1]
+    xm = x + dt / 2 * 0.5 * y * h / h
+    ym = y + dt / 2 * (-0.3) * x * h / h
+    assert np.allclose(out[:, 0], x + dt * 0.5 * ym, atol=1e-12)
+    assert np.allclose(out[:, 1], y + dt * (-0.3) * xm, atol=1e-12)
+
+
+def test_pls_correct_hand_built():
+    n = 8
+    w = win3(n)
+    phi = np.full((n, n, n), 2.0)                     # everything outside
+    band = np.argwhere(np.ones((n, n, n)))[:, ::-1].astype(np.int32)
+    # a negative particle at phi = +2 -> escaped by more than its radius
+    P = np.array([[3.25, 4.5, 2.75, -0.4]])
+    out = O.pls_correct(P, phi, w, band)
+    changed = np.argwhere(out != phi)
+    assert len(changed) == 8
+    for (z, y, x) in changed:
+        dist = math.sqrt((3.25 - x) ** 2 + (4.5 - y) ** 2 + (2.75 - z) ** 2)
+        phi_m = -(0.4 - dist)                          # s (s r - |X - a|), s = -1
+        assert x in (3, 4) and y in (4, 5) and z in (2, 3)
+        want = phi_m if abs(2.0) > abs(phi_m) else 2.0  # merge |phi+| <= |phi-|
+        assert out[z, y, x] == pytest.approx(want, abs=1e-15)
+    # not escaped (same side) -> unchanged; too close (|phi| <= |r|) -> unchanged
+    assert np.array_equal(O.pls_correct(np.array([[3.2, 4.5, 2.7, 0.4]]), phi, w, band), phi)
+    phi2 = np.full((n, n, n), 0.3)
+    assert np.array_equal(O.pls_correct(np.array([[3.2, 4.5, 2.7, -0.4]]), phi2, w, band), phi2)
+    # two particles on one corner: the minimum wins for phi-
+    P2 = np.array([[3.1, 4.1, 2.1, -0.5], [3.4, 4.4, 2.4, -0.5]])
+    out = O.pls_correct(P2, phi, w, band)
+    d1 = math.sqrt(3 * 0.1 ** 2)
+    d2 = math.sqrt(3 * 0.4 ** 2)
+    assert out[2, 4, 3] == pytest.approx(min(-(0.5 - d1), -(0.5 - d2)), abs=1e-15)
+    # merge only over the band
+    out = O.pls_correct(P, phi, w, band[:3])
+    assert np.array_equal(out, phi)
+    # the reference's A.z = floor(X.y) variant moves the corners to z = 4, 5
+    out = O.pls_correct(P, phi, w, band, flags=O.PLS_REF_AZ)
+    zs = sorted(set(np.argwhere(out != phi)[:, 0]))
+    assert zs == [4, 5]
+
+
+def test_pls_adjust_radii_branches():
+    n = 6
+    w = win3(n)
+    phi = np.full((n, n, n), 0.3)
+    P = np.array([[2.5, 2.5, 2.5, 1.0], [2.5, 2.5, 2.5, -1.0], [2.5, 2.5, 2.5, 0.0]])
+    out = O.pls_adjust_radii(P, phi, w)
+    assert out[0, 3] == pytest.approx(0.3)          # rmin < s phi < rmax
+    assert out[1, 3] == pytest.approx(-0.1)         # s phi = -0.3 < rmin -> s rmin
+    assert out[2, 3] == 0.0                         # PetscSign(0) = 0
+    out = O.pls_adjust_radii(P, np.full((n, n, n), 0.9), w)
+    assert out[0, 3] == pytest.approx(0.5)
+    # equality with a bound leaves the radius as it was (strict comparisons)
+    out = O.pls_adjust_radii(P, np.full((n, n, n), 0.5), w)
+    assert out[0, 3] == 1.0
+
+
+# ----------------------------------------------------------------- reductions
+def test_max_velocity_and_band_compact():
+    n = 7
+    w = win3(n)
+    rng = np.random.default_rng(13)
+    vel = rng.normal(size=(n, n, n, 3))
+    nodes = np.array([[1, 2, 3], [4, 4, 4], [5, 1, 2]], dtype=np.int32)
+    ref = 0.0
+    for x, y, z in nodes:
+        V = [0.5 * (vel[z, y, x, 0] + vel[z, y, x + 1, 0]), 0.5 * (vel[z, y, x, 1] + vel[z, y + 1, x, 1]),
+             0.5 * (vel[z, y, x, 2] + vel[z + 1, y, x, 2])]
+        ref = max(ref, math.sqrt(sum(v * v for v in V)))
+    assert O.max_velocity(vel, w, nodes) == pytest.approx(ref, rel=1e-15)
+    phi = rng.normal(size=(n, n, n)) * 3
+    b = O.band_compact(phi, Win((1, 2, 3), (n, n, n), (1, 1, 1)), 1.5)
+    ref = np.argwhere(np.abs(phi) < 1.5)[:, ::-1] + np.array([1, 2, 3])
+    assert np.array_equal(b, ref)
+
+
+def test_interface_metrics_sphere_and_circle():
+    m = 97
+    h = 1.0 / (m - 1)
+    phi = configs.sphere_sdf(m, (0.5, 0.5, 0.5), 0.3)
+    w = win3(m, d=(h, h, h))
+    kap = O.curvature(phi, w)
+    area, kint, kmean, cnt = O.interface_metrics(phi, w, kap)
+    assert area == pytest.approx(4 * math.pi * 0.09, rel=2e-3)
+    assert kmean == pytest.approx(2 / 0.3, rel=5e-3)
+    phi2 = configs.circle_sdf(257, (0.5, 0.5), 0.25)
+    h2 = 1.0 / 256
+    length = O.interface_metrics(phi2, win2(257, d=(h2, h2, h2)))[0]
+    assert length == pytest.approx(2 * math.pi * 0.25, rel=1e-3)
+
+
+# -------------------------------------------------------------------- adjoint
+@pytest.mark.parametrize("interp,bt", [(O.CUBIC, O.RK2), (O.LINEAR, O.EULER), (O.CUBIC, O.EULER)])
+def test_adjoint_finite_differences(interp, bt):
+    n = 9
+    h = 1.0 / (n - 1)
+    w = win3(n, d=(h, h, h))
+    rng = np.random.default_rng(14)
+    phi0 = rng.normal(size=(n, n, n))
+    vel = rng.normal(size=(n, n, n, 3)) * 0.5
+    lam_next = rng.normal(size=(n, n, n))
+    dt = 0.9 * h
+    lam, gvel = O.sl_adjoint(phi0, vel, lam_next, w, dt, interp, bt)
+    J = lambda p, v: float((lam_next * O.sl_advect(p, v, w, dt, interp, bt)).sum())  # noqa
+    # phi0 is linear: the adjoint is exact
+    for _ in range(6):
+        idx = tuple(rng.integers(0, n, 3))
+        e = np.zeros_like(phi0)
+        e[idx] = 1.0
+        assert lam[idx] == pytest.approx(J(e, vel) - J(0 * e, vel), abs=1e-12)
+    eps = 1e-6
+    for _ in range(12):
+        idx = tuple(rng.integers(1, n - 2, 3)) + (int(rng.integers(0, 3)),)
+        vp, vm = vel.copy(), vel.copy()
+        vp[idx] += eps
+        vm[idx] -= eps
+        fd = (J(phi0, vp) - J(phi0, vm)) / (2 * eps)
+        assert gvel[idx] == pytest.approx(fd, rel=1e-5, abs=1e-7)
