@@ -1,0 +1,300 @@
+// adjoint.hip -- reverse-mode derivative of the dense 3D SL step on gfx950.
+//
+// Forward (sl_advect.hip, dense): phi[X] = I(phi0; S(X)), S = X - c V(Xm),
+// Xm = X - (c/2) V(X) for RK2 (S = X - c V(X) for Euler), c = dt/dh, with
+// V the MAC-staggered trilinear velocity and I the trilinear or Catmull-Rom
+// gather.  Given lam_next = dL/dphi, one thread per node recomputes its
+// forward state and scatters
+//   dL/dphi0 += g w_ijk                               (transposed gather)
+//   dL/dvel  += MAC weights x (-c g dI/dS)            (at Xm, or at X for Euler)
+//             + node weights x (-(c/2) (dV/dXm)^T ...) (RK2 first stage)
+// where a clamped coordinate contributes no derivative.  The transposed
+// gather of one block lands in a small box of phi0 around the block's nodes,
+// so it is accumulated in LDS first (one atomic per LDS word) and flushed to
+// HBM with one atomic per touched word; stencil entries that fall outside the
+// block's box go to HBM directly.  Outer two layers: phi = phi0 there, so
+// dL/dphi0 += lam_next.
+#include "dcl_host.hpp"
+
+namespace dcl {
+
+struct AdjP {
+  KWin w;
+  KVel v;
+  double c[3];
+  double ch[3];
+};
+
+template <typename T>
+__device__ __forceinline__ void atomic_add(T* p, T v) {
+  unsafeAtomicAdd(p, v);
+}
+
+// MAC velocity at an arbitrary point with its spatial derivative and the
+// data needed to scatter into the velocity: per component c the local base
+// offset of its 2x2x2 corner cube and the three fractions.
+template <typename T>
+struct MacState {
+  T V[3];
+  T dV[3][3];   // dV[c][axis]
+  int base[3];  // element offset of corner (0,0,0) of component c (includes c + udof)
+  T f[3][3];    // fractions [c][axis]
+};
+
+template <typename T>
+__device__ __forceinline__ void mac_state(const T* __restrict__ vel, const KVel& v, const double Xin[3],
+                                          MacState<T>& m, bool clamped[3]) {
+  double X[3];
+  const int lo[3] = {v.w.px, v.w.py, v.w.pz};
+  const int nn[3] = {v.w.nx, v.w.ny, v.w.nz};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double hi = (double)(lo[a] + nn[a] - 2);
+    X[a] = clampd(Xin[a], (double)lo[a], hi);
+    clamped[a] = !(Xin[a] >= (double)lo[a] && Xin[a] <= hi);
+  }
+  const int sx = v.dof, sy = v.w.nx * v.dof, sz = v.w.ny * sy;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    int bi[3];
+    T fr[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double sh = a == c ? 0.5 : 0.0;
+      const double fl = floor(X[a] + sh);
+      bi[a] = (int)fl - lo[a];
+      fr[a] = (T)(X[a] - fl + sh);
+      m.f[c][a] = fr[a];
+    }
+    const int base = bi[2] * sz + bi[1] * sy + bi[0] * sx + v.udof + c;
+    m.base[c] = base;
+    const T* b = vel + base;
+    const T f000 = b[0], f100 = b[sx], f010 = b[sy], f110 = b[sy + sx];
+    const T f001 = b[sz], f101 = b[sz + sx], f011 = b[sz + sy], f111 = b[sz + sy + sx];
+    const T x1 = fr[0], x0 = T(1) - fr[0], y1 = fr[1], y0 = T(1) - fr[1], z1 = fr[2], z0 = T(1) - fr[2];
+    // values along x for the four (y, z) rows
+    const T r00 = x0 * f000 + x1 * f100, r10 = x0 * f010 + x1 * f110;
+    const T r01 = x0 * f001 + x1 * f101, r11 = x0 * f011 + x1 * f111;
+    m.V[c] = z0 * (y0 * r00 + y1 * r10) + z1 * (y0 * r01 + y1 * r11);
+    const T d00 = f100 - f000, d10 = f110 - f010, d01 = f101 - f001, d11 = f111 - f011;
+    m.dV[c][0] = clamped[0] ? T(0) : z0 * (y0 * d00 + y1 * d10) + z1 * (y0 * d01 + y1 * d11);
+    m.dV[c][1] = clamped[1] ? T(0) : z0 * (r10 - r00) + z1 * (r11 - r01);
+    m.dV[c][2] = clamped[2] ? T(0) : (y0 * r01 + y1 * r11) - (y0 * r00 + y1 * r10);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void mac_scatter(T* __restrict__ gvel, const KVel& v, const MacState<T>& m,
+                                            const T H[3]) {
+  const int sx = v.dof, sy = v.w.nx * v.dof, sz = v.w.ny * sy;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (H[c] == T(0)) continue;
+    T* b = gvel + m.base[c];
+    const T x1 = m.f[c][0], x0 = T(1) - x1, y1 = m.f[c][1], y0 = T(1) - y1, z1 = m.f[c][2], z0 = T(1) - z1;
+    const T h = H[c];
+    atomic_add(b, h * x0 * y0 * z0);
+    atomic_add(b + sx, h * x1 * y0 * z0);
+    atomic_add(b + sy, h * x0 * y1 * z0);
+    atomic_add(b + sy + sx, h * x1 * y1 * z0);
+    atomic_add(b + sz, h * x0 * y0 * z1);
+    atomic_add(b + sz + sx, h * x1 * y0 * z1);
+    atomic_add(b + sz + sy, h * x0 * y1 * z1);
+    atomic_add(b + sz + sy + sx, h * x1 * y1 * z1);
+  }
+}
+
+// LDS box of lam for one 32x4x2 block of nodes plus the stencil reach R
+constexpr int ABX = 32, ABY = 4, ABZ = 2;
+constexpr int AR = 5;  // reach: |departure| <= AR - 2 cells lands in the box (cubic stencil -1..+2)
+constexpr int LBX = ABX + 2 * AR, LBY = ABY + 2 * AR, LBZ = ABZ + 2 * AR;
+
+template <typename T, int INTERP, int BT>
+__global__ __launch_bounds__(256) void sl_adjoint_k(const T* __restrict__ phi0, const T* __restrict__ vel,
+                                                    const T* __restrict__ lam_next, T* __restrict__ lam,
+                                                    T* __restrict__ gvel, AdjP P) {
+  __shared__ T box[LBZ * LBY * LBX];
+  const int tid = threadIdx.x + ABX * (threadIdx.y + ABY * threadIdx.z);
+  for (int i = tid; i < LBZ * LBY * LBX; i += 256) box[i] = T(0);
+  __syncthreads();
+
+  const int ox = blockIdx.x * ABX, oy = blockIdx.y * ABY, oz = blockIdx.z * ABZ;
+  const int i = ox + threadIdx.x, j = oy + threadIdx.y, k = oz + threadIdx.z;
+  const int nx = P.w.nx, ny = P.w.ny, nz = P.w.nz;
+  const int Y = nx, Z = nx * ny;
+  const bool valid = i < nx && j < ny && k < nz;
+  const bool inner = valid && i >= 2 && i <= nx - 3 && j >= 2 && j <= ny - 3 && k >= 2 && k <= nz - 3;
+  if (valid && !inner) {
+    const int id = k * Z + j * Y + i;
+    const T g = lam_next[id];
+    if (g != T(0)) atomic_add(&lam[id], g);
+  }
+  if (inner) {
+    const int id = k * Z + j * Y + i;
+    const T g = lam_next[id];
+    if (g != T(0)) {
+      const int X = P.w.px + i, Yg = P.w.py + j, Zg = P.w.pz + k;
+      const double Xd[3] = {(double)X, (double)Yg, (double)Zg};
+      T V1[3];
+      node_vel<T, 3>(vel, P.v, X, Yg, Zg, V1);
+      MacState<T> m2;
+      bool cm[3] = {false, false, false};
+      double S[3];
+      if (BT == DCL_RK2) {
+        double Xm[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) Xm[a] = Xd[a] - (double)V1[a] * P.ch[a];
+        mac_state<T>(vel, P.v, Xm, m2, cm);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) S[a] = Xd[a] - (double)m2.V[a] * P.c[a];
+      } else {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) S[a] = Xd[a] - (double)V1[a] * P.c[a];
+      }
+      // split + weights
+      const int NP = INTERP == DCL_CUBIC ? 4 : 2;
+      const int OFF = INTERP == DCL_CUBIC ? -1 : 0;
+      const int pp[3] = {P.w.px, P.w.py, P.w.pz}, nn[3] = {nx, ny, nz};
+      int b[3];
+      bool cs[3];
+      T w[3][4], dw[3][4];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        double f;
+        if (INTERP == DCL_CUBIC) {
+          const double lo = pp[a] + 1, hi = pp[a] + nn[a] - 2;
+          cs[a] = !(S[a] >= lo && S[a] <= hi);
+          b[a] = cub_split(S[a], pp[a], nn[a], f);
+          cr_weights<T>((T)f, w[a]);
+          cr_dweights<T>((T)f, dw[a]);
+        } else {
+          const double lo = pp[a], hi = pp[a] + nn[a] - 1;
+          cs[a] = !(S[a] >= lo && S[a] <= hi);
+          b[a] = lin_split(S[a], pp[a], nn[a], f);
+          w[a][0] = (T)(1.0 - f);
+          w[a][1] = (T)f;
+          dw[a][0] = T(-1);
+          dw[a][1] = T(1);
+        }
+      }
+      // transposed gather into the LDS box (or HBM outside it) + gradient gather
+      const int bx0 = b[0] + OFF - (ox - AR), by0 = b[1] + OFF - (oy - AR), bz0 = b[2] + OFF - (oz - AR);
+      const bool in_box = bx0 >= 0 && bx0 + NP <= LBX && by0 >= 0 && by0 + NP <= LBY && bz0 >= 0 &&
+                          bz0 + NP <= LBZ;
+      T G[3] = {T(0), T(0), T(0)};
+      const T* src = phi0 + (b[2] + OFF) * Z + (b[1] + OFF) * Y + (b[0] + OFF);
+#pragma unroll
+      for (int kk = 0; kk < NP; ++kk)
+#pragma unroll
+        for (int jj = 0; jj < NP; ++jj) {
+          const T wyz = w[1][jj] * w[2][kk];
+          const T* r = src + kk * Z + jj * Y;
+          T rx = T(0), rd = T(0);
+#pragma unroll
+          for (int ii = 0; ii < NP; ++ii) {
+            const T f = r[ii];
+            rx += w[0][ii] * f;
+            rd += dw[0][ii] * f;
+            const T add = g * w[0][ii] * wyz;
+            if (in_box)
+              atomicAdd(&box[((bz0 + kk) * LBY + (by0 + jj)) * LBX + (bx0 + ii)], add);
+            else
+              atomic_add(&lam[(b[2] + OFF + kk) * Z + (b[1] + OFF + jj) * Y + (b[0] + OFF + ii)], add);
+          }
+          G[0] += rd * wyz;
+          G[1] += rx * dw[1][jj] * w[2][kk];
+          G[2] += rx * w[1][jj] * dw[2][kk];
+        }
+      T H[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) H[a] = cs[a] ? T(0) : (T)(-P.c[a]) * g * G[a];
+      T H1[3];
+      if (BT == DCL_RK2) {
+        mac_scatter<T>(gvel, P.v, m2, H);
+#pragma unroll
+        for (int bb = 0; bb < 3; ++bb) {
+          const T K = H[0] * m2.dV[0][bb] + H[1] * m2.dV[1][bb] + H[2] * m2.dV[2][bb];
+          H1[bb] = (T)(-P.ch[bb]) * K;
+        }
+      } else {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) H1[a] = H[a];
+      }
+      // node velocity: V_a = (f(X) + f(X + e_a)) / 2
+      const int sx = P.v.dof, sy = P.w.nx * P.v.dof, sz = P.w.ny * sy;
+      T* gb = gvel + (k * P.w.ny + j) * sy + i * sx + P.v.udof;
+      const T h0 = T(0.5) * H1[0], h1 = T(0.5) * H1[1], h2 = T(0.5) * H1[2];
+      atomic_add(gb, h0);
+      atomic_add(gb + sx, h0);
+      atomic_add(gb + 1, h1);
+      atomic_add(gb + sy + 1, h1);
+      atomic_add(gb + 2, h2);
+      atomic_add(gb + sz + 2, h2);
+    }
+  }
+  __syncthreads();
+  // flush the box: one HBM atomic per nonzero word inside the window
+  for (int t = tid; t < LBZ * LBY * LBX; t += 256) {
+    const T v = box[t];
+    if (v == T(0)) continue;
+    const int lx = t % LBX, ly = (t / LBX) % LBY, lz = t / (LBX * LBY);
+    const int gx = ox - AR + lx, gy = oy - AR + ly, gz = oz - AR + lz;
+    if (gx < 0 || gx >= nx || gy < 0 || gy >= ny || gz < 0 || gz >= nz) continue;
+    atomic_add(&lam[gz * Z + gy * Y + gx], v);
+  }
+}
+
+template <typename T, int INTERP, int BT>
+static int launch_adj(const void* phi0, const void* vel, const void* lam_next, void* lam, void* gvel,
+                      const AdjP& P, hipStream_t s) {
+  dim3 grid((P.w.nx + ABX - 1) / ABX, (P.w.ny + ABY - 1) / ABY, (P.w.nz + ABZ - 1) / ABZ);
+  hipLaunchKernelGGL((sl_adjoint_k<T, INTERP, BT>), grid, dim3(ABX, ABY, ABZ), 0, s,
+                     static_cast<const T*>(phi0), static_cast<const T*>(vel),
+                     static_cast<const T*>(lam_next), static_cast<T*>(lam), static_cast<T*>(gvel), P);
+  DCL_LAUNCHED("sl_adjoint");
+  return 0;
+}
+
+template <typename T>
+static int dispatch_adj(int interp, int bt, const void* phi0, const void* vel, const void* ln, void* lam,
+                        void* gv, const AdjP& P, hipStream_t s) {
+  if (interp == DCL_CUBIC && bt == DCL_RK2) return launch_adj<T, DCL_CUBIC, DCL_RK2>(phi0, vel, ln, lam, gv, P, s);
+  if (interp == DCL_CUBIC) return launch_adj<T, DCL_CUBIC, DCL_EULER>(phi0, vel, ln, lam, gv, P, s);
+  if (bt == DCL_RK2) return launch_adj<T, DCL_LINEAR, DCL_RK2>(phi0, vel, ln, lam, gv, P, s);
+  return launch_adj<T, DCL_LINEAR, DCL_EULER>(phi0, vel, ln, lam, gv, P, s);
+}
+
+}  // namespace dcl
+
+using namespace dcl;
+
+extern "C" int dcl_sl_adjoint(const void* phi0, const dcl_window* win, const dcl_velocity* vel,
+                              const void* lam_next, void* lam, void* gvel, double dt, int precision,
+                              int interp, int backtrace, void* stream) {
+  DCL_TRY(check_precision(precision));
+  if (!phi0 || !lam_next || !lam || !gvel) return set_error(DCL_ERR_ARG_NULL, "a buffer is NULL");
+  DCL_TRY(check_window(win, "phi", 5));
+  if (win->n.z < 2) return set_error(DCL_ERR_SUP, "the adjoint is 3D only");
+  DCL_TRY(check_velocity(vel, 3));
+  const dcl_window& vw = vel->win;
+  if (vw.p.x != win->p.x || vw.p.y != win->p.y || vw.p.z != win->p.z || vw.n.x != win->n.x ||
+      vw.n.y != win->n.y || vw.n.z != win->n.z)
+    return set_error(DCL_ERR_SUP, "the adjoint needs the velocity window to equal phi's");
+  if (interp != DCL_LINEAR && interp != DCL_CUBIC) return set_error(DCL_ERR_ARG_WRONG, "bad interp");
+  if (backtrace != DCL_EULER && backtrace != DCL_RK2) return set_error(DCL_ERR_ARG_WRONG, "bad backtrace");
+  if (lam == lam_next || lam == phi0) return set_error(DCL_ERR_ARG_WRONG, "lam must not alias inputs");
+  AdjP P;
+  P.w = kwin(*win);
+  P.v.w = kwin(vw);
+  P.v.dof = vel->dof;
+  P.v.udof = vel->udof;
+  const double d[3] = {win->d.x, win->d.y, win->d.z};
+  for (int a = 0; a < 3; ++a) {
+    P.c[a] = dt / d[a];
+    P.ch[a] = dt / (2.0 * d[a]);
+  }
+  hipStream_t s = as_stream(stream);
+  if (precision == DCL_F64)
+    return dispatch_adj<double>(interp, backtrace, phi0, vel->data, lam_next, lam, gvel, P, s);
+  return dispatch_adj<float>(interp, backtrace, phi0, vel->data, lam_next, lam, gvel, P, s);
+}

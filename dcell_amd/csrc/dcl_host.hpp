@@ -29,6 +29,12 @@ int check_precision(int precision);
 
 inline unsigned grid1d(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
+// Device scratch owned by the calling host thread on the current device
+// (grown on demand, kept until the process ends).  Only the synchronous
+// entry points (reductions with host results) use it, and they synchronize
+// their stream before returning, so the buffer is free again on return.
+int scratch(size_t bytes, void** ptr);
+
 }  // namespace dcl
 
 #define DCL_HIP(call)                                                          \

@@ -53,6 +53,36 @@ int check_precision(int precision) {
   return 0;
 }
 
+struct ScratchSlot {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+int scratch(size_t bytes, void** ptr) {
+  static thread_local ScratchSlot slots[16];
+  int dev = 0;
+  DCL_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 16) return set_error(DCL_ERR_SUP, "device %d out of range", dev);
+  ScratchSlot& s = slots[dev];
+  if (s.bytes < bytes) {
+    if (s.ptr) {
+      DCL_HIP(hipDeviceSynchronize());
+      DCL_HIP(hipFree(s.ptr));
+      s.ptr = nullptr;
+      s.bytes = 0;
+    }
+    size_t want = bytes < 4096 ? 4096 : bytes;
+    hipError_t e = hipMalloc(&s.ptr, want);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return set_error(DCL_ERR_MEM, "scratch hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+    }
+    s.bytes = want;
+  }
+  *ptr = s.ptr;
+  return 0;
+}
+
 }  // namespace dcl
 
 using namespace dcl;

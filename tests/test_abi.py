@@ -13,7 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def header_symbols(path):
     text = open(path).read()
-    return sorted(set(re.findall(r"\b(dcl_[a-z0-9_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"^(?:int|const char \*)\s*(dcl_[a-z0-9_]+)\s*\(", text, re.M)))
 
 
 def test_library_exports_every_declared_symbol():

@@ -1,0 +1,118 @@
+"""The gfx950 adjoint of the SL step (dcl_sl_adjoint) against the FP64 oracle
+adjoint (itself pinned by finite differences in test_oracle.py) and, through
+the torch.autograd wrapper with checkpointing, against central finite
+differences of the multi-step loss.
+
+FP64 kernel vs oracle: 1e-11 relative (atomics reorder the sums).
+FP32 kernel vs oracle: 1e-4 relative on 99.9% of entries and 2e-3 in
+relative L2 (a float32 departure point can land across a cell face from the
+float64 one, where trilinear derivatives jump).  FP32 autograd gradient vs
+FP64 central differences: within 1e-3 of the difference quotient plus 1e-3
+of the largest gradient entry (the differentiation-form target).
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+import dcell_amd as D
+from dcell_amd import adjoint as A
+from dcell_amd import configs
+from helpers import owin, random_velocity, rel_max, rounded, smooth_field, to_dev, to_np
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-11), (torch.float32, 1e-4)])
+@pytest.mark.parametrize("interp,bt", [("cubic", "rk2"), ("linear", "euler"), ("cubic", "euler"),
+                                       ("linear", "rk2")])
+def test_adjoint_matches_oracle(dev, dtype, tol, interp, bt):
+    rng = np.random.default_rng(90)
+    n = 28
+    h = 1.0 / (n - 1)
+    win = D.GridWindow((3, 1, -2), (n, n, n), (h, h, h))
+    phi0 = rounded(smooth_field(n, 3, rng), dtype)
+    vel = rounded(random_velocity(n, 3, rng, 1.0), dtype)
+    lam_next = rounded(rng.normal(size=(n, n, n)), dtype)
+    dt = 2.5 * h                                   # departures beyond the LDS box too
+    im = O.CUBIC if interp == "cubic" else O.LINEAR
+    bm = O.RK2 if bt == "rk2" else O.EULER
+    lam_ref, gvel_ref = O.sl_adjoint(phi0, vel, lam_next, owin(win), dt, im, bm)
+    lam, gvel = A.sl_adjoint(to_dev(phi0, dtype), to_dev(vel, dtype), to_dev(lam_next, dtype), win, dt,
+                             interp, bt)
+    if dtype == torch.float64:
+        assert rel_max(to_np(lam), lam_ref) <= tol
+        assert rel_max(to_np(gvel), gvel_ref) <= tol
+    else:
+        # float32 forward states can fall on the other side of a cell face than
+        # float64 ones, where the trilinear derivatives jump: allow 0.1% of
+        # entries beyond the bound and bound the relative L2 error
+        for got, ref in ((to_np(lam), lam_ref), (to_np(gvel), gvel_ref)):
+            bad = np.abs(got - ref) > tol * np.max(np.abs(ref))
+            assert bad.mean() <= 1e-3
+            assert np.linalg.norm(got - ref) <= 2e-3 * np.linalg.norm(ref)
+
+
+def _c5_problem(n, steps, dtype):
+    """A scaled-down differentiation form: sphere r 0.2 centred, steady
+    velocity of random Fourier modes (k 1..3, seed 5), target from seed 6."""
+    h = 1.0 / (n - 1)
+    win = D.GridWindow.unit_box(n)
+    phi0 = configs.sphere_sdf(n, (0.5, 0.5, 0.5), 0.2)
+    vel = configs.random_fourier_velocity(n, 6, 3, 0.5, seed=5)
+    vt = configs.random_fourier_velocity(n, 6, 3, 0.5, seed=6)
+    vmax = max(np.abs(vel).max(), np.abs(vt).max())
+    dt = 0.8 * h / vmax
+    ow = owin(win)
+    target = phi0
+    for _ in range(steps):
+        target = O.sl_advect(target, vt, ow, dt, O.CUBIC, O.RK2)
+    return win, phi0, vel, target, dt
+
+
+def test_autograd_checkpointed_gradient_vs_finite_differences(dev):
+    n, steps, eps = 32, 12, 1.5
+    win, phi0, vel, target, dt = _c5_problem(n, steps, torch.float32)
+    ow = owin(win)
+
+    def loss_fp64(v):
+        p = phi0
+        for _ in range(steps):
+            p = O.sl_advect(p, v, ow, dt, O.CUBIC, O.RK2)
+        H = lambda x: 0.5 * (1 + np.clip(x / eps, -1, 1) + np.sin(np.pi * np.clip(x / eps, -1, 1)) / np.pi)  # noqa
+        return float(((H(p) - H(target)) ** 2).sum())
+
+    v = to_dev(vel, torch.float32).requires_grad_(True)
+    tgt = to_dev(target, torch.float32)
+    phiT = A.advect_n(to_dev(phi0, torch.float32), v, win, dt, steps, checkpoint_every=5)
+    loss = ((A.heaviside(phiT, eps) - A.heaviside(tgt, eps)) ** 2).sum()
+    loss.backward()
+    g = to_np(v.grad)
+    # the FP64 oracle gradient through the same checkpoint-free chain
+    lam = None
+    states = [phi0]
+    for _ in range(steps):
+        states.append(O.sl_advect(states[-1], vel, ow, dt, O.CUBIC, O.RK2))
+    x = np.clip(states[-1] / eps, -1, 1)
+    Hp = 0.5 * (1 + x + np.sin(np.pi * x) / np.pi)
+    xt = np.clip(target / eps, -1, 1)
+    Ht = 0.5 * (1 + xt + np.sin(np.pi * xt) / np.pi)
+    dH = np.where(np.abs(states[-1]) <= eps, (1 + np.cos(np.pi * x)) / (2 * eps), 0.0)
+    lam = 2 * (Hp - Ht) * dH
+    gref = np.zeros_like(vel)
+    for s in reversed(range(steps)):
+        lam, gv = O.sl_adjoint(states[s], vel, lam, ow, dt, O.CUBIC, O.RK2)
+        gref += gv
+    assert np.linalg.norm(g - gref) <= 2e-3 * np.linalg.norm(gref)
+    # central finite differences in FP64 on the largest-gradient components
+    rng = np.random.default_rng(7)
+    flat = np.argsort(np.abs(gref).ravel())[::-1][:200]
+    pick = rng.choice(flat, size=8, replace=False)
+    for f in pick:
+        idx = np.unravel_index(f, vel.shape)
+        d = 1e-4 * np.abs(vel).max()
+        vp, vm = vel.copy(), vel.copy()
+        vp[idx] += d
+        vm[idx] -= d
+        fd = (loss_fp64(vp) - loss_fp64(vm)) / (2 * d)
+        assert abs(g[idx] - fd) <= 1e-3 * abs(fd) + 1e-3 * np.abs(gref).max(), (idx, g[idx], fd)
