@@ -8,13 +8,16 @@ over the grid; one grid-point update = one phi value written by the SL step
 and one curvature value at the same node, counted over the (N-4)^3 interior
 nodes the SL step advects.  Inputs are resident in HBM when timing starts.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu]
+Beside the headline, "forms" reports the same metric for the reference's own
+scheme (Euler + trilinear), the FP32 form at 384^3 and the other workloads,
+each over its own timed steps.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu] [--no-forms]
 """
 from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -24,10 +27,15 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-HBM_PEAK = 8.0e12           # MI355X HBM3E, bytes/s
-B_SL_F64 = 8 + 24 + 8       # phi0 read, velocity read, phi write (per update)
-B_CURV_F64 = 8 + 8          # phi read, kappa write
-B_STEP_F64 = 48             # phi0 + velocity + phi + kappa
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def b_alg(itemsize):
+    """Algorithmic bytes per update: phi0 read, 3 velocity components read, phi
+    and kappa written (48 B in FP64, 24 B in FP32); per kernel: the SL pass
+    reads phi0 + velocity and writes phi, the curvature pass reads phi and
+    writes kappa."""
+    return dict(step=6 * itemsize, sl=5 * itemsize, curv=2 * itemsize)
 
 
 def parse():
@@ -37,61 +45,60 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-steps", type=int, default=2,
-                    help="full-grid steps timed in the CPU baseline leg")
+    ap.add_argument("--no-forms", action="store_true", help="only the headline workload")
+    ap.add_argument("--cpu-steps", type=int, default=2, help="full-grid steps of the CPU baseline leg")
     return ap.parse_args()
 
 
-def headline(args):
+def dense_run(n, dtype, interp, bt, steps, warmup, deform=True):
+    """Time `steps` dense updates (SL kernel then curvature kernel) of an n^3
+    sphere under the deformation field; per-kernel times from HIP events on
+    the stream the kernels run on."""
     import torch
     import dcell_amd as D
     from dcell_amd import configs
 
-    n = args.n
     win = D.GridWindow.unit_box(n)
     dev = torch.device("cuda:0")
-    phi = torch.from_numpy(configs.sphere_sdf(n)).to(dev)
-    vel = torch.from_numpy(configs.deformation_velocity(n)).to(dev)
+    phi = torch.from_numpy(configs.sphere_sdf(n)).to(dev, dtype)
+    vel = torch.from_numpy(configs.deformation_velocity(n)).to(dev, dtype)
     bufs = [phi, torch.empty_like(phi)]
     kappa = torch.empty_like(phi)
-    dts = [configs.deformation_dt(n, s) for s in range(args.warmup + args.steps)]
+    dts = [configs.deformation_dt(n, s) for s in range(warmup + steps)]
 
     def step(s, ev=None):
         src, dst = bufs[s % 2], bufs[(s + 1) % 2]
         if ev is not None:
             ev[0].record()
-        D.advect_sl(src, vel, win, dts[s], "cubic", "rk2", out=dst)
+        D.advect_sl(src, vel, win, dts[s], interp, bt, out=dst)
         if ev is not None:
             ev[1].record()
         D.curvature(dst, win, out=kappa)
         if ev is not None:
             ev[2].record()
 
-    for s in range(args.warmup):
+    for s in range(warmup):
         step(s)
     torch.cuda.synchronize()
-    events = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    events = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
     t0 = time.perf_counter()
-    for i in range(args.steps):
-        step(args.warmup + i, events[i])
+    for i in range(steps):
+        step(warmup + i, events[i])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    sl_ms = np.array([e[0].elapsed_time(e[1]) for e in events])
-    cv_ms = np.array([e[1].elapsed_time(e[2]) for e in events])
+    sl_s = float(np.mean([e[0].elapsed_time(e[1]) for e in events])) * 1e-3
+    cv_s = float(np.mean([e[1].elapsed_time(e[2]) for e in events])) * 1e-3
     interior = (n - 4) ** 3
-    ups = interior * args.steps / wall
-    sl_avg = float(sl_ms.mean()) * 1e-3
-    achieved = interior * B_SL_F64 / sl_avg
-    return dict(
-        n=n, wall=wall, ups=ups, ms_per_step=wall / args.steps * 1e3,
-        sl_ms=float(sl_ms.mean()), curv_ms=float(cv_ms.mean()),
-        roofline=dict(bound="hbm", kernel="sl_dense<double,3,cubic,rk2>",
-                      achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s",
-                      frac=achieved / HBM_PEAK, traffic=None,
-                      bytes_per_launch=interior * B_SL_F64),
-        step_frac=ups * B_STEP_F64 / HBM_PEAK,
-        curv_gbs=interior * B_CURV_F64 / (float(cv_ms.mean()) * 1e-3) / 1e9,
-    )
+    B = b_alg(torch.tensor([], dtype=dtype).element_size())
+    ups = interior * steps / wall
+    out = dict(n=n, steps=steps, wall_s=wall, updates_per_s=ups, ms_per_step=wall / steps * 1e3,
+               step_roofline_frac=ups * B["step"] / HBM_PEAK,
+               sl_ms=sl_s * 1e3, curv_ms=cv_s * 1e3,
+               sl_gbs=interior * B["sl"] / sl_s / 1e9, curv_gbs=interior * B["curv"] / cv_s / 1e9,
+               bytes_per_update=B["step"])
+    del phi, vel, bufs, kappa
+    torch.cuda.empty_cache()
+    return out
 
 
 def cpu_baseline(args):
@@ -134,17 +141,82 @@ def cpu_baseline(args):
                 all_cores=dict(value=out[nthreads], cores=nthreads), cpu=model)
 
 
+def batch_run(ncells, seed, steps, warmup, interp="cubic", bt="rk2", autotune=True):
+    """The batching form: ncells cells (edges log-uniform in [32, 128]), FP32,
+    each a perturbed sphere in random Fourier-mode velocity with its own dt;
+    one step = one hipGraph launch (SL + curvature over the packed tile list)."""
+    import torch
+    import dcell_amd as D
+    from dcell_amd import configs
+    from dcell_amd.batch import Batch
+
+    edges = configs.batch_edges(ncells, seed)
+    g = configs.philox(seed)
+    wins = [D.GridWindow.unit_box(int(n)) for n in edges]
+    b = Batch(wins, torch.float32, interp, bt)
+    phi0, phi1, kap, vel = b.new_field(), b.new_field(), b.new_field(), b.new_field(3)
+    dts = []
+    for c, n in enumerate(edges):
+        phi, v, dt = configs.cell_fields(int(n), g, "cuda:0", torch.float32)
+        b.cell(phi0, c).copy_(phi)
+        b.cell(vel, c, 3).copy_(v)
+        dts.append(dt)
+    b.set_dt(dts)
+    tune = None
+    if autotune:
+        best, table = b.autotune(phi0, phi1, kap, vel, reps=3)
+        tune = dict(best_ms=best, table_ms=table.round(4).tolist(), chosen=b.config())
+    graphs = [b.graph(phi0, phi1, kap, vel), b.graph(phi1, phi0, kap, vel)]
+    for s in range(warmup):
+        Batch.launch(graphs[s % 2])
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for s in range(steps):
+        Batch.launch(graphs[s % 2])
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_s = e0.elapsed_time(e1) * 1e-3
+    interior = int(sum((int(n) - 4) ** 3 for n in edges))
+    ups = interior * steps / wall
+    out = dict(cells=ncells, nodes=int(b.nodes), interior_updates_per_step=interior, steps=steps,
+               interp=interp, backtrace=bt, updates_per_s=ups, ms_per_step=wall / steps * 1e3,
+               gpu_ms_per_step=gpu_s / steps * 1e3, step_roofline_frac=ups * 24 / HBM_PEAK,
+               autotune=tune)
+    b.close()
+    del phi0, phi1, kap, vel
+    torch.cuda.empty_cache()
+    return out
+
+
+def forms(args):
+    import torch
+    res = {}
+    res["batched_fp32_config_scheme"] = batch_run(512, 4, 100, 3, "cubic", "rk2")
+    res["batched_fp32_reference_scheme"] = batch_run(512, 4, 100, 3, "linear", "euler")
+    res["fp64_reference_scheme_256"] = dense_run(256, torch.float64, "linear", "euler", 50, 2)
+    res["fp32_config_scheme_384"] = dense_run(384, torch.float32, "cubic", "rk2", 50, 2)
+    res["fp32_reference_scheme_384"] = dense_run(384, torch.float32, "linear", "euler", 50, 2)
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported "
-              "(the workload is one GPU's share; see DESIGN.md for how it would shard)",
+              "(the workload is one GPU's share; DESIGN.md describes how it would shard)",
               file=sys.stderr)
         sys.exit(2)
-    r = headline(args)
+    import torch
+    r = dense_run(args.n, torch.float64, "cubic", "rk2", args.steps, args.warmup)
+    interior = (args.n - 4) ** 3
+    B = b_alg(8)
+    achieved = interior * B["sl"] / (r["sl_ms"] * 1e-3)
     line = {
         "metric": "grid-point updates/s (SL advect+curvature), % HBM roofline, FP32 vs FP64",
-        "value": r["ups"],
+        "value": r["updates_per_s"],
         "unit": "grid-point updates/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -155,14 +227,20 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"3D deformation, {r['n']}^3 FP64 sphere r=0.15, RK2 backtrace + "
-                               "tricubic SL + 19-point curvature, dt=h",
-                   "grid": [r["n"]] * 3, "interior_updates_per_step": (r["n"] - 4) ** 3,
+        "config": {"workload": f"3D deformation, {args.n}^3 FP64 sphere r=0.15, RK2 backtrace + "
+                               "Catmull-Rom tricubic SL + 19-point curvature, dt=h",
+                   "grid": [args.n] * 3, "interior_updates_per_step": interior,
                    "parallelism": "single GPU"},
-        "roofline": r["roofline"],
-        "step_roofline_frac": r["step_frac"],
+        "roofline": {"bound": "hbm", "kernel": "sl_tiled3d<double,cubic,rk2>",
+                     "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "bytes_per_launch": interior * B["sl"]},
+        "step_roofline_frac": r["step_roofline_frac"],
         "kernels_ms": {"sl_advect": r["sl_ms"], "curvature": r["curv_ms"]},
+        "curvature_gbs": r["curv_gbs"],
     }
+    if not args.no_forms:
+        line["forms"] = forms(args)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(line), flush=True)

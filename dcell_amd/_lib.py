@@ -26,6 +26,9 @@ EXPORTS = (
     "dcl_pls_advect", "dcl_pls_ws_create", "dcl_pls_ws_destroy", "dcl_pls_correct",
     "dcl_pls_adjust_radii", "dcl_max_velocity", "dcl_band_compact",
     "dcl_interface_metrics", "dcl_error_norms", "dcl_sl_adjoint",
+    "dcl_batch_create", "dcl_batch_destroy", "dcl_batch_layout", "dcl_batch_set_dt",
+    "dcl_batch_set_config", "dcl_batch_get_config", "dcl_batch_step", "dcl_batch_autotune",
+    "dcl_batch_graph_create", "dcl_batch_graph_launch", "dcl_batch_graph_destroy",
 )
 
 
@@ -82,6 +85,17 @@ def _declare(L):
         "dcl_interface_metrics": ([vp, vp, W, dbl, i32, C.POINTER(dbl), vp], i32),
         "dcl_error_norms": ([vp, i32, vp, i32, i64, C.POINTER(dbl), vp], i32),
         "dcl_sl_adjoint": ([vp, W, V, vp, vp, vp, dbl, i32, i32, i32, vp], i32),
+        "dcl_batch_create": ([i32, W, i32, i32, i32, C.POINTER(vp)], i32),
+        "dcl_batch_destroy": ([vp], i32),
+        "dcl_batch_layout": ([vp, vp, vp], i32),
+        "dcl_batch_set_dt": ([vp, vp], i32),
+        "dcl_batch_set_config": ([vp, i32, i32], i32),
+        "dcl_batch_get_config": ([vp, vp, vp, vp, vp], i32),
+        "dcl_batch_step": ([vp, vp, vp, vp, vp, vp], i32),
+        "dcl_batch_autotune": ([vp, vp, vp, vp, vp, i32, vp, vp, vp], i32),
+        "dcl_batch_graph_create": ([vp, vp, vp, vp, vp, C.POINTER(vp)], i32),
+        "dcl_batch_graph_launch": ([vp, vp], i32),
+        "dcl_batch_graph_destroy": ([vp], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name, None)

@@ -1,0 +1,94 @@
+"""Batched update of many independent level sets over the C ABI (dcl_batch_*).
+
+The cells' windows are packed back to back in one phi buffer, one kappa
+buffer and one velocity buffer; ``Batch.step`` runs the SL step and the
+curvature of every cell in two persistent launches, ``Batch.autotune`` picks
+the tile shape and occupancy by timing them all, and ``Batch.graph`` records a
+step as a hipGraph for launch-bound loops.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .levelset import INTERP, BACKTRACE, GridWindow, _p, stream_handle
+
+NCONFIG = 18  # 6 tile shapes x 3 occupancies
+
+
+class Batch:
+    def __init__(self, windows: list[GridWindow], dtype=torch.float32, interp="cubic", backtrace="rk2"):
+        self.windows = list(windows)
+        self.dtype = dtype
+        self.prec = L.DCL_F64 if dtype == torch.float64 else L.DCL_F32
+        arr = (L.Window * len(windows))(*[w.c() for w in windows])
+        h = C.c_void_p()
+        L.call("dcl_batch_create", len(windows), arr, self.prec, INTERP[interp], BACKTRACE[backtrace],
+               C.byref(h))
+        self.handle = h
+        po = np.zeros(len(windows) + 1, dtype=np.int64)
+        vo = np.zeros(len(windows) + 1, dtype=np.int64)
+        L.call("dcl_batch_layout", h, po.ctypes.data_as(C.c_void_p), vo.ctypes.data_as(C.c_void_p))
+        self.phi_offsets, self.vel_offsets = po, vo
+        self.nodes = int(po[-1])
+        self.graphs = []
+
+    def new_field(self, components: int = 1) -> torch.Tensor:
+        return torch.zeros(self.nodes * components, dtype=self.dtype, device="cuda:0")
+
+    def cell(self, t: torch.Tensor, c: int, components: int = 1) -> torch.Tensor:
+        """View of cell c in a packed buffer, shaped like its window."""
+        w = self.windows[c]
+        if components == 1:
+            return t[self.phi_offsets[c]:self.phi_offsets[c + 1]].view(w.shape)
+        return t[self.vel_offsets[c]:self.vel_offsets[c + 1]].view(w.shape + (components,))
+
+    def set_dt(self, dt):
+        dt = np.ascontiguousarray(dt, dtype=np.float64)
+        L.call("dcl_batch_set_dt", self.handle, dt.ctypes.data_as(C.c_void_p))
+
+    def set_config(self, shape: int, occupancy: int):
+        L.call("dcl_batch_set_config", self.handle, shape, occupancy)
+
+    def config(self):
+        s, o, b = C.c_int(), C.c_int(), C.c_int()
+        tile = (C.c_int32 * 3)()
+        L.call("dcl_batch_get_config", self.handle, C.byref(s), C.byref(o), tile, C.byref(b))
+        return dict(shape=s.value, occupancy=o.value, tile=list(tile), blocks=b.value)
+
+    def step(self, phi0, phi, kappa, vel):
+        L.call("dcl_batch_step", self.handle, _p(phi0), _p(phi), _p(kappa), _p(vel), stream_handle())
+
+    def autotune(self, phi0, phi, kappa, vel, reps: int = 3):
+        table = np.zeros(NCONFIG)
+        best = C.c_double()
+        L.call("dcl_batch_autotune", self.handle, _p(phi0), _p(phi), _p(kappa), _p(vel), reps,
+               table.ctypes.data_as(C.c_void_p), C.byref(best), stream_handle())
+        return best.value, table.reshape(6, 3)
+
+    def graph(self, phi0, phi, kappa, vel):
+        g = C.c_void_p()
+        L.call("dcl_batch_graph_create", self.handle, _p(phi0), _p(phi), _p(kappa), _p(vel), C.byref(g))
+        self.graphs.append(g)
+        return g
+
+    @staticmethod
+    def launch(graph):
+        L.call("dcl_batch_graph_launch", graph, stream_handle())
+
+    def close(self):
+        for g in self.graphs:
+            L.lib().dcl_batch_graph_destroy(g)
+        self.graphs = []
+        if self.handle:
+            L.lib().dcl_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -178,3 +178,63 @@ def trilinear_numpy(phi: np.ndarray, pos: np.ndarray) -> np.ndarray:
                 w = (fx if di else 1 - fx) * (fy if dj else 1 - fy) * (fz if dk else 1 - fz)
                 out += w * phi[k + dk, j + dj, i + di]
     return out
+
+
+# ----------------------------------------------------------------------------
+# batch of cells (torch, so the full-size batch is generated on the device)
+def _harmonics(x, y, z):
+    """The 24 real solid harmonics of degrees 1..4 (unnormalised) of a unit
+    direction (x, y, z)."""
+    x2, y2, z2 = x * x, y * y, z * z
+    return [
+        x, y, z,
+        x * y, y * z, 3 * z2 - 1, x * z, x2 - y2,
+        (3 * x2 - y2) * y, x * y * z, y * (5 * z2 - 1), z * (5 * z2 - 3), x * (5 * z2 - 1),
+        z * (x2 - y2), x * (x2 - 3 * y2),
+        x * y * (x2 - y2), y * z * (3 * x2 - y2), x * y * (7 * z2 - 1), y * z * (7 * z2 - 3),
+        35 * z2 * z2 - 30 * z2 + 3, x * z * (7 * z2 - 3), (x2 - y2) * (7 * z2 - 1),
+        x * z * (x2 - 3 * y2), x2 * (x2 - 3 * y2) - y2 * (3 * x2 - y2),
+    ]
+
+
+def cell_fields(n: int, g: np.random.Generator, device, dtype, radius=0.25, modes=8, amp=0.05,
+                vmodes=6, kmax=4, vamp=0.5):
+    """phi (grid units) and MAC velocity of one cell on [0, 1]^3 with n nodes
+    per axis: a sphere of `radius` perturbed by `modes` random real spherical
+    harmonics of degree <= 4 (peak amplitude amp * radius), in a sum of
+    `vmodes` random divergence-free Fourier modes (|k|_inf 1..kmax, A ~ N(0, vamp))."""
+    import torch
+    h = 1.0 / (n - 1)
+    c = torch.arange(n, device=device, dtype=torch.float64) * h
+    Z, Y, X = torch.meshgrid(c, c, c, indexing="ij")
+    dx, dy, dz = X - 0.5, Y - 0.5, Z - 0.5
+    r = torch.sqrt(dx * dx + dy * dy + dz * dz).clamp_min(1e-12)
+    basis = _harmonics(dx / r, dy / r, dz / r)
+    pick = g.choice(len(basis), size=modes, replace=False)
+    coef = g.normal(size=modes)
+    pert = sum(float(a) * basis[i] for a, i in zip(coef, pick))
+    pert = pert * (amp / float(pert.abs().max()))
+    phi = (r - radius * (1.0 + pert)) / h
+    vel = torch.zeros((n, n, n, 3), device=device, dtype=torch.float64)
+    for _ in range(vmodes):
+        while True:
+            k = g.integers(-kmax, kmax + 1, size=3)
+            if 1 <= np.abs(k).max() <= kmax:
+                break
+        a = g.normal(0.0, vamp, size=3)
+        a = a - k * (a @ k) / (k @ k)
+        theta = g.uniform(0, 2 * math.pi)
+        for comp in range(3):
+            sh = [0.0, 0.0, 0.0]
+            sh[comp] = -0.5 * h
+            arg = 2 * math.pi * (k[0] * (X + sh[0]) + k[1] * (Y + sh[1]) + k[2] * (Z + sh[2])) + theta
+            vel[..., comp] += float(a[comp]) * torch.sin(arg)
+    vmax = float(vel.abs().max())
+    dt = 0.5 * h / vmax if vmax > 0 else h
+    return phi.to(dtype), vel.to(dtype), dt
+
+
+def batch_edges(ncells: int = 512, seed: int = 4, lo: int = 32, hi: int = 128) -> np.ndarray:
+    """Cell edges log-uniform in [lo, hi] (Philox, seed)."""
+    g = philox(seed)
+    return np.rint(np.exp(g.uniform(math.log(lo), math.log(hi), size=ncells))).astype(int)

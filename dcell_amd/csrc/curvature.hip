@@ -6,7 +6,11 @@
 //       (LevelSetMethod/Curvature.c:3-186)
 // kappa = 0 where the stencil leaves the window (and, for the first two, where
 // the formula gives NaN, as in the reference).
+#include <stdlib.h>
+#include <string.h>
+
 #include "dcl_host.hpp"
+#include "dcl_curv.hpp"
 
 namespace dcl {
 
@@ -15,70 +19,15 @@ struct CurvP {
   double dx, dy, dz;
 };
 
-// GridFunction3D_Curv at local node (i, j, k) of a window with row stride Y and
-// plane stride Z; p points at the node.
-template <typename T>
-__device__ __forceinline__ T curv3d_at(const T* __restrict__ p, int Y, int Z, T dx, T dy, T dz) {
-  const T c0 = p[0];
-  const T px = (p[1] - p[-1]) / (T(2.0) * dx);
-  const T py = (p[Y] - p[-Y]) / (T(2.0) * dy);
-  const T pz = (p[Z] - p[-Z]) / (T(2.0) * dz);
-  const T px2 = px * px, py2 = py * py, pz2 = pz * pz;
-  const T ppp = px2 + py2 + pz2;
-  const T pxx = (p[-1] - T(2.) * c0 + p[1]) / (dx * dx);
-  const T pyy = (p[-Y] - T(2.) * c0 + p[Y]) / (dy * dy);
-  const T pzz = (p[-Z] - T(2.) * c0 + p[Z]) / (dz * dz);
-  const T pxy = (p[-Y - 1] + p[Y + 1] - p[Y - 1] - p[-Y + 1]) / (T(4.) * dx * dy);
-  const T pxz = (p[-Z - 1] + p[Z + 1] - p[Z - 1] - p[-Z + 1]) / (T(4.) * dx * dz);
-  const T pyz = (p[-Z - Y] + p[Z + Y] - p[Z - Y] - p[-Z + Y]) / (T(4.) * dy * dz);
-  T c = px2 * (pyy + pzz) + py2 * (pxx + pzz) + pz2 * (pxx + pyy);
-  c -= T(2.) * (px * py * pxy + px * pz * pxz + py * pz * pyz);
-  c /= sqrt(ppp * ppp * ppp);
-  if (c != c) c = T(0);
-  return c;
-}
-
-template <typename T>
-__device__ __forceinline__ T curv2d_at(const T* __restrict__ p, int Y, T dx, T dy) {
-  const T px = (p[1] - p[-1]) / (T(2.) * dx);
-  const T py = (p[Y] - p[-Y]) / (T(2.) * dy);
-  const T px2 = px * px, py2 = py * py;
-  const T pxx = (p[-1] - T(2.) * p[0] + p[1]) / (dx * dx);
-  const T pyy = (p[-Y] - T(2.) * p[0] + p[Y]) / (dy * dy);
-  const T pxy = (p[-Y - 1] + p[Y + 1] - p[Y - 1] - p[-Y + 1]) / (T(4.) * dx * dy);
-  T k = (pxx * py2 - T(2.) * px * py * pxy + pyy * px2) / sqrt((px2 + py2) * (px2 + py2) * (px2 + py2));
-  if (k != k) k = T(0);
-  return k;
-}
-
-// Least-squares fit c0 + c1 a + c2 b + c3 a^2 + c4 ab + c5 b^2 over the 9x9
-// offsets; the reference's generated sums are these coefficients, with
-// pxx = c3 and pyy = c5 (half the second derivatives).  Decoupled normal
-// equations over the symmetric design: c1 = sum a p / 540, c4 = sum ab p /
-// 3600, c3 = sum (a^2 - 20/3) p / 2772.
-template <typename T>
-__device__ __forceinline__ T curv2d_lss_at(const T* __restrict__ p, int Y, T dx, T dy) {
-  T s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
-  for (int b = -4; b <= 4; ++b) {
-    const T* r = p + b * Y;
-    const T bb = T(b * b) - T(20.0 / 3.0);
-#pragma unroll
-    for (int a = -4; a <= 4; ++a) {
-      const T v = r[a];
-      s1 += T(a) * v;
-      s2 += T(b) * v;
-      s3 += (T(a * a) - T(20.0 / 3.0)) * v;
-      s4 += T(a * b) * v;
-      s5 += bb * v;
-    }
+// dense 3D windows use the 2.5D LDS kernel; DCL_DENSE_KERNEL=gather selects
+// the plain per-node kernel, for comparisons
+static bool use_tiled_curv() {
+  static int mode = -1;
+  if (mode < 0) {
+    const char* e = getenv("DCL_DENSE_KERNEL");
+    mode = (e && strcmp(e, "gather") == 0) ? 0 : 1;
   }
-  const T px = s1 / T(540.0) / dx;
-  const T py = s2 / T(540.0) / dy;
-  const T pxx = s3 / T(2772.0) / (dx * dx);
-  const T pyy = s5 / T(2772.0) / (dy * dy);
-  const T pxy = s4 / T(3600.0) / (dx * dy);
-  const T px2 = px * px, py2 = py * py;
-  return (pxx * py2 - T(2.) * px * py * pxy + pyy * px2) / sqrt((px2 + py2) * (px2 + py2) * (px2 + py2));
+  return mode == 1;
 }
 
 template <typename T, int DIM, int ST>
@@ -88,7 +37,7 @@ __device__ __forceinline__ T curv_node(const T* __restrict__ phi, const CurvP& P
   if (DIM == 3 && (k < 1 || k > P.w.nz - 2)) return T(0);
   const int Y = P.w.nx, Z = P.w.nx * P.w.ny;
   const T* p = phi + (k * P.w.ny + j) * P.w.nx + i;
-  if (DIM == 3) return curv3d_at<T>(p, Y, Z, (T)P.dx, (T)P.dy, (T)P.dz);
+  if (DIM == 3) return curv3d_at<T>(p, Y, Z, curv_recips<T>(P.dx, P.dy, P.dz));
   if (ST == DCL_CURV_LSS) return curv2d_lss_at<T>(p, Y, (T)P.dx, (T)P.dy);
   return curv2d_at<T>(p, Y, (T)P.dx, (T)P.dy);
 }
@@ -101,6 +50,57 @@ __global__ __launch_bounds__(256) void curv_dense(const T* __restrict__ phi, T* 
   const int k = blockIdx.z;
   if (i >= P.w.nx || j >= P.w.ny) return;
   kappa[(k * P.w.ny + j) * P.w.nx + i] = curv_node<T, DIM, ST>(phi, P, i, j, k);
+}
+
+// Dense 3D curvature, 2.5D tiling: a block owns a 64 x 4 x 16 column of
+// nodes; it keeps the planes k-1, k, k+1 (with a one-node halo in x and y) in
+// a rolling three-slot LDS buffer, loading one new plane per step with
+// coalesced reads, so each phi value is read from memory about once per
+// block and the 19 stencil reads come from LDS.
+constexpr int CTX = 64, CTY = 4, CTZ = 16, CPX = CTX + 2, CPY = CTY + 2;
+
+template <typename T>
+__device__ __forceinline__ void curv_load_plane(T* dst, const T* __restrict__ phi, int k, int x0, int y0,
+                                                int nx, int ny, int nz, int tid) {
+  const int kk = clampi(k, 0, nz - 1);
+  const T* src = phi + (size_t)kk * nx * ny;
+  for (int e = tid; e < CPX * CPY; e += 256) {
+    const int bx = e % CPX, by = e / CPX;
+    const int gx = clampi(x0 - 1 + bx, 0, nx - 1), gy = clampi(y0 - 1 + by, 0, ny - 1);
+    dst[e] = src[gy * nx + gx];
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void curv_tiled3d(const T* __restrict__ phi, T* __restrict__ kappa,
+                                                    CurvP P, int ntx, int nty, int ntz) {
+  __shared__ T pl[3][CPX * CPY];
+  const int nx = P.w.nx, ny = P.w.ny, nz = P.w.nz;
+  const unsigned nb = (unsigned)(ntx * nty * ntz);
+  const unsigned per = nb / 8u;
+  const unsigned lb = (per == 0 || blockIdx.x >= per * 8u) ? blockIdx.x : (blockIdx.x % 8u) * per + blockIdx.x / 8u;
+  const int tx = lb % ntx, ty = (lb / ntx) % nty, tz = lb / (ntx * nty);
+  const int x0 = tx * CTX, y0 = ty * CTY, z0 = tz * CTZ;
+  const int tid = threadIdx.x + CTX * threadIdx.y;
+  const CurvR<T> r = curv_recips<T>(P.dx, P.dy, P.dz);
+  curv_load_plane<T>(pl[0], phi, z0 - 1, x0, y0, nx, ny, nz, tid);
+  curv_load_plane<T>(pl[1], phi, z0, x0, y0, nx, ny, nz, tid);
+  const int i = x0 + threadIdx.x, j = y0 + threadIdx.y;
+  const bool in_xy = i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2;
+  const int lo = (threadIdx.y + 1) * CPX + threadIdx.x + 1;
+  for (int kz = 0; kz < CTZ; ++kz) {
+    const int k = z0 + kz;
+    if (k >= nz) break;  // uniform across the block
+    curv_load_plane<T>(pl[(kz + 2) % 3], phi, k + 1, x0, y0, nx, ny, nz, tid);
+    __syncthreads();
+    if (i < nx && j < ny) {
+      T c = T(0);
+      if (in_xy && k >= 1 && k <= nz - 2)
+        c = curv3d_planes<T>(pl[kz % 3] + lo, pl[(kz + 1) % 3] + lo, pl[(kz + 2) % 3] + lo, CPX, r);
+      kappa[((size_t)k * ny + j) * nx + i] = c;
+    }
+    __syncthreads();
+  }
 }
 
 template <typename T, int DIM, int ST, bool COMPACT>
@@ -121,6 +121,13 @@ template <typename T, int DIM, int ST>
 static int launch_curv(const void* phi, void* kappa, const CurvP& P, const dcl_icoor* band,
                        int64_t n, void* bv, hipStream_t s) {
   const T* p = static_cast<const T*>(phi);
+  if (!band && DIM == 3 && use_tiled_curv()) {
+    const int ntx = (P.w.nx + CTX - 1) / CTX, nty = (P.w.ny + CTY - 1) / CTY, ntz = (P.w.nz + CTZ - 1) / CTZ;
+    hipLaunchKernelGGL((curv_tiled3d<T>), dim3(ntx * nty * ntz), dim3(CTX, CTY), 0, s, p,
+                       static_cast<T*>(kappa), P, ntx, nty, ntz);
+    DCL_LAUNCHED("curv_tiled3d");
+    return 0;
+  }
   if (!band) {
     dim3 grid((P.w.nx + 63) / 64, (P.w.ny + 3) / 4, DIM == 3 ? P.w.nz : 1);
     hipLaunchKernelGGL((curv_dense<T, DIM, ST>), grid, dim3(64, 4), 0, s, p,

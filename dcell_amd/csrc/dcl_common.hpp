@@ -102,62 +102,55 @@ __device__ __forceinline__ void node_vel(const T* __restrict__ vel, const KVel& 
 }
 
 // MAC-staggered trilinear velocity at an arbitrary point: InterpolateVelocity3D
-// / 2D (Common/InterpolateVelocity.c:7-67), loop order k, j, i.  The
-// fraction split is done in double; weights and sums in T.
+// / 2D (Common/InterpolateVelocity.c:7-67).  The fraction split is done in
+// double; along each axis only two splits exist (at X for the components
+// not staggered along it, at X + 1/2 for the one that is), so they are shared
+// between the components.  Weights and the trilinear sum are in T.
 template <typename T, int DIM>
 __device__ __forceinline__ void mac_vel(const T* __restrict__ vel, const KVel& v,
                                         const double Xin[DIM], T V[DIM]) {
-  double X[3];
-  X[0] = clampd(Xin[0], (double)v.w.px, (double)(v.w.px + v.w.nx - 2));
-  X[1] = clampd(Xin[1], (double)v.w.py, (double)(v.w.py + v.w.ny - 2));
-  if (DIM == 3) X[2] = clampd(Xin[2], (double)v.w.pz, (double)(v.w.pz + v.w.nz - 2));
+  const int lo[3] = {v.w.px, v.w.py, v.w.pz};
+  const int nn[3] = {v.w.nx, v.w.ny, v.w.nz};
+  int b0[3], b1[3];
+  T f0[3], f1[3];
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) {
+    const double X = clampd(Xin[a], (double)lo[a], (double)(lo[a] + nn[a] - 2));
+    const double fl0 = floor(X), fl1 = floor(X + 0.5);
+    b0[a] = (int)fl0 - lo[a];
+    b1[a] = (int)fl1 - lo[a];
+    f0[a] = (T)(X - fl0);
+    f1[a] = (T)(X - fl1 + 0.5);
+  }
   const int sx = v.dof, sy = v.w.nx * v.dof, sz = v.w.ny * sy;
 #pragma unroll
   for (int c = 0; c < DIM; ++c) {
-    const double hx = c == 0 ? 0.5 : 0.0, hy = c == 1 ? 0.5 : 0.0, hz = c == 2 ? 0.5 : 0.0;
-    const double fx = floor(X[0] + hx), fy = floor(X[1] + hy);
-    const double ax = X[0] - fx + hx, ay = X[1] - fy + hy;
-    const T wx1 = (T)ax, wx0 = (T)(1.0 - ax), wy1 = (T)ay, wy0 = (T)(1.0 - ay);
-    const int xs = (int)fx - v.w.px, ys = (int)fy - v.w.py;
+    const int ix = c == 0 ? b1[0] : b0[0], iy = c == 1 ? b1[1] : b0[1];
+    const T x1 = c == 0 ? f1[0] : f0[0], y1 = c == 1 ? f1[1] : f0[1];
     if (DIM == 3) {
-      const double fz = floor(X[2] + hz);
-      const double az = X[2] - fz + hz;
-      const T wz1 = (T)az, wz0 = (T)(1.0 - az);
-      const int zs = (int)fz - v.w.pz;
-      const T* b = vel + zs * sz + ys * sy + xs * sx + v.udof + c;
-      T sum = T(0);
-      sum += wx0 * wy0 * wz0 * b[0];
-      sum += wx1 * wy0 * wz0 * b[sx];
-      sum += wx0 * wy1 * wz0 * b[sy];
-      sum += wx1 * wy1 * wz0 * b[sy + sx];
-      sum += wx0 * wy0 * wz1 * b[sz];
-      sum += wx1 * wy0 * wz1 * b[sz + sx];
-      sum += wx0 * wy1 * wz1 * b[sz + sy];
-      sum += wx1 * wy1 * wz1 * b[sz + sy + sx];
-      V[c] = sum;
+      const int iz = c == 2 ? b1[2] : b0[2];
+      const T z1 = c == 2 ? f1[2] : f0[2];
+      const T* b = vel + iz * sz + iy * sy + ix * sx + v.udof + c;
+      const T r00 = b[0] + x1 * (b[sx] - b[0]);
+      const T r10 = b[sy] + x1 * (b[sy + sx] - b[sy]);
+      const T r01 = b[sz] + x1 * (b[sz + sx] - b[sz]);
+      const T r11 = b[sz + sy] + x1 * (b[sz + sy + sx] - b[sz + sy]);
+      const T s0 = r00 + y1 * (r10 - r00), s1 = r01 + y1 * (r11 - r01);
+      V[c] = s0 + z1 * (s1 - s0);
     } else {
-      const T* b = vel + ys * sy + xs * sx + v.udof + c;
-      T sum = T(0);
-      sum += wx0 * wy0 * b[0];
-      sum += wx1 * wy0 * b[sx];
-      sum += wx0 * wy1 * b[sy];
-      sum += wx1 * wy1 * b[sy + sx];
-      V[c] = sum;
+      const T* b = vel + iy * sy + ix * sx + v.udof + c;
+      const T r0 = b[0] + x1 * (b[sx] - b[0]);
+      const T r1 = b[sy] + x1 * (b[sy + sx] - b[sy]);
+      V[c] = r0 + y1 * (r1 - r0);
     }
   }
 }
 
-// Bilinear3D with GridFunction3D_Identity (Common/Grid3D.c:4-31), order i, j, k
+// Trilinear gather of Bilinear3D with GridFunction3D_Identity
+// (Common/Grid3D.c:4-31), order i, j, k, from a base pointer with row stride
+// Y and plane stride Z (global memory or an LDS box)
 template <typename T>
-__device__ __forceinline__ T trilinear(const T* __restrict__ phi, const KWin& w, const double S[3]) {
-  double fx, fy, fz;
-  const int xs = lin_split(S[0], w.px, w.nx, fx);
-  const int ys = lin_split(S[1], w.py, w.ny, fy);
-  const int zs = lin_split(S[2], w.pz, w.nz, fz);
-  const T sx = (T)fx, sy = (T)fy, sz = (T)fz;
-  const T ox = (T)(1.0 - fx), oy = (T)(1.0 - fy), oz = (T)(1.0 - fz);
-  const int Y = w.nx, Z = w.nx * w.ny;
-  const T* b = phi + zs * Z + ys * Y + xs;
+__device__ __forceinline__ T linear_gather3(const T* b, int Y, int Z, T sx, T sy, T sz, T ox, T oy, T oz) {
   T sum = T(0);
   sum += b[0] * ox * oy * oz;
   sum += b[Z] * ox * oy * sz;
@@ -168,6 +161,17 @@ __device__ __forceinline__ T trilinear(const T* __restrict__ phi, const KWin& w,
   sum += b[1 + Y] * sx * sy * oz;
   sum += b[1 + Y + Z] * sx * sy * sz;
   return sum;
+}
+
+template <typename T>
+__device__ __forceinline__ T trilinear(const T* __restrict__ phi, const KWin& w, const double S[3]) {
+  double fx, fy, fz;
+  const int xs = lin_split(S[0], w.px, w.nx, fx);
+  const int ys = lin_split(S[1], w.py, w.ny, fy);
+  const int zs = lin_split(S[2], w.pz, w.nz, fz);
+  const int Y = w.nx, Z = w.nx * w.ny;
+  return linear_gather3<T>(phi + zs * Z + ys * Y + xs, Y, Z, (T)fx, (T)fy, (T)fz, (T)(1.0 - fx),
+                           (T)(1.0 - fy), (T)(1.0 - fz));
 }
 
 // Bilinear2D (Common/Grid.c:190-212): the (2x-1) form, x outer, y inner, /4
@@ -221,19 +225,10 @@ __device__ __forceinline__ T grid_interp(const T* __restrict__ phi, const KWin& 
   }
 }
 
-// Catmull-Rom tricubic gather over the 4x4x4 stencil around S
+// Catmull-Rom tricubic sum over a 4x4x4 stencil whose (0,0,0) corner is at b
 template <typename T>
-__device__ __forceinline__ T tricubic(const T* __restrict__ phi, const KWin& w, const double S[3]) {
-  double tx, ty, tz;
-  const int bx = cub_split(S[0], w.px, w.nx, tx);
-  const int by = cub_split(S[1], w.py, w.ny, ty);
-  const int bz = cub_split(S[2], w.pz, w.nz, tz);
-  T wx[4], wy[4], wz[4];
-  cr_weights<T>((T)tx, wx);
-  cr_weights<T>((T)ty, wy);
-  cr_weights<T>((T)tz, wz);
-  const int Y = w.nx, Z = w.nx * w.ny;
-  const T* b = phi + (bz - 1) * Z + (by - 1) * Y + (bx - 1);
+__device__ __forceinline__ T cubic_gather3(const T* b, int Y, int Z, const T wx[4], const T wy[4],
+                                           const T wz[4]) {
   T sum = T(0);
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -251,6 +246,21 @@ __device__ __forceinline__ T tricubic(const T* __restrict__ phi, const KWin& w, 
     sum += wz[k] * sy;
   }
   return sum;
+}
+
+// Catmull-Rom tricubic gather over the 4x4x4 stencil around S
+template <typename T>
+__device__ __forceinline__ T tricubic(const T* __restrict__ phi, const KWin& w, const double S[3]) {
+  double tx, ty, tz;
+  const int bx = cub_split(S[0], w.px, w.nx, tx);
+  const int by = cub_split(S[1], w.py, w.ny, ty);
+  const int bz = cub_split(S[2], w.pz, w.nz, tz);
+  T wx[4], wy[4], wz[4];
+  cr_weights<T>((T)tx, wx);
+  cr_weights<T>((T)ty, wy);
+  cr_weights<T>((T)tz, wz);
+  const int Y = w.nx, Z = w.nx * w.ny;
+  return cubic_gather3<T>(phi + (bz - 1) * Z + (by - 1) * Y + (bx - 1), Y, Z, wx, wy, wz);
 }
 
 template <typename T>

@@ -11,38 +11,27 @@
 // a contiguous x-run of phi0 and velocity; blocks are numbered so that the
 // eight XCDs each take a contiguous run of z-slabs (their L2s then hold the
 // slabs their gathers revisit).  Band lists: one thread per band node.
+#include <stdlib.h>
+#include <string.h>
+
 #include "dcl_host.hpp"
+#include "dcl_sl.hpp"
 
 namespace dcl {
 
-struct SLP {
-  KWin w;
-  KVel v;
-  double c[3];   // dt / dh per axis
-  double ch[3];  // dt / (2 dh) per axis
-};
+int sl_tiled_launch(const void* phi0, void* phi, const void* vel, const SLP& P, int precision, int interp,
+                    int bt, hipStream_t s);
 
-template <typename T, int DIM, int INTERP, int BT>
-__device__ __forceinline__ T sl_point(const T* __restrict__ phi0, const T* __restrict__ vel,
-                                      const SLP& P, int X, int Y, int Z) {
-  T V[DIM];
-  node_vel<T, DIM>(vel, P.v, X, Y, Z, V);
-  const double Xd[3] = {(double)X, (double)Y, (double)Z};
-  double S[3] = {0, 0, 0};
-  if (BT == DCL_RK2) {
-    double Xm[3] = {0, 0, 0};
-#pragma unroll
-    for (int a = 0; a < DIM; ++a) Xm[a] = Xd[a] - (double)V[a] * P.ch[a];
-    T V2[DIM];
-    mac_vel<T, DIM>(vel, P.v, Xm, V2);
-#pragma unroll
-    for (int a = 0; a < DIM; ++a) S[a] = Xd[a] - (double)V2[a] * P.c[a];
-  } else {
-#pragma unroll
-    for (int a = 0; a < DIM; ++a) S[a] = Xd[a] - (double)V[a] * P.c[a];
+// Dense 3D windows use the LDS-tiled kernel (sl_tiled.hip); setting
+// DCL_DENSE_KERNEL=gather selects the plain gather kernel below instead, for
+// comparisons.
+static bool use_tiled() {
+  static int mode = -1;
+  if (mode < 0) {
+    const char* e = getenv("DCL_DENSE_KERNEL");
+    mode = (e && strcmp(e, "gather") == 0) ? 0 : 1;
   }
-  if (INTERP == DCL_CUBIC) return DIM == 3 ? tricubic<T>(phi0, P.w, S) : bicubic<T>(phi0, P.w, S);
-  return DIM == 3 ? trilinear<T>(phi0, P.w, S) : bilinear_ref2d<T>(phi0, P.w, S);
+  return mode == 1;
 }
 
 // Map the hardware block id so that consecutive ids on one XCD (ids are dealt
@@ -153,6 +142,8 @@ int sl_launch(const void* phi0, void* phi, const dcl_window* win, const dcl_velo
   const int dim = win->n.z < 2 ? 2 : 3;
   SLP P;
   sl_params(win, vel, dt, P);
+  if (!band && dim == 3 && use_tiled())
+    return sl_tiled_launch(phi0, phi, vel->data, P, precision, interp, backtrace, s);
   if (precision == DCL_F64)
     return dispatch_sl<double>(dim, interp, backtrace, phi0, phi, vel->data, P, band, band_len,
                                band_values, s);

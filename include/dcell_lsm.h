@@ -209,6 +209,44 @@ int dcl_interface_metrics(const void *phi, const void *kappa, const dcl_window *
 int dcl_error_norms(const void *a, int prec_a, const void *b, int prec_b, int64_t n,
                     double out[4], void *stream);
 
+/* ---------------------------------------------------------------- batches */
+/* Many independent 3D level sets (the reference's DCells, which each rank
+ * advects one after another, DWorld/DCellsArray.c:57-60) updated together:
+ * the cells' windows are packed back to back in one phi buffer, one kappa
+ * buffer and one velocity buffer (3 components per node, as a velgrid), cut
+ * into tiles, and one step is two persistent launches over the packed tile
+ * list (SL then curvature, as dcl_levelset_step per cell).  No counterpart in
+ * the reference.  The velocity of each cell shares the cell's window. */
+typedef struct dcl_batch_s *dcl_batch;
+
+int dcl_batch_create(int32_t ncells, const dcl_window *wins, int precision, int interp,
+                     int backtrace, dcl_batch *batch);
+int dcl_batch_destroy(dcl_batch batch);
+/* element offsets of every cell in the packed phi/kappa buffer and in the
+ * packed velocity buffer; ncells + 1 entries each (the last is the total) */
+int dcl_batch_layout(dcl_batch batch, int64_t *phi_offsets, int64_t *vel_offsets);
+/* the step size of every cell (host array of ncells); required before stepping */
+int dcl_batch_set_dt(dcl_batch batch, const double *dt);
+/* tile shape (0..5) and resident blocks per CU (0..2 -> 2, 4, 8) of the launches */
+int dcl_batch_set_config(dcl_batch batch, int shape, int occupancy);
+int dcl_batch_get_config(dcl_batch batch, int *shape, int *occupancy, int32_t tile[3],
+                         int *blocks);
+/* one step phi0 -> phi, kappa(phi) for every cell */
+int dcl_batch_step(dcl_batch batch, const void *phi0, void *phi, void *kappa, const void *vel,
+                   void *stream);
+/* Times every configuration (reps steps each, after one warm-up step) on
+ * these buffers and keeps the fastest.  ms_table (optional, 18 entries,
+ * shape-major) receives the per-step times.  Synchronous. */
+int dcl_batch_autotune(dcl_batch batch, const void *phi0, void *phi, void *kappa,
+                       const void *vel, int reps, double *ms_table, double *best_ms,
+                       void *stream);
+/* One step with these buffers recorded as an instantiated hipGraph (two
+ * kernel nodes); launch it with dcl_batch_graph_launch. */
+int dcl_batch_graph_create(dcl_batch batch, const void *phi0, void *phi, void *kappa,
+                           const void *vel, void **graph_exec);
+int dcl_batch_graph_launch(void *graph_exec, void *stream);
+int dcl_batch_graph_destroy(void *graph_exec);
+
 /* --------------------------------------------------------------- adjoint */
 /* Reverse-mode derivative of one dense 3D SL step (dcl_sl_advect, dense):
  * given lam_next = dL/dphi, accumulate lam += dL/dphi0 and gvel += dL/dvel

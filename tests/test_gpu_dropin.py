@@ -1,0 +1,227 @@
+"""The C drop-in of LevelSetMethod.h (libdcell_levelset.so) driven the way the
+reference's own drivers drive it (LevelSetMethod/tests/advectSLRK2.c,
+particlels.c): create a LevelSet, fill phi, the band and a velocity grid,
+call the entry points by their reference names, and compare with the
+oracle's restatement of the same call sequence.
+
+FP64 throughout: max |diff| <= 1e-12 max |phi|; particles <= 1e-12 of the
+window size.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle as O
+from dcell_amd import dropin as DI
+from dcell_amd import _lib as DL
+from helpers import rel_max
+
+pytestmark = pytest.mark.gpu
+
+
+def _band_of(phi, p, width, dim):
+    nodes = np.argwhere(np.abs(phi) < width)[:, ::-1].astype(np.int32)
+    band = np.zeros((len(nodes), 3), dtype=np.int32)
+    band[:, :dim] = nodes + np.array(p[:dim], dtype=np.int32)
+    if dim == 2:
+        band[:, 2] = 0
+    return band
+
+
+def test_advect_sl_rk2_half_full_star_2d(dev):
+    """advectSLRK2.c: dx = 1, a circle of radius 10 about the origin, rigid
+    rotation (-y, x) sampled on the MAC faces, RK2 half step with dt/2 and the
+    full step with dt from the same phi0."""
+    L = DI.lib()
+    dx, r = 1.0, 10.0
+    p, n = (-22, -22, 0), (44, 44, 0)
+    ls = DI.create_levelset((dx, dx, 0.0), p, n)
+    phi = DI.grid_array(ls.contents.phi)
+    J, I = np.meshgrid(np.arange(n[1]) + p[1], np.arange(n[0]) + p[0], indexing="ij")
+    phi[:] = np.sqrt(I * I + J * J) - r
+    band = _band_of(phi, p, 7.5, 2)
+    DI.set_array(ls.contents.band, band, 12)
+    vg = DI.create_grid((dx, dx, 0.0), p, n, 2)
+    vel = DI.grid_array(vg)
+    vel[..., 0] = -(J * dx)                 # u at ((i - 1/2) dx, j dx): -y
+    vel[..., 1] = I * dx                    # v at (i dx, (j - 1/2) dx): x
+    w = O.Win(p, (n[0], n[1], 1), (dx, dx, dx))
+    dt = dx / r
+    ref = phi.copy()
+    for _ in range(5):
+        DI.check(L.LevelSetAdvectSLRK2HalfStep(ls, vg, dt / 2), "half")
+        half = O.sl_advect(ref, vel, w, dt / 2, band=band)
+        ref_half = ref.copy()
+        ref_half[band[:, 1] - p[1], band[:, 0] - p[0]] = half
+        assert rel_max(phi, ref_half) <= 1e-12
+        DI.check(L.LevelSetAdvectSLRK2FullStep(ls, vg, dt), "full")
+        full = O.sl_advect(ref, vel, w, dt, band=band)       # from the same phi0
+        ref = ref_half.copy()
+        ref[band[:, 1] - p[1], band[:, 0] - p[0]] = full
+        assert rel_max(phi, ref) <= 1e-12
+    L.GridDestroy(vg)
+    L.LevelSetDestroy(ls)
+
+
+@pytest.mark.parametrize("scheme", [(0, 0), (1, 1)])
+def test_advect_and_reinit_3d_offset_velocity_window(dev, scheme):
+    """LevelSetAdvectAndReinit over a band, with a velocity grid 10x larger
+    than phi's window and offset (particlels.c), and the CFL counter fed by
+    the GPU max-velocity over the irregular nodes."""
+    L = DI.lib()
+    DI.check(L.LevelSetAdvectSLSetScheme(*scheme), "scheme")
+    h = 0.05
+    p, n = (10, -5, 3), (28, 26, 24)
+    ls = DI.create_levelset((h, h, h), p, n)
+    phi = DI.grid_array(ls.contents.phi)
+    K, J, I = np.meshgrid(*(np.arange(n[a]) + p[a] for a in (2, 1, 0)), indexing="ij")
+    c = np.array(p) + np.array(n) / 2
+    phi[:] = np.sqrt((I - c[0]) ** 2 + (J - c[1]) ** 2 + (K - c[2]) ** 2) - 8.0
+    band = _band_of(phi, p, 6.0, 3)
+    DI.set_array(ls.contents.band, band, 12)
+    vp, vn = (p[0] - 60, p[1] - 60, p[2] - 60), (150, 150, 150)
+    vg = DI.create_grid((h, h, h), vp, vn, 3)
+    vel = DI.grid_array(vg)
+    rng = np.random.default_rng(4)
+    Kv, Jv, Iv = np.meshgrid(*(np.arange(vn[a]) + vp[a] for a in (2, 1, 0)), indexing="ij")
+    for comp in range(3):
+        vel[..., comp] = 0.5 * np.sin(0.05 * (Iv + 2 * comp)) * np.cos(0.04 * Jv) + 0.2 * np.sin(0.03 * Kv)
+    nodes = np.zeros(40, dtype=DI.IrregularNodeS)
+    pick = band[rng.choice(len(band), 40, replace=False)]
+    nodes_bytes = np.zeros((40, C.sizeof(DI.IrregularNodeS) // 4), dtype=np.int32)
+    nodes_bytes[:, :3] = pick
+    DI.set_array(ls.contents.irregularNodes, nodes_bytes, C.sizeof(DI.IrregularNodeS))
+    ls.contents.CFLthres = 0.25
+    w, vw = O.Win(p, n, (h, h, h)), O.Win(vp, vn, (h, h, h))
+    dt = 0.04
+    ref = phi.copy()
+    cfl = 0.0
+    for step in range(3):
+        DI.check(L.LevelSetAdvectAndReinit(ls, vg, dt), "AdvectAndReinit")
+        vals = O.sl_advect(ref, vel, w, dt, scheme[0], scheme[1], band=band, vwin=vw)
+        ref[band[:, 2] - p[2], band[:, 1] - p[1], band[:, 0] - p[0]] = vals
+        assert rel_max(phi, ref) <= 1e-12
+        mv = O.max_velocity(vel, vw, pick)
+        assert ls.contents.maxVel == pytest.approx(mv, rel=1e-14)
+        cfl += dt * mv / h
+        if cfl > 0.25:              # threshold crossed: counters reset (Advect.c:42-48)
+            cfl = 0.0
+        assert ls.contents.CFLcount == pytest.approx(cfl, rel=1e-12, abs=1e-15)
+    DI.check(L.LevelSetAdvectSLSetScheme(0, 0), "scheme")
+    L.GridDestroy(vg)
+    L.LevelSetDestroy(ls)
+
+
+def test_curvature_entry_point(dev):
+    L = DI.lib()
+    h = 1.0 / 40
+    p, n = (-3, 4, 1), (36, 34, 30)
+    ls = DI.create_levelset((h, h, h), p, n)
+    phi = DI.grid_array(ls.contents.phi)
+    K, J, I = np.meshgrid(*(np.arange(n[a]) for a in (2, 1, 0)), indexing="ij")
+    phi[:] = np.sqrt((I - 17.3) ** 2 + (J - 16.1) ** 2 + (K - 15.2) ** 2) - 9.0
+    kg = DI.create_grid((h, h, h), p, n, 1)
+    DI.check(L.LevelSetCurvature(ls, kg), "LevelSetCurvature")
+    ref = O.curvature(phi, O.Win(p, n, (h, h, h)))
+    assert rel_max(DI.grid_array(kg), ref) <= 1e-12
+    L.GridDestroy(kg)
+    L.LevelSetDestroy(ls)
+
+
+@pytest.mark.parametrize("ref_az", [1, 0])
+def test_advect_pls_3d(dev, ref_az):
+    """LevelSetAdvectPLS (ParticleLS.c:299-329) step by step: particle RK2,
+    phi0 <- phi, SL over the band, escaped-particle correction over the band
+    (with the reference's A.z = floor(X.y) or the corrected corner), CFL, radii."""
+    L = DI.lib()
+    DI.check(L.LevelSetPLSSetReferenceCornerZ(ref_az), "corner")
+    h = 2.0 / 15
+    p, n = (-12, -12, -12), (24, 24, 24)
+    ls = DI.create_levelset((h, h, h), p, n)
+    phi = DI.grid_array(ls.contents.phi)
+    K, J, I = np.meshgrid(*(np.arange(n[a]) + p[a] for a in (2, 1, 0)), indexing="ij")
+    phi[:] = np.sqrt(I * I + J * J + K * K) - 6.0
+    band = _band_of(phi, p, 6.0, 3)
+    DI.set_array(ls.contents.band, band, 12)
+    rng = np.random.default_rng(12)
+    near = band[np.abs(phi[band[:, 2] - p[2], band[:, 1] - p[1], band[:, 0] - p[0]]) < 3]
+    P = np.zeros((len(near) * 4, 4))
+    P[:, :3] = np.repeat(near, 4, 0) + rng.uniform(-0.5, 0.5, (len(P), 3))
+    P[:, 3] = rng.choice([-1, 1], len(P)) * rng.uniform(0.1, 0.5, len(P))    # many on the wrong side
+    keep = DI.attach_particles(ls, P, 3)
+    vg = DI.create_grid((h, h, h), p, n, 3)
+    vel = DI.grid_array(vg)
+    vel[..., 0] = -0.4 * (J * h)
+    vel[..., 1] = 0.4 * (I * h)
+    vel[..., 2] = 0.1
+    w = O.Win(p, n, (h, h, h))
+    dt = 0.05
+    refP, ref = P.copy(), phi.copy()
+    for _ in range(3):
+        DI.check(L.LevelSetAdvectPLS(ls, vg, dt), "LevelSetAdvectPLS")
+        refP = O.pls_advect(refP, vel, w, (h, h, h), dt)
+        vals = O.sl_advect(ref, vel, w, dt, band=band)
+        ref[band[:, 2] - p[2], band[:, 1] - p[1], band[:, 0] - p[0]] = vals
+        ref = O.pls_correct(refP, ref, w, band, O.PLS_REF_AZ if ref_az else 0, 3)
+        refP = O.pls_adjust_radii(refP, ref, w, 0.1, 0.5, 3)
+        assert rel_max(phi, ref) <= 1e-12
+        got = DI.get_array(keep.particles, np.float64, 4)
+        assert rel_max(got[:, :3], refP[:, :3], 24.0) <= 1e-12
+        assert rel_max(got[:, 3], refP[:, 3], 0.5) <= 1e-12
+    DI.check(L.LevelSetPLSSetReferenceCornerZ(1), "corner")
+    ls.contents.pls = None
+    L.GridDestroy(vg)
+    L.LevelSetDestroy(ls)
+
+
+def test_advect_pls_rk2_2d(dev):
+    """LevelSetAdvectPLSRK2HalfStep / FullStep (ParticleLS.c:333-375): particles
+    restored from particles0 before the full step, the 2D particle advection
+    hard-coded as in the reference."""
+    L = DI.lib()
+    dx = 2.0 / 15
+    p, n = (-40, -40, 0), (80, 80, 0)
+    ls = DI.create_levelset((dx, dx, 0.0), p, n)
+    phi = DI.grid_array(ls.contents.phi)
+    J, I = np.meshgrid(np.arange(n[1]) + p[1], np.arange(n[0]) + p[0], indexing="ij")
+    phi[:] = np.sqrt(I * I + J * J) - 6.0
+    band = _band_of(phi, p, 6.0, 2)
+    DI.set_array(ls.contents.band, band, 12)
+    rng = np.random.default_rng(3)
+    P = np.zeros((3000, 4))
+    ang = rng.uniform(0, 2 * np.pi, 3000)
+    rad = 6.0 + rng.uniform(-3, 3, 3000)
+    P[:, 0], P[:, 1] = rad * np.cos(ang), rad * np.sin(ang)
+    P[:, 3] = rng.choice([-1, 1], 3000) * rng.uniform(0.1, 0.5, 3000)
+    keep = DI.attach_particles(ls, P, 2)
+    vg = DI.create_grid((dx, dx, 0.0), p, n, 2)
+    vel = DI.grid_array(vg)
+    vel[..., 0] = -(J * dx)
+    vel[..., 1] = I * dx
+    w = O.Win(p, (n[0], n[1], 1), (dx, dx, dx))
+    dt = 0.2
+    DI.check(L.LevelSetAdvectPLSRK2HalfStep(ls, vg, dt / 2), "half")
+    P1 = O.pls_advect(P, vel, w, (dx, dx, dx), dt / 2, dim=2)
+    ref = phi.copy()
+    phi0 = DI.grid_array(ls.contents.phi0).copy()
+    assert np.array_equal(phi0, np.sqrt(I * I + J * J) - 6.0)
+    half = O.sl_advect(phi0, vel, w, dt / 2, band=band)
+    ref = phi0.copy()
+    ref[band[:, 1] - p[1], band[:, 0] - p[0]] = half
+    ref = O.pls_correct(P1, ref, w, band, 0, 2)
+    assert rel_max(phi, ref) <= 1e-12
+    DI.check(L.LevelSetAdvectPLSRK2FullStep(ls, vg, dt), "full")
+    P2 = O.pls_advect(P, vel, w, (dx, dx, dx), dt, dim=2)             # from particles0
+    full = O.sl_advect(phi0, vel, w, dt, band=band)
+    ref[band[:, 1] - p[1], band[:, 0] - p[0]] = full
+    ref = O.pls_correct(P2, ref, w, band, 0, 2)
+    P2 = O.pls_adjust_radii(P2, ref, w, 0.1, 0.5, 2)
+    assert rel_max(phi, ref) <= 1e-12
+    got = DI.get_array(keep.particles, np.float64, 4)
+    assert rel_max(got[:, :2], P2[:, :2], 80.0) <= 1e-12
+    assert rel_max(got[:, 3], P2[:, 3], 0.5) <= 1e-12
+    ls.contents.pls = None
+    L.GridDestroy(vg)
+    L.LevelSetDestroy(ls)
+    DI.check(L.LevelSetGPUFinalize(), "finalize")
