@@ -191,11 +191,127 @@ def batch_run(ncells, seed, steps, warmup, interp="cubic", bt="rk2", autotune=Tr
     return out
 
 
+def pls_run(n, steps, per_cell=32, seed=3):
+    """The precision form: n^3 deformation workload with phi and velocity in
+    FP32 (departure coordinates FP64), a particle level set of `per_cell`
+    particles per cell in |phi| < 3 (seeded), per step: particle RK2, SL +
+    curvature, escaped-particle correction, radius update.  The same run in
+    FP64 gives the error figures (max / RMS phi difference, interface area,
+    mean curvature)."""
+    import torch
+    import dcell_amd as D
+    from dcell_amd import configs
+    from dcell_amd import metrics as M
+    from dcell_amd import pls as PL
+
+    win = D.GridWindow.unit_box(n)
+    dev = torch.device("cuda:0")
+    phi_np = configs.sphere_sdf(n)
+    vel_np = configs.deformation_velocity(n)
+    P_np = configs.seed_particles(phi_np, per_cell, 3.0, seed)
+    dts = [configs.deformation_dt(n, s) for s in range(steps)]
+    out = {}
+    for dtype in (torch.float32, torch.float64):
+        phi = torch.from_numpy(phi_np).to(dev, dtype)
+        vel = torch.from_numpy(vel_np).to(dev, dtype)
+        P = torch.from_numpy(P_np).to(dev, dtype)
+        other, kappa = torch.empty_like(phi), torch.empty_like(phi)
+        ws = PL.CorrectionWorkspace(win.nodes, 8 * len(P), dtype)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+        t_pa = t_sl = t_pc = t_rad = 0.0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for s in range(steps):
+            ev[0].record()
+            PL.advect_particles(P, vel, win, win.d, dts[s])
+            ev[1].record()
+            D.advect_sl(phi, vel, win, dts[s], "cubic", "rk2", out=other)
+            D.curvature(other, win, out=kappa)
+            ev[2].record()
+            PL.correct(P, other, win, ws)
+            ev[3].record()
+            PL.adjust_radii(P, other, win)
+            ev[4].record()
+            phi, other = other, phi
+            torch.cuda.synchronize()
+            t_pa += ev[0].elapsed_time(ev[1])
+            t_sl += ev[1].elapsed_time(ev[2])
+            t_pc += ev[2].elapsed_time(ev[3])
+            t_rad += ev[3].elapsed_time(ev[4])
+        wall = time.perf_counter() - t0
+        kappa = D.curvature(phi, win, out=kappa)
+        area, kint, kmean, cnt = M.interface_metrics(phi, win, kappa)
+        key = "fp32" if dtype == torch.float32 else "fp64"
+        interior = (n - 4) ** 3
+        out[key] = dict(ms_per_step=wall / steps * 1e3, sl_curv_ms=t_sl / steps, pls_advect_ms=t_pa / steps,
+                        pls_correct_ms=t_pc / steps, pls_radii_ms=t_rad / steps,
+                        sl_curv_updates_per_s=interior / (t_sl / steps * 1e-3),
+                        particle_updates_per_s=len(P) / ((t_pa + t_pc) / steps * 1e-3),
+                        area=area, mean_curvature=kmean)
+        out[key + "_phi"] = phi
+        ws.close()
+        del vel, P, other, kappa
+    a, b = out.pop("fp32_phi"), out.pop("fp64_phi")
+    mx, ss, mb, cnt = M.error_norms(a, b)
+    near = (b.abs() < 6)
+    out["errors_vs_fp64"] = dict(
+        max_abs_phi=mx, rms_phi=float(np.sqrt(ss / cnt)),
+        max_abs_phi_band=float((a.double() - b)[near].abs().max()),
+        area_rel=abs(out["fp32"]["area"] - out["fp64"]["area"]) / out["fp64"]["area"],
+        mean_curvature_rel=abs(out["fp32"]["mean_curvature"] - out["fp64"]["mean_curvature"])
+        / abs(out["fp64"]["mean_curvature"]))
+    out.update(n=n, steps=steps, particles=int(len(P_np)), per_cell=per_cell)
+    del a, b
+    torch.cuda.empty_cache()
+    return out
+
+
+def adjoint_run(n=128, steps=50, every=10):
+    """The differentiation form: steady random Fourier velocity, sphere r 0.2,
+    FP32 SL (RK2 + tricubic) for `steps` steps with a checkpoint every `every`
+    steps, loss = sum (H(phi_T) - H(phi_target))^2, gradient w.r.t. velocity
+    by the adjoint kernel.  Reports forward and backward times."""
+    import torch
+    import dcell_amd as D
+    from dcell_amd import adjoint as A
+    from dcell_amd import configs
+
+    win = D.GridWindow.unit_box(n)
+    h = 1.0 / (n - 1)
+    dev = torch.device("cuda:0")
+    phi0 = torch.from_numpy(configs.sphere_sdf(n, (0.5, 0.5, 0.5), 0.2)).to(dev, torch.float32)
+    vel_np = configs.random_fourier_velocity(n, 6, 3, 0.5, seed=5)
+    vt_np = configs.random_fourier_velocity(n, 6, 3, 0.5, seed=6)
+    dt = 0.8 * h / max(np.abs(vel_np).max(), np.abs(vt_np).max())
+    vt = torch.from_numpy(vt_np).to(dev, torch.float32)
+    tgt = phi0
+    for _ in range(steps):
+        tgt = D.advect_sl(tgt, vt, win, dt, "cubic", "rk2")
+    v = torch.from_numpy(vel_np).to(dev, torch.float32).requires_grad_(True)
+    eps = 1.5
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    phiT = A.advect_n(phi0, v, win, dt, steps, checkpoint_every=every)
+    loss = ((A.heaviside(phiT, eps) - A.heaviside(tgt, eps)) ** 2).sum()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    interior = (n - 4) ** 3
+    return dict(n=n, steps=steps, checkpoint_every=every, loss=float(loss), forward_ms=(t1 - t0) * 1e3,
+                backward_ms=(t2 - t1) * 1e3,
+                adjoint_updates_per_s=interior * steps / (t2 - t1),
+                grad_norm=float(v.grad.norm()))
+
+
 def forms(args):
     import torch
     res = {}
     res["batched_fp32_config_scheme"] = batch_run(512, 4, 100, 3, "cubic", "rk2")
     res["batched_fp32_reference_scheme"] = batch_run(512, 4, 100, 3, "linear", "euler")
+    res["precision_pls_384"] = pls_run(384, 150)
+    res["adjoint_128"] = adjoint_run(128, 50, 10)
     res["fp64_reference_scheme_256"] = dense_run(256, torch.float64, "linear", "euler", 50, 2)
     res["fp32_config_scheme_384"] = dense_run(384, torch.float32, "cubic", "rk2", 50, 2)
     res["fp32_reference_scheme_384"] = dense_run(384, torch.float32, "linear", "euler", 50, 2)

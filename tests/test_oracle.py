@@ -474,3 +474,31 @@ def test_adjoint_finite_differences(interp, bt):
         vm[idx] -= eps
         fd = (J(phi0, vp) - J(phi0, vm)) / (2 * eps)
         assert gvel[idx] == pytest.approx(fd, rel=1e-5, abs=1e-7)
+
+
+def test_pls_2d_closed_forms_and_max_velocity_2d():
+    """The 2D particle routines (ParticleLS2D.c): constant-velocity advection,
+    one escaped particle's four corners, and the 2D max |V|."""
+    n = 9
+    h = 0.1
+    w = win2(n, d=(h, h, h))
+    P = np.array([[3.3, 4.6, 0.0, -0.3]])
+    V = np.array([0.3, -0.2])
+    out = O.pls_advect(P, _const_vel(n, V, dim=2), w, (h, h, h), 0.1, dim=2)
+    assert np.allclose(out[0, :2], P[0, :2] + 0.1 * V / h, atol=1e-14)
+    phi = np.full((n, n), 1.5)
+    band = np.argwhere(np.ones((n, n)))[:, ::-1]
+    band = np.c_[band, np.zeros(len(band), dtype=int)].astype(np.int32)
+    out = O.pls_correct(P, phi, w, band, 0, 2)
+    changed = np.argwhere(out != phi)
+    assert sorted(map(tuple, changed)) == [(4, 3), (4, 4), (5, 3), (5, 4)]
+    for (y, x) in changed:
+        d = math.hypot(3.3 - x, 4.6 - y)
+        want = -(0.3 - d)
+        assert out[y, x] == pytest.approx(want if abs(want) < 1.5 else 1.5, abs=1e-15)
+    J, I = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    vel = np.stack([0.2 * J, -0.1 * I], axis=-1).astype(float)
+    nodes = np.array([[2, 3, 0], [6, 1, 0]], dtype=np.int32)
+    ref = max(math.hypot(0.5 * (vel[y, x, 0] + vel[y, x + 1, 0]), 0.5 * (vel[y, x, 1] + vel[y + 1, x, 1]))
+              for x, y, _ in nodes)
+    assert O.max_velocity(vel, w, nodes, dim=2) == pytest.approx(ref, rel=1e-15)
