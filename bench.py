@@ -28,6 +28,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+SL_KERNEL = "sl_dense<double,3,cubic,rk2,buf>"
+COPY_BYTES = 256 << 20  # bytes read (and written) by one launch of the probe's copy kernel
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sl_dense_f64_cubic_rk2_256.traffic.txt")
 
 
 def b_alg(itemsize):
@@ -47,6 +50,12 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-forms", action="store_true", help="only the headline workload")
     ap.add_argument("--cpu-steps", type=int, default=2, help="full-grid steps of the CPU baseline leg")
+    ap.add_argument("--form", choices=sorted(FORMS),
+                    help="run one form only (for profiling) and print its JSON")
+    ap.add_argument("--probe", action="store_true",
+                    help="only 5 launches of the headline SL kernel and 5 of a copy kernel, for rocprofv3 --pmc")
+    ap.add_argument("--traffic-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="write the HBM traffic summary from two rocprofv3 --pmc output folders")
     return ap.parse_args()
 
 
@@ -99,6 +108,100 @@ def dense_run(n, dtype, interp, bt, steps, warmup, deform=True):
     del phi, vel, bufs, kappa
     torch.cuda.empty_cache()
     return out
+
+
+def probe(args):
+    """A short program for the counter runs: 5 launches of the headline SL
+    kernel on the headline workload, then 5 launches of a plain copy kernel
+    (torch's elementwise neg over COPY_BYTES of FP32: reads and writes
+    COPY_BYTES), whose counter reading per byte scales the SL kernel's."""
+    import torch
+    import dcell_amd as D
+    from dcell_amd import configs
+
+    n = args.n
+    win = D.GridWindow.unit_box(n)
+    dev = torch.device("cuda:0")
+    phi = torch.from_numpy(configs.sphere_sdf(n)).to(dev, torch.float64)
+    vel = torch.from_numpy(configs.deformation_velocity(n)).to(dev, torch.float64)
+    out = torch.empty_like(phi)
+    for s in range(5):
+        D.advect_sl(phi, vel, win, configs.deformation_dt(n, s), "cubic", "rk2", out=out)
+    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    print(f"probe done: 5 x {SL_KERNEL} at {n}^3, 5 x neg over {COPY_BYTES} B", flush=True)
+
+
+def _pmc_per_kernel(folder, counter):
+    """Mean per-dispatch value of `counter` by kernel name, from the
+    counter_collection CSV files rocprofv3 wrote under `folder`."""
+    import csv
+    import glob
+    import collections
+    per = collections.defaultdict(float)
+    name = {}
+    for f in glob.glob(os.path.join(folder, "**", "*counter_collection.csv"), recursive=True):
+        with open(f) as fh:
+            for row in csv.DictReader(fh):
+                if row["Counter_Name"] != counter:
+                    continue
+                key = (f, row["Dispatch_Id"])
+                per[key] += float(row["Counter_Value"])
+                name[key] = row["Kernel_Name"]
+    acc = collections.defaultdict(list)
+    for key, v in per.items():
+        acc[name[key]].append(v)
+    return {k: sum(v) / len(v) for k, v in acc.items()}
+
+
+def traffic_summary(args):
+    """HBM bytes per launch of the headline SL kernel from a FETCH_SIZE run and
+    a WRITE_SIZE run of `--probe` (values in KiB per dispatch), each scaled by
+    COPY_BYTES / (the same counter for the copy kernel)."""
+    fetch_dir, write_dir = args.traffic_summary
+    lines = []
+    res = {}
+    for counter, folder in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        k = _pmc_per_kernel(folder, counter)
+        sl = [v for name, v in k.items() if "sl_dense" in name]
+        cp = [v for name, v in k.items() if "neg" in name]
+        if len(sl) != 1 or len(cp) != 1:
+            sys.exit(f"{folder}: expected one sl_dense and one neg kernel, found {sorted(k)}")
+        scale = COPY_BYTES / (cp[0] * 1024.0)
+        res[counter] = sl[0] * 1024.0 * scale
+        lines.append(f"{counter}: sl_dense {sl[0]:.1f} KiB/dispatch, copy {cp[0]:.1f} KiB/dispatch "
+                     f"for {COPY_BYTES} B, scale {scale:.4f}")
+    n = args.n
+    alg = (n - 4) ** 3 * b_alg(8)["sl"]
+    with open(TRAFFIC_FILE, "w") as f:
+        f.write(f"# HBM traffic of {SL_KERNEL}, {n}^3 FP64 deformation workload (RK2 + tricubic)\n")
+        f.write("# commands (from the repository root):\n")
+        f.write(f"#   rocprofv3 --pmc FETCH_SIZE -d {fetch_dir} --output-format csv -- python3 bench.py --probe --n {n}\n")
+        f.write(f"#   rocprofv3 --pmc WRITE_SIZE -d {write_dir} --output-format csv -- python3 bench.py --probe --n {n}\n")
+        f.write(f"#   python3 bench.py --n {n} --traffic-summary {fetch_dir} {write_dir}\n")
+        for ln in lines:
+            f.write(f"# {ln}\n")
+        f.write(f"n: {n}\n")
+        f.write(f"fetch_bytes_per_launch: {res['FETCH_SIZE']:.0f}\n")
+        f.write(f"write_bytes_per_launch: {res['WRITE_SIZE']:.0f}\n")
+        f.write(f"algorithmic_bytes_per_launch: {alg}\n")
+    print(open(TRAFFIC_FILE).read())
+
+
+def read_traffic(n):
+    """HBM bytes per launch of the headline SL kernel from the committed
+    counter summary, or None when there is none for this grid."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            kv = dict(ln.split(":", 1) for ln in f if ln.strip() and not ln.startswith("#"))
+    except OSError:
+        return None
+    if int(kv["n"]) != n:
+        return None
+    return float(kv["fetch_bytes_per_launch"]) + float(kv["write_bytes_per_launch"])
 
 
 def cpu_baseline(args):
@@ -305,17 +408,29 @@ def adjoint_run(n=128, steps=50, every=10):
                 grad_norm=float(v.grad.norm()))
 
 
-def forms(args):
+def _f64():
     import torch
-    res = {}
-    res["batched_fp32_config_scheme"] = batch_run(512, 4, 100, 3, "cubic", "rk2")
-    res["batched_fp32_reference_scheme"] = batch_run(512, 4, 100, 3, "linear", "euler")
-    res["precision_pls_384"] = pls_run(384, 150)
-    res["adjoint_128"] = adjoint_run(128, 50, 10)
-    res["fp64_reference_scheme_256"] = dense_run(256, torch.float64, "linear", "euler", 50, 2)
-    res["fp32_config_scheme_384"] = dense_run(384, torch.float32, "cubic", "rk2", 50, 2)
-    res["fp32_reference_scheme_384"] = dense_run(384, torch.float32, "linear", "euler", 50, 2)
-    return res
+    return torch.float64
+
+
+def _f32():
+    import torch
+    return torch.float32
+
+
+FORMS = {
+    "batched_fp32_config_scheme": lambda: batch_run(512, 4, 100, 3, "cubic", "rk2"),
+    "batched_fp32_reference_scheme": lambda: batch_run(512, 4, 100, 3, "linear", "euler"),
+    "precision_pls_384": lambda: pls_run(384, 150),
+    "adjoint_128": lambda: adjoint_run(128, 50, 10),
+    "fp64_reference_scheme_256": lambda: dense_run(256, _f64(), "linear", "euler", 50, 2),
+    "fp32_config_scheme_384": lambda: dense_run(384, _f32(), "cubic", "rk2", 50, 2),
+    "fp32_reference_scheme_384": lambda: dense_run(384, _f32(), "linear", "euler", 50, 2),
+}
+
+
+def forms(args):
+    return {name: run() for name, run in FORMS.items()}
 
 
 def main():
@@ -325,6 +440,15 @@ def main():
               "(the workload is one GPU's share; DESIGN.md describes how it would shard)",
               file=sys.stderr)
         sys.exit(2)
+    if args.traffic_summary:
+        traffic_summary(args)
+        return
+    if args.probe:
+        probe(args)
+        return
+    if args.form:
+        print(json.dumps({args.form: FORMS[args.form]()}), flush=True)
+        return
     import torch
     r = dense_run(args.n, torch.float64, "cubic", "rk2", args.steps, args.warmup)
     interior = (args.n - 4) ** 3
@@ -347,10 +471,11 @@ def main():
                                "Catmull-Rom tricubic SL + 19-point curvature, dt=h",
                    "grid": [args.n] * 3, "interior_updates_per_step": interior,
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": "sl_tiled3d<double,cubic,rk2>",
+        "roofline": {"bound": "hbm", "kernel": SL_KERNEL,
                      "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
-                     "bytes_per_launch": interior * B["sl"]},
+                     "frac": achieved / HBM_PEAK, "traffic": read_traffic(args.n),
+                     "bytes_per_launch": interior * B["sl"],
+                     "avg_launch_ms": r["sl_ms"]},
         "step_roofline_frac": r["step_roofline_frac"],
         "kernels_ms": {"sl_advect": r["sl_ms"], "curvature": r["curv_ms"]},
         "curvature_gbs": r["curv_gbs"],

@@ -16,7 +16,8 @@ import torch
 from . import _lib as L
 from .levelset import INTERP, BACKTRACE, GridWindow, _p, stream_handle
 
-NCONFIG = 18  # 6 tile shapes x 3 occupancies
+NSHAPES, NOCC = 12, 3  # 6 tile shapes x 2 SL kernels (tile gather, LDS ring), x 3 occupancies
+NCONFIG = NSHAPES * NOCC
 
 
 class Batch:
@@ -67,7 +68,7 @@ class Batch:
         best = C.c_double()
         L.call("dcl_batch_autotune", self.handle, _p(phi0), _p(phi), _p(kappa), _p(vel), reps,
                table.ctypes.data_as(C.c_void_p), C.byref(best), stream_handle())
-        return best.value, table.reshape(6, 3)
+        return best.value, table.reshape(NSHAPES, NOCC)
 
     def graph(self, phi0, phi, kappa, vel):
         g = C.c_void_p()

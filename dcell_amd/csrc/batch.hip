@@ -4,8 +4,8 @@
 // (DWorld/DCellsArray.c:57-60), each over its own small window.  Here the
 // windows are packed back to back in one buffer per field and cut into tiles;
 // a persistent kernel walks the packed tile list (each XCD a contiguous run
-// of it, for L2 locality) and runs the LDS-tiled SL step of dcl_tile.hpp on
-// every tile, then a second persistent kernel does the curvature.  One step
+// of it, for L2 locality) and runs the SL step of dcl_tile.hpp on every tile,
+// then a second persistent kernel does the curvature.  One step
 // can be recorded as a hipGraph of these two launches.  Tile shape and
 // resident blocks per CU come from a table of configurations; dcl_batch_autotune
 // times them all on the caller's buffers and keeps the fastest.
@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "dcl_host.hpp"
+#include "dcl_ring.hpp"
 #include "dcl_tile.hpp"
 
 namespace dcl {
@@ -31,9 +32,15 @@ struct BCell {
 struct TileShape {
   int tx, ty, tz;
 };
-// tile shapes of the configuration table (threads = tx * ty)
-constexpr int NSHAPES = 6;
+// tile shapes of the configuration table (threads = tx * ty).  Shapes
+// NBASE .. 2 NBASE - 1 use the same curvature tiles as shape s - NBASE with
+// the SL step over 16 x 16 x RING_ZC columns staged in an LDS ring.
+constexpr int NBASE = 6;
+constexpr int NSHAPES = 2 * NBASE;
+constexpr int RING_LIST = NBASE;  // index of the ring column list
+constexpr int RING_ZC = 32;
 __host__ __device__ constexpr TileShape shape_of(int s) {
+  s = s % NBASE;
   return s == 0 ? TileShape{32, 8, 4}
        : s == 1 ? TileShape{32, 8, 8}
        : s == 2 ? TileShape{64, 4, 4}
@@ -58,7 +65,6 @@ __global__ __launch_bounds__(shape_of(S).tx * shape_of(S).ty) void batch_sl_k(
     const T* __restrict__ phi0, T* __restrict__ phi, const T* __restrict__ vel, const BCell* __restrict__ cells,
     const int4* __restrict__ tiles, int ntiles) {
   constexpr TileShape sh = shape_of(S);
-  __shared__ T box[SLBox<T, INTERP, sh.tx, sh.ty, sh.tz>::SIZE];
   int t0, t1, slot, nslot;
   tile_range(ntiles, t0, t1, slot, nslot);
   for (int t = t0 + slot; t < t1; t += nslot) {
@@ -74,9 +80,37 @@ __global__ __launch_bounds__(shape_of(S).tx * shape_of(S).ty) void batch_sl_k(
       P.c[a] = c.c[a];
       P.ch[a] = c.ch[a];
     }
-    sl_tile<T, INTERP, BT, sh.tx, sh.ty, sh.tz>(phi0 + c.phi_off, phi + c.phi_off, vel + c.vel_off, P, tl.y,
-                                                tl.z, tl.w, box);
-    __syncthreads();
+    const unsigned nodes = (unsigned)c.nx * c.ny * c.nz;
+    const rsrc_t rp = make_rsrc(phi0 + c.phi_off, nodes * (unsigned)sizeof(T));
+    const rsrc_t rv = make_rsrc(vel + c.vel_off, 3u * nodes * (unsigned)sizeof(T));
+    sl_tile<T, INTERP, BT, sh.tx, sh.ty, sh.tz>(rp, rv, phi0 + c.phi_off, phi + c.phi_off, P, tl.y, tl.z, tl.w);
+  }
+}
+
+template <typename T, int INTERP, int BT>
+__global__ __launch_bounds__(RG_NT) void batch_ring_k(const T* __restrict__ phi0, T* __restrict__ phi,
+                                                      const T* __restrict__ vel, const BCell* __restrict__ cells,
+                                                      const int4* __restrict__ tiles, int ntiles) {
+  __shared__ T ring[RG_R * RG_PLANE];
+  int t0, t1, slot, nslot;
+  tile_range(ntiles, t0, t1, slot, nslot);
+  for (int t = t0 + slot; t < t1; t += nslot) {
+    const int4 tl = tiles[t];
+    const BCell c = cells[tl.x];
+    SLP P;
+    P.w.px = c.px; P.w.py = c.py; P.w.pz = c.pz;
+    P.w.nx = c.nx; P.w.ny = c.ny; P.w.nz = c.nz;
+    P.v.w = P.w;
+    P.v.dof = 3;
+    P.v.udof = 0;
+    for (int a = 0; a < 3; ++a) {
+      P.c[a] = c.c[a];
+      P.ch[a] = c.ch[a];
+    }
+    const unsigned nodes = (unsigned)c.nx * c.ny * c.nz;
+    const rsrc_t rp = make_rsrc(phi0 + c.phi_off, nodes * (unsigned)sizeof(T));
+    const rsrc_t rv = make_rsrc(vel + c.vel_off, 3u * nodes * (unsigned)sizeof(T));
+    ring_column<T, INTERP, BT, RING_ZC>(phi0 + c.phi_off, phi + c.phi_off, rp, rv, P, tl.y, tl.z, tl.w, ring);
   }
 }
 
@@ -130,6 +164,14 @@ static const void* curv_kernel(int s) {
 }
 
 template <typename T>
+static const void* ring_kernel(int interp, int bt) {
+  if (interp == DCL_CUBIC && bt == DCL_RK2) return (const void*)&batch_ring_k<T, DCL_CUBIC, DCL_RK2>;
+  if (interp == DCL_CUBIC) return (const void*)&batch_ring_k<T, DCL_CUBIC, DCL_EULER>;
+  if (bt == DCL_RK2) return (const void*)&batch_ring_k<T, DCL_LINEAR, DCL_RK2>;
+  return (const void*)&batch_ring_k<T, DCL_LINEAR, DCL_EULER>;
+}
+
+template <typename T>
 static const void* sl_kernel(int interp, int bt, int s) {
   if (interp == DCL_CUBIC && bt == DCL_RK2) return KTable<T, DCL_CUBIC, DCL_RK2>::sl(s);
   if (interp == DCL_CUBIC) return KTable<T, DCL_CUBIC, DCL_EULER>::sl(s);
@@ -147,16 +189,17 @@ struct dcl_batch_s {
   std::vector<BCell> host_cells;
   std::vector<long long> phi_off, vel_off;
   BCell* cells = nullptr;
-  int4* tiles[NSHAPES] = {nullptr};
-  int ntiles[NSHAPES] = {0};
+  int4* tiles[NBASE + 1] = {nullptr};  // per tile shape, then the ring column list
+  int ntiles[NBASE + 1] = {0};
   int shape = 1, occ = 1;
   int cus = 256;
   bool dt_set = false;
 };
 
+// tile list s < NBASE: tiles of shape s; s == RING_LIST: 16 x 16 x RING_ZC columns
 static int build_tiles(dcl_batch b, int s) {
   if (b->tiles[s]) return 0;
-  const TileShape sh = shape_of(s);
+  const TileShape sh = s == RING_LIST ? TileShape{RG_TX, RG_TY, RING_ZC} : shape_of(s);
   std::vector<int4> t;
   for (int c = 0; c < b->ncells; ++c) {
     const dcl_window& w = b->wins[c];
@@ -182,24 +225,54 @@ static int grid_blocks(dcl_batch b, int occ) {
   return g < 8 ? 8 : g - g % 8;
 }
 
+// the two launches of one step with configuration (s, occ)
+struct Plan {
+  const void* ksl;
+  const void* kcv;
+  dim3 grid, bsl, bcv;
+  const int4* tsl;
+  const int4* tcv;
+  int nsl, ncv;
+};
+
+static int make_plan(dcl_batch b, int s, int occ, Plan& p) {
+  const int cs = s % NBASE;
+  const bool ring = s >= NBASE;
+  DCL_TRY(build_tiles(b, cs));
+  if (ring) DCL_TRY(build_tiles(b, RING_LIST));
+  const TileShape sh = shape_of(cs);
+  const bool f64 = b->precision == DCL_F64;
+  p.grid = dim3(grid_blocks(b, occ));
+  p.bcv = dim3(sh.tx, sh.ty);
+  p.tcv = b->tiles[cs];
+  p.ncv = b->ntiles[cs];
+  p.kcv = f64 ? curv_kernel<double>(cs) : curv_kernel<float>(cs);
+  if (ring) {
+    p.ksl = f64 ? ring_kernel<double>(b->interp, b->backtrace) : ring_kernel<float>(b->interp, b->backtrace);
+    p.bsl = dim3(RG_TX, RG_TY);
+    p.tsl = b->tiles[RING_LIST];
+    p.nsl = b->ntiles[RING_LIST];
+  } else {
+    p.ksl = f64 ? sl_kernel<double>(b->interp, b->backtrace, cs)
+                : sl_kernel<float>(b->interp, b->backtrace, cs);
+    p.bsl = p.bcv;
+    p.tsl = p.tcv;
+    p.nsl = p.ncv;
+  }
+  return 0;
+}
+
 // enqueue one step with a given configuration
 static int batch_launch(dcl_batch b, int s, int occ, const void* phi0, void* phi, void* kappa, const void* vel,
                         hipStream_t st) {
-  DCL_TRY(build_tiles(b, s));
-  const TileShape sh = shape_of(s);
-  const int nt = b->ntiles[s];
-  const int4* tl = b->tiles[s];
+  Plan p;
+  DCL_TRY(make_plan(b, s, occ, p));
   const BCell* cells = b->cells;
-  dim3 grid(grid_blocks(b, occ)), block(sh.tx, sh.ty);
-  const bool f64 = b->precision == DCL_F64;
-  const void* ksl = f64 ? sl_kernel<double>(b->interp, b->backtrace, s)
-                        : sl_kernel<float>(b->interp, b->backtrace, s);
-  const void* kcv = f64 ? curv_kernel<double>(s) : curv_kernel<float>(s);
-  void* a1[] = {(void*)&phi0, (void*)&phi, (void*)&vel, (void*)&cells, (void*)&tl, (void*)&nt};
-  DCL_HIP(hipLaunchKernel(ksl, grid, block, a1, 0, st));
+  void* a1[] = {(void*)&phi0, (void*)&phi, (void*)&vel, (void*)&cells, (void*)&p.tsl, (void*)&p.nsl};
+  DCL_HIP(hipLaunchKernel(p.ksl, p.grid, p.bsl, a1, 0, st));
   const void* cphi = phi;
-  void* a2[] = {(void*)&cphi, (void*)&kappa, (void*)&cells, (void*)&tl, (void*)&nt};
-  DCL_HIP(hipLaunchKernel(kcv, grid, block, a2, 0, st));
+  void* a2[] = {(void*)&cphi, (void*)&kappa, (void*)&cells, (void*)&p.tcv, (void*)&p.ncv};
+  DCL_HIP(hipLaunchKernel(p.kcv, p.grid, p.bcv, a2, 0, st));
   return 0;
 }
 
@@ -262,7 +335,7 @@ extern "C" int dcl_batch_destroy(dcl_batch b) {
   if (!b) return 0;
   DCL_HIP(hipDeviceSynchronize());
   dcl_free(b->cells);
-  for (int s = 0; s < NSHAPES; ++s) dcl_free(b->tiles[s]);
+  for (int s = 0; s <= NBASE; ++s) dcl_free(b->tiles[s]);
   delete b;
   return 0;
 }
@@ -375,28 +448,23 @@ extern "C" int dcl_batch_graph_create(dcl_batch b, const void* phi0, void* phi, 
                                       void** graph_exec) {
   DCL_TRY(check_step(b, phi0, phi, kappa, vel));
   if (!graph_exec) return set_error(DCL_ERR_ARG_NULL, "graph_exec is NULL");
-  DCL_TRY(build_tiles(b, b->shape));
-  const int s = b->shape;
-  const TileShape sh = shape_of(s);
-  const int nt = b->ntiles[s];
-  const int4* tl = b->tiles[s];
+  Plan p;
+  DCL_TRY(make_plan(b, b->shape, b->occ, p));
   const BCell* cells = b->cells;
-  const bool f64 = b->precision == DCL_F64;
-  const void* ksl = f64 ? sl_kernel<double>(b->interp, b->backtrace, s) : sl_kernel<float>(b->interp, b->backtrace, s);
-  const void* kcv = f64 ? curv_kernel<double>(s) : curv_kernel<float>(s);
   const void* cphi = phi;
-  void* a1[] = {(void*)&phi0, (void*)&phi, (void*)&vel, (void*)&cells, (void*)&tl, (void*)&nt};
-  void* a2[] = {(void*)&cphi, (void*)&kappa, (void*)&cells, (void*)&tl, (void*)&nt};
+  void* a1[] = {(void*)&phi0, (void*)&phi, (void*)&vel, (void*)&cells, (void*)&p.tsl, (void*)&p.nsl};
+  void* a2[] = {(void*)&cphi, (void*)&kappa, (void*)&cells, (void*)&p.tcv, (void*)&p.ncv};
   hipGraph_t g;
   DCL_HIP(hipGraphCreate(&g, 0));
   hipKernelNodeParams p1;
   memset(&p1, 0, sizeof(p1));
-  p1.func = const_cast<void*>(ksl);
-  p1.gridDim = dim3(grid_blocks(b, b->occ));
-  p1.blockDim = dim3(sh.tx, sh.ty);
+  p1.func = const_cast<void*>(p.ksl);
+  p1.gridDim = p.grid;
+  p1.blockDim = p.bsl;
   p1.kernelParams = a1;
   hipKernelNodeParams p2 = p1;
-  p2.func = const_cast<void*>(kcv);
+  p2.func = const_cast<void*>(p.kcv);
+  p2.blockDim = p.bcv;
   p2.kernelParams = a2;
   hipGraphNode_t n1, n2;
   hipGraphExec_t ex = nullptr;

@@ -1,0 +1,224 @@
+// dcl_buf.hpp -- the 3D SL node update with buffer loads.
+//
+// Every gather of the update (the node velocity, the RK2 midpoint's 3 x 8
+// MAC corners, the 4 x 4 x 4 Catmull-Rom stencil or the 2 x 2 x 2 trilinear
+// one) is a set of loads at one per-lane base offset plus offsets that are
+// the same for every lane (corner and row strides).  Buffer loads take the
+// per-lane part as a 32-bit VGPR offset and the uniform part as an SGPR
+// offset, so no per-load 64-bit address arithmetic is spent, and each
+// stencil row of 4 consecutive values is one 16-byte load (two in FP64).
+// Buffer descriptors also bound every access to the window's bytes.
+#pragma once
+
+#include "dcl_sl.hpp"
+
+namespace dcl {
+
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+// raw (untyped) buffer of `bytes` bytes starting at p
+__device__ __forceinline__ rsrc_t make_rsrc(const void* p, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+
+// The loaded dwords are reinterpreted as a whole vector.  (hipcc 7.2 loads
+// only the first dword when __builtin_bit_cast is applied to a single element
+// of the returned vector, so that form is not used.)
+template <typename T>
+struct BL;
+
+template <>
+struct BL<float> {
+  static __device__ __forceinline__ float one(rsrc_t r, int v, int s) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, v, s, 0));
+  }
+  static __device__ __forceinline__ void two(rsrc_t r, int v, int s, float o[2]) {
+    const f32x2 x = __builtin_bit_cast(f32x2, __builtin_amdgcn_raw_buffer_load_b64(r, v, s, 0));
+    o[0] = x.x;
+    o[1] = x.y;
+  }
+  static __device__ __forceinline__ void four(rsrc_t r, int v, int s, float o[4]) {
+    const f32x4 x = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, v, s, 0));
+    o[0] = x.x;
+    o[1] = x.y;
+    o[2] = x.z;
+    o[3] = x.w;
+  }
+};
+
+template <>
+struct BL<double> {
+  static __device__ __forceinline__ double one(rsrc_t r, int v, int s) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, v, s, 0));
+  }
+  static __device__ __forceinline__ void two(rsrc_t r, int v, int s, double o[2]) {
+    const f64x2 x = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(r, v, s, 0));
+    o[0] = x.x;
+    o[1] = x.y;
+  }
+  static __device__ __forceinline__ void four(rsrc_t r, int v, int s, double o[4]) {
+    const f64x2 x = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(r, v, s, 0));
+    const f64x2 y = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(r, v + 16, s, 0));
+    o[0] = x.x;
+    o[1] = x.y;
+    o[2] = y.x;
+    o[3] = y.y;
+  }
+};
+
+// node velocity (see node_vel): the two faces around the node per component
+template <typename T>
+__device__ __forceinline__ void node_vel_buf(rsrc_t rv, const KVel& v, int X, int Y, int Z, T V[3]) {
+  const int x = clampi(X, v.w.px, v.w.px + v.w.nx - 2) - v.w.px;
+  const int y = clampi(Y, v.w.py, v.w.py + v.w.ny - 2) - v.w.py;
+  const int z = clampi(Z, v.w.pz, v.w.pz + v.w.nz - 2) - v.w.pz;
+  const int sx = v.dof, sy = v.w.nx * v.dof, sz = v.w.ny * sy;
+  const int e = (int)sizeof(T);
+  const int vo = (((z * v.w.ny + y) * v.w.nx + x) * v.dof + v.udof) * e;
+  V[0] = T(0.5) * BL<T>::one(rv, vo, 0) + T(0.5) * BL<T>::one(rv, vo, sx * e);
+  V[1] = T(0.5) * BL<T>::one(rv, vo + e, 0) + T(0.5) * BL<T>::one(rv, vo + e, sy * e);
+  V[2] = T(0.5) * BL<T>::one(rv, vo + 2 * e, 0) + T(0.5) * BL<T>::one(rv, vo + 2 * e, sz * e);
+}
+
+// MAC trilinear velocity at an arbitrary point (see mac_vel)
+template <typename T>
+__device__ __forceinline__ void mac_vel_buf(rsrc_t rv, const KVel& v, const double Xin[3], T V[3]) {
+  const int lo[3] = {v.w.px, v.w.py, v.w.pz};
+  const int nn[3] = {v.w.nx, v.w.ny, v.w.nz};
+  int b0[3], b1[3];
+  T f0[3], f1[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double X = clampd(Xin[a], (double)lo[a], (double)(lo[a] + nn[a] - 2));
+    const double fl0 = floor(X), fl1 = floor(X + 0.5);
+    b0[a] = (int)fl0 - lo[a];
+    b1[a] = (int)fl1 - lo[a];
+    f0[a] = (T)(X - fl0);
+    f1[a] = (T)(X - fl1 + 0.5);
+  }
+  const int sx = v.dof, sy = v.w.nx * v.dof, sz = v.w.ny * sy;
+  const int e = (int)sizeof(T);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int ix = c == 0 ? b1[0] : b0[0], iy = c == 1 ? b1[1] : b0[1], iz = c == 2 ? b1[2] : b0[2];
+    const T x1 = c == 0 ? f1[0] : f0[0], y1 = c == 1 ? f1[1] : f0[1], z1 = c == 2 ? f1[2] : f0[2];
+    const int vo = (iz * sz + iy * sy + ix * sx + v.udof + c) * e;
+    const T a000 = BL<T>::one(rv, vo, 0), a100 = BL<T>::one(rv, vo, sx * e);
+    const T a010 = BL<T>::one(rv, vo, sy * e), a110 = BL<T>::one(rv, vo, (sy + sx) * e);
+    const T a001 = BL<T>::one(rv, vo, sz * e), a101 = BL<T>::one(rv, vo, (sz + sx) * e);
+    const T a011 = BL<T>::one(rv, vo, (sz + sy) * e), a111 = BL<T>::one(rv, vo, (sz + sy + sx) * e);
+    const T r00 = a000 + x1 * (a100 - a000), r10 = a010 + x1 * (a110 - a010);
+    const T r01 = a001 + x1 * (a101 - a001), r11 = a011 + x1 * (a111 - a011);
+    const T s0 = r00 + y1 * (r10 - r00), s1 = r01 + y1 * (r11 - r01);
+    V[c] = s0 + z1 * (s1 - s0);
+  }
+}
+
+// departure point of node (X, Y, Z) (see sl_departure) with buffer loads
+template <typename T, int BT>
+__device__ __forceinline__ void sl_dep3_buf(rsrc_t rv, const SLP& P, int X, int Y, int Z, double S[3]) {
+  T V[3];
+  node_vel_buf<T>(rv, P.v, X, Y, Z, V);
+  const double Xd[3] = {(double)X, (double)Y, (double)Z};
+  if (BT == DCL_RK2) {
+    double Xm[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) Xm[a] = Xd[a] - (double)V[a] * P.ch[a];
+    T V2[3];
+    mac_vel_buf<T>(rv, P.v, Xm, V2);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) S[a] = Xd[a] - (double)V2[a] * P.c[a];
+  } else {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) S[a] = Xd[a] - (double)V[a] * P.c[a];
+  }
+}
+
+// Catmull-Rom sum over 16 rows of 4 values; row(j, k) yields the row at
+// stencil offsets (y, z) = (j, k)
+template <typename T, typename Row>
+__device__ __forceinline__ T cubic_sum(const T wx[4], const T wy[4], const T wz[4], Row row) {
+  T sum = T(0);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    T sy = T(0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      T r[4];
+      row(j, k, r);
+      T sx = T(0);
+      sx += wx[0] * r[0];
+      sx += wx[1] * r[1];
+      sx += wx[2] * r[2];
+      sx += wx[3] * r[3];
+      sy += wy[j] * sx;
+    }
+    sum += wz[k] * sy;
+  }
+  return sum;
+}
+
+// Bilinear3D (Common/Grid3D.c:17-28) sum in the order i, j, k over the
+// [y][z] rows of (x, x+1)
+template <typename T>
+__device__ __forceinline__ T linear_sum(const T a00[2], const T a01[2], const T a10[2], const T a11[2], T sx,
+                                        T sy, T sz, T ox, T oy, T oz) {
+  T sum = T(0);
+  sum += a00[0] * ox * oy * oz;
+  sum += a01[0] * ox * oy * sz;
+  sum += a10[0] * ox * sy * oz;
+  sum += a11[0] * ox * sy * sz;
+  sum += a00[1] * sx * oy * oz;
+  sum += a01[1] * sx * oy * sz;
+  sum += a10[1] * sx * sy * oz;
+  sum += a11[1] * sx * sy * sz;
+  return sum;
+}
+
+// gather of phi0 at the departure point S with buffer loads
+template <typename T, int INTERP>
+__device__ __forceinline__ T gather3_buf(rsrc_t rp, const KWin& w, const double S[3]) {
+  const int Yr = w.nx, Zr = w.nx * w.ny;
+  const int e = (int)sizeof(T);
+  double fx, fy, fz;
+  if (INTERP == DCL_CUBIC) {
+    const int bx = cub_split(S[0], w.px, w.nx, fx);
+    const int by = cub_split(S[1], w.py, w.ny, fy);
+    const int bz = cub_split(S[2], w.pz, w.nz, fz);
+    T wx[4], wy[4], wz[4];
+    cr_weights<T>((T)fx, wx);
+    cr_weights<T>((T)fy, wy);
+    cr_weights<T>((T)fz, wz);
+    const int vo = ((bz - 1) * Zr + (by - 1) * Yr + (bx - 1)) * e;
+    return cubic_sum<T>(wx, wy, wz, [&](int j, int k, T r[4]) { BL<T>::four(rp, vo, (k * Zr + j * Yr) * e, r); });
+  } else {
+    const int bx = lin_split(S[0], w.px, w.nx, fx);
+    const int by = lin_split(S[1], w.py, w.ny, fy);
+    const int bz = lin_split(S[2], w.pz, w.nz, fz);
+    const int vo = (bz * Zr + by * Yr + bx) * e;
+    T a00[2], a01[2], a10[2], a11[2];
+    BL<T>::two(rp, vo, 0, a00);
+    BL<T>::two(rp, vo, Zr * e, a01);
+    BL<T>::two(rp, vo, Yr * e, a10);
+    BL<T>::two(rp, vo, (Yr + Zr) * e, a11);
+    return linear_sum<T>(a00, a01, a10, a11, (T)fx, (T)fy, (T)fz, (T)(1.0 - fx), (T)(1.0 - fy), (T)(1.0 - fz));
+  }
+}
+
+// the full 3D node update (sl_point<T, 3, INTERP, BT>) with buffer loads
+template <typename T, int INTERP, int BT>
+__device__ __forceinline__ T sl_point3_buf(rsrc_t rp, rsrc_t rv, const SLP& P, int X, int Y, int Z) {
+  double S[3];
+  sl_dep3_buf<T, BT>(rv, P, X, Y, Z, S);
+  return gather3_buf<T, INTERP>(rp, P.w, S);
+}
+
+// windows whose bytes a buffer descriptor can span
+__host__ inline bool buf_ok(long long phi_bytes, long long vel_bytes) {
+  return phi_bytes < 0xFFFFFFFFll && vel_bytes < 0xFFFFFFFFll;
+}
+
+}  // namespace dcl

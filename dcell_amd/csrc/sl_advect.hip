@@ -14,25 +14,10 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "dcl_ring.hpp"
 #include "dcl_host.hpp"
-#include "dcl_sl.hpp"
 
 namespace dcl {
-
-int sl_tiled_launch(const void* phi0, void* phi, const void* vel, const SLP& P, int precision, int interp,
-                    int bt, hipStream_t s);
-
-// Dense 3D windows use the LDS-tiled kernel (sl_tiled.hip); setting
-// DCL_DENSE_KERNEL=gather selects the plain gather kernel below instead, for
-// comparisons.
-static bool use_tiled() {
-  static int mode = -1;
-  if (mode < 0) {
-    const char* e = getenv("DCL_DENSE_KERNEL");
-    mode = (e && strcmp(e, "gather") == 0) ? 0 : 1;
-  }
-  return mode == 1;
-}
 
 // Map the hardware block id so that consecutive ids on one XCD (ids are dealt
 // round-robin over the 8 XCDs) walk consecutive tiles.
@@ -42,10 +27,10 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned hw, unsigned nblocks) {
   return (hw % 8u) * per + hw / 8u;
 }
 
-template <typename T, int DIM, int INTERP, int BT>
+template <typename T, int DIM, int INTERP, int BT, bool BUF>
 __global__ __launch_bounds__(256) void sl_dense(const T* __restrict__ phi0, T* __restrict__ phi,
                                                 const T* __restrict__ vel, SLP P, unsigned tx,
-                                                unsigned ty) {
+                                                unsigned ty, unsigned pbytes, unsigned vbytes) {
   const unsigned nb = tx * ty * gridDim.y;
   const unsigned lb = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, nb);
   const unsigned bx = lb % tx, by = (lb / tx) % ty, bz = lb / (tx * ty);
@@ -60,17 +45,26 @@ __global__ __launch_bounds__(256) void sl_dense(const T* __restrict__ phi0, T* _
     phi[id] = phi0[id];
     return;
   }
-  phi[id] = sl_point<T, DIM, INTERP, BT>(phi0, vel, P, P.w.px + i, P.w.py + j, P.w.pz + k);
+  if (BUF && DIM == 3)
+    phi[id] = sl_point3_buf<T, INTERP, BT>(make_rsrc(phi0, pbytes), make_rsrc(vel, vbytes), P, P.w.px + i,
+                                           P.w.py + j, P.w.pz + k);
+  else
+    phi[id] = sl_point<T, DIM, INTERP, BT>(phi0, vel, P, P.w.px + i, P.w.py + j, P.w.pz + k);
 }
 
-template <typename T, int DIM, int INTERP, int BT, bool COMPACT>
+template <typename T, int DIM, int INTERP, int BT, bool COMPACT, bool BUF>
 __global__ __launch_bounds__(256) void sl_band(const T* __restrict__ phi0, T* __restrict__ out,
                                                const T* __restrict__ vel, SLP P,
-                                               const dcl_icoor* __restrict__ band, int64_t n) {
+                                               const dcl_icoor* __restrict__ band, int64_t n,
+                                               unsigned pbytes, unsigned vbytes) {
   const int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (b >= n) return;
   const dcl_icoor q = band[b];
-  const T v = sl_point<T, DIM, INTERP, BT>(phi0, vel, P, q.x, q.y, DIM == 3 ? q.z : P.w.pz);
+  T v;
+  if (BUF && DIM == 3)
+    v = sl_point3_buf<T, INTERP, BT>(make_rsrc(phi0, pbytes), make_rsrc(vel, vbytes), P, q.x, q.y, q.z);
+  else
+    v = sl_point<T, DIM, INTERP, BT>(phi0, vel, P, q.x, q.y, DIM == 3 ? q.z : P.w.pz);
   if (COMPACT) {
     out[b] = v;
   } else {
@@ -80,30 +74,62 @@ __global__ __launch_bounds__(256) void sl_band(const T* __restrict__ phi0, T* __
   }
 }
 
-template <typename T, int DIM, int INTERP, int BT>
-static int launch_sl(const void* phi0, void* phi, const void* vel, const SLP& P,
-                     const dcl_icoor* band, int64_t band_len, void* band_values,
-                     hipStream_t s) {
+// dense 3D windows: phi0 staged in an LDS ring (sl_ring3d) or gathered from
+// global memory per node (sl_dense); DCL_SL_KERNEL=ring|gather overrides the
+// default, read at every call
+static bool sl_ring_selected() {
+  const char* e = getenv("DCL_SL_KERNEL");
+  if (e && strcmp(e, "gather") == 0) return false;
+  if (e && strcmp(e, "ring") == 0) return true;
+  return false;
+}
+
+template <typename T, int DIM, int INTERP, int BT, bool BUF>
+static int launch_sl_b(const void* phi0, void* phi, const void* vel, const SLP& P,
+                       const dcl_icoor* band, int64_t band_len, void* band_values, unsigned pb,
+                       unsigned vb, hipStream_t s) {
   const T* p0 = static_cast<const T*>(phi0);
   const T* v = static_cast<const T*>(vel);
+  if (!band && BUF && DIM == 3 && sl_ring_selected()) {
+    constexpr int ZC = 32;
+    const unsigned tx = (P.w.nx + RG_TX - 1) / RG_TX, ty = (P.w.ny + RG_TY - 1) / RG_TY;
+    const unsigned nzc = (P.w.nz + ZC - 1) / ZC;
+    hipLaunchKernelGGL((sl_ring3d<T, INTERP, BT, ZC>), dim3(tx * ty * nzc), dim3(RG_TX, RG_TY), 0, s, p0,
+                       static_cast<T*>(phi), v, P, tx, ty, nzc, pb, vb);
+    DCL_LAUNCHED("sl_ring3d");
+    return 0;
+  }
   if (!band) {
     const unsigned tx = (P.w.nx + 63) / 64, ty = (P.w.ny + 3) / 4;
     dim3 grid(tx * ty, DIM == 3 ? P.w.nz : 1);
-    hipLaunchKernelGGL((sl_dense<T, DIM, INTERP, BT>), grid, dim3(64, 4), 0, s, p0,
-                       static_cast<T*>(phi), v, P, tx, ty);
+    hipLaunchKernelGGL((sl_dense<T, DIM, INTERP, BT, BUF>), grid, dim3(64, 4), 0, s, p0,
+                       static_cast<T*>(phi), v, P, tx, ty, pb, vb);
     DCL_LAUNCHED("sl_dense");
     return 0;
   }
   if (band_len == 0) return 0;
   const unsigned g = grid1d(band_len, 256);
   if (band_values)
-    hipLaunchKernelGGL((sl_band<T, DIM, INTERP, BT, true>), dim3(g), dim3(256), 0, s, p0,
-                       static_cast<T*>(band_values), v, P, band, band_len);
+    hipLaunchKernelGGL((sl_band<T, DIM, INTERP, BT, true, BUF>), dim3(g), dim3(256), 0, s, p0,
+                       static_cast<T*>(band_values), v, P, band, band_len, pb, vb);
   else
-    hipLaunchKernelGGL((sl_band<T, DIM, INTERP, BT, false>), dim3(g), dim3(256), 0, s, p0,
-                       static_cast<T*>(phi), v, P, band, band_len);
+    hipLaunchKernelGGL((sl_band<T, DIM, INTERP, BT, false, BUF>), dim3(g), dim3(256), 0, s, p0,
+                       static_cast<T*>(phi), v, P, band, band_len, pb, vb);
   DCL_LAUNCHED("sl_band");
   return 0;
+}
+
+// 3D windows whose bytes fit a buffer descriptor take the buffer-load path
+template <typename T, int DIM, int INTERP, int BT>
+static int launch_sl(const void* phi0, void* phi, const void* vel, const SLP& P,
+                     const dcl_icoor* band, int64_t band_len, void* band_values,
+                     hipStream_t s) {
+  const long long pb = (long long)P.w.nx * P.w.ny * P.w.nz * (long long)sizeof(T);
+  const long long vb = (long long)P.v.w.nx * P.v.w.ny * P.v.w.nz * P.v.dof * (long long)sizeof(T);
+  if (DIM == 3 && buf_ok(pb, vb))
+    return launch_sl_b<T, DIM, INTERP, BT, true>(phi0, phi, vel, P, band, band_len, band_values,
+                                                  (unsigned)pb, (unsigned)vb, s);
+  return launch_sl_b<T, DIM, INTERP, BT, false>(phi0, phi, vel, P, band, band_len, band_values, 0, 0, s);
 }
 
 template <typename T>
@@ -142,8 +168,6 @@ int sl_launch(const void* phi0, void* phi, const dcl_window* win, const dcl_velo
   const int dim = win->n.z < 2 ? 2 : 3;
   SLP P;
   sl_params(win, vel, dt, P);
-  if (!band && dim == 3 && use_tiled())
-    return sl_tiled_launch(phi0, phi, vel->data, P, precision, interp, backtrace, s);
   if (precision == DCL_F64)
     return dispatch_sl<double>(dim, interp, backtrace, phi0, phi, vel->data, P, band, band_len,
                                band_values, s);

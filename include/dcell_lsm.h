@@ -227,7 +227,10 @@ int dcl_batch_destroy(dcl_batch batch);
 int dcl_batch_layout(dcl_batch batch, int64_t *phi_offsets, int64_t *vel_offsets);
 /* the step size of every cell (host array of ncells); required before stepping */
 int dcl_batch_set_dt(dcl_batch batch, const double *dt);
-/* tile shape (0..5) and resident blocks per CU (0..2 -> 2, 4, 8) of the launches */
+/* configuration of the launches: shape 0..5 = curvature and SL tile shape
+ * (tile[] of dcl_batch_get_config), shape 6..11 = the curvature tiles of
+ * shape - 6 with the SL step over 16 x 16 x 32 columns staged in an LDS
+ * ring; occupancy 0..2 -> 2, 4, 8 resident blocks per CU */
 int dcl_batch_set_config(dcl_batch batch, int shape, int occupancy);
 int dcl_batch_get_config(dcl_batch batch, int *shape, int *occupancy, int32_t tile[3],
                          int *blocks);
@@ -235,7 +238,7 @@ int dcl_batch_get_config(dcl_batch batch, int *shape, int *occupancy, int32_t ti
 int dcl_batch_step(dcl_batch batch, const void *phi0, void *phi, void *kappa, const void *vel,
                    void *stream);
 /* Times every configuration (reps steps each, after one warm-up step) on
- * these buffers and keeps the fastest.  ms_table (optional, 18 entries,
+ * these buffers and keeps the fastest.  ms_table (optional, 36 entries,
  * shape-major) receives the per-step times.  Synchronous. */
 int dcl_batch_autotune(dcl_batch batch, const void *phi0, void *phi, void *kappa,
                        const void *vel, int reps, double *ms_table, double *best_ms,

@@ -13,7 +13,7 @@ import torch
 import oracle as O
 import dcell_amd as D
 from dcell_amd import configs
-from dcell_amd.batch import Batch
+from dcell_amd.batch import NOCC, NSHAPES, Batch
 from helpers import owin, rel_max, rounded, to_np
 
 pytestmark = pytest.mark.gpu
@@ -41,8 +41,10 @@ def _build(dtype, interp, bt, seed=5):
 
 @pytest.mark.parametrize("dtype,tp,tk", [(torch.float64, 1e-12, 1e-12), (torch.float32, 2e-6, 5e-5)])
 @pytest.mark.parametrize("interp,bt", [("cubic", "rk2"), ("linear", "euler")])
-def test_batch_matches_per_cell_oracle(dev, dtype, tp, tk, interp, bt):
+@pytest.mark.parametrize("shape", [1, 7])   # SL by tile gathers / by LDS-ring columns
+def test_batch_matches_per_cell_oracle(dev, dtype, tp, tk, interp, bt, shape):
     b, (phi0, phi1, kap, vel), wins, phis, vels, dts = _build(dtype, interp, bt)
+    b.set_config(shape, 1)
     b.step(phi0, phi1, kap, vel)
     for c, w in enumerate(wins):
         ref = O.sl_advect(phis[c], vels[c], owin(w), dts[c], O.CUBIC if interp == "cubic" else O.LINEAR,
@@ -58,8 +60,8 @@ def test_batch_configs_identical_graph_replays_autotune(dev):
     b, (phi0, phi1, kap, vel), wins, *_ = _build(torch.float32, "cubic", "rk2")
     b.step(phi0, phi1, kap, vel)
     ref_phi, ref_kap = phi1.clone(), kap.clone()
-    for shape in range(6):
-        for occ in range(3):
+    for shape in range(NSHAPES):
+        for occ in range(NOCC):
             b.set_config(shape, occ)
             phi1.zero_()
             kap.zero_()

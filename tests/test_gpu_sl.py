@@ -31,11 +31,19 @@ def _case(dim, n, seed, p=(3, -2, 5), amp=1.0):
     return win, phi0, vel, rng
 
 
+@pytest.fixture(params=["gather", "ring"])
+def sl_kernel(request, monkeypatch):
+    """Both dense 3D SL kernels: per-node gathers from global memory, and phi0
+    staged in an LDS ring (selected by DCL_SL_KERNEL at every call)."""
+    monkeypatch.setenv("DCL_SL_KERNEL", request.param)
+    return request.param
+
+
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 @pytest.mark.parametrize("interp", ["linear", "cubic"])
 @pytest.mark.parametrize("bt", ["euler", "rk2"])
 @pytest.mark.parametrize("dim", [2, 3])
-def test_sl_dense_parity(dev, dtype, interp, bt, dim):
+def test_sl_dense_parity(dev, dtype, interp, bt, dim, sl_kernel):
     n = 40 if dim == 3 else 97
     win, phi0, vel, _ = _case(dim, n, seed=dim * 10 + len(interp) + len(bt))
     phi0, vel = rounded(phi0, dtype), rounded(vel, dtype)
@@ -75,6 +83,31 @@ def test_sl_band_parity_offset_velocity_window(dev, dtype, interp, bt, dim):
     idx = tuple((band[:, a] - win.p[a]) for a in range(dim))[::-1]
     full[idx] = ref
     assert rel_max(to_np(out), full) <= TOL[dtype]
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("interp", ["linear", "cubic"])
+@pytest.mark.parametrize("shape,amp", [((37, 23, 50), 1.0), ((16, 16, 32), 0.3), ((70, 9, 33), 3.0),
+                                       ((5, 6, 7), 1.0)])
+def test_sl_ring_equals_gather(dev, monkeypatch, dtype, interp, shape, amp):
+    """The LDS-ring kernel gives the per-node gather kernel's values bit for bit
+    (same arithmetic, stencil values read from LDS instead of memory) on
+    windows that are not multiples of the 16 x 16 x 32 block, several z
+    chunks, a flow slow enough to stay in the ring, and one fast enough to
+    leave it (those nodes gather from memory)."""
+    rng = np.random.default_rng(sum(shape))
+    nx, ny, nz = shape
+    h = 1.0 / (max(shape) - 1)
+    win = D.GridWindow((1, -4, 2), (nx, ny, nz), (h, h, h))
+    phi0 = rounded(rng.normal(size=(nz, ny, nx)), dtype)
+    vel = rounded(rng.uniform(-amp, amp, size=(nz, ny, nx, 3)), dtype)
+    outs = {}
+    for mode in ("gather", "ring"):
+        monkeypatch.setenv("DCL_SL_KERNEL", mode)
+        outs[mode] = to_np(D.advect_sl(to_dev(phi0, dtype), to_dev(vel, dtype), win, h, interp, "rk2"))
+    assert np.array_equal(outs["gather"], outs["ring"])
+    ref = O.sl_advect(phi0, vel, owin(win), h, IM[interp], O.RK2)
+    assert rel_max(outs["ring"], ref, np.max(np.abs(phi0))) <= TOL[dtype] * 4
 
 
 def test_sl_fluid_layout_udof(dev):
@@ -118,7 +151,7 @@ def test_sl_edge_cases(dev):
         D.advect_sl(t, to_dev(vel), win, 0.1, out=t)
 
 
-def test_levelset_step_deformation_c2_shape(dev):
+def test_levelset_step_deformation_c2_shape(dev, sl_kernel):
     """The C2 workload (sphere r 0.15 at 0.35, LeVeque deformation, dt = h,
     RK2 + tricubic, FP64) at 64^3 for 5 steps, and at its full 256^3 size on a
     random sample of 20000 nodes (the oracle's band mode, same arithmetic)."""
