@@ -28,9 +28,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
-SL_KERNEL = "sl_dense<double,3,cubic,rk2,buf>"
 COPY_BYTES = 256 << 20  # bytes read (and written) by one launch of the probe's copy kernel
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sl_dense_f64_cubic_rk2_256.traffic.txt")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sl_f64_cubic_rk2_256.traffic.txt")
+SL_KERNELS = ("sl_dense", "sl_ring3d")  # the dense 3D SL kernels (name prefixes)
 
 
 def b_alg(itemsize):
@@ -132,7 +132,7 @@ def probe(args):
     for _ in range(5):
         torch.neg(src, out=dst)
     torch.cuda.synchronize()
-    print(f"probe done: 5 x {SL_KERNEL} at {n}^3, 5 x neg over {COPY_BYTES} B", flush=True)
+    print(f"probe done: 5 x SL (RK2 + tricubic, FP64) at {n}^3, 5 x neg over {COPY_BYTES} B", flush=True)
 
 
 def _pmc_per_kernel(folder, counter):
@@ -164,26 +164,29 @@ def traffic_summary(args):
     fetch_dir, write_dir = args.traffic_summary
     lines = []
     res = {}
+    kname = None
     for counter, folder in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
         k = _pmc_per_kernel(folder, counter)
-        sl = [v for name, v in k.items() if "sl_dense" in name]
+        sl = [(name, v) for name, v in k.items() if any(p in name for p in SL_KERNELS)]
         cp = [v for name, v in k.items() if "neg" in name]
         if len(sl) != 1 or len(cp) != 1:
-            sys.exit(f"{folder}: expected one sl_dense and one neg kernel, found {sorted(k)}")
+            sys.exit(f"{folder}: expected one SL kernel and one neg kernel, found {sorted(k)}")
+        kname = sl[0][0]
         scale = COPY_BYTES / (cp[0] * 1024.0)
-        res[counter] = sl[0] * 1024.0 * scale
-        lines.append(f"{counter}: sl_dense {sl[0]:.1f} KiB/dispatch, copy {cp[0]:.1f} KiB/dispatch "
+        res[counter] = sl[0][1] * 1024.0 * scale
+        lines.append(f"{counter}: SL kernel {sl[0][1]:.1f} KiB/dispatch, copy kernel {cp[0]:.1f} KiB/dispatch "
                      f"for {COPY_BYTES} B, scale {scale:.4f}")
     n = args.n
     alg = (n - 4) ** 3 * b_alg(8)["sl"]
     with open(TRAFFIC_FILE, "w") as f:
-        f.write(f"# HBM traffic of {SL_KERNEL}, {n}^3 FP64 deformation workload (RK2 + tricubic)\n")
+        f.write(f"# HBM traffic of the dense SL kernel, {n}^3 FP64 deformation workload (RK2 + tricubic)\n")
         f.write("# commands (from the repository root):\n")
         f.write(f"#   rocprofv3 --pmc FETCH_SIZE -d {fetch_dir} --output-format csv -- python3 bench.py --probe --n {n}\n")
         f.write(f"#   rocprofv3 --pmc WRITE_SIZE -d {write_dir} --output-format csv -- python3 bench.py --probe --n {n}\n")
         f.write(f"#   python3 bench.py --n {n} --traffic-summary {fetch_dir} {write_dir}\n")
         for ln in lines:
             f.write(f"# {ln}\n")
+        f.write(f"kernel: {kname}\n")
         f.write(f"n: {n}\n")
         f.write(f"fetch_bytes_per_launch: {res['FETCH_SIZE']:.0f}\n")
         f.write(f"write_bytes_per_launch: {res['WRITE_SIZE']:.0f}\n")
@@ -192,16 +195,18 @@ def traffic_summary(args):
 
 
 def read_traffic(n):
-    """HBM bytes per launch of the headline SL kernel from the committed
-    counter summary, or None when there is none for this grid."""
+    """(kernel name, HBM bytes per launch) of the headline SL kernel from the
+    committed counter summary, or (None, None) when there is none for this
+    grid."""
     try:
         with open(TRAFFIC_FILE) as f:
-            kv = dict(ln.split(":", 1) for ln in f if ln.strip() and not ln.startswith("#"))
+            kv = dict((a.strip(), b.strip()) for a, b in
+                      (ln.split(":", 1) for ln in f if ln.strip() and not ln.startswith("#")))
     except OSError:
-        return None
+        return None, None
     if int(kv["n"]) != n:
-        return None
-    return float(kv["fetch_bytes_per_launch"]) + float(kv["write_bytes_per_launch"])
+        return None, None
+    return kv["kernel"], float(kv["fetch_bytes_per_launch"]) + float(kv["write_bytes_per_launch"])
 
 
 def cpu_baseline(args):
@@ -454,6 +459,7 @@ def main():
     interior = (args.n - 4) ** 3
     B = b_alg(8)
     achieved = interior * B["sl"] / (r["sl_ms"] * 1e-3)
+    kname, traffic = read_traffic(args.n)
     line = {
         "metric": "grid-point updates/s (SL advect+curvature), % HBM roofline, FP32 vs FP64",
         "value": r["updates_per_s"],
@@ -471,9 +477,9 @@ def main():
                                "Catmull-Rom tricubic SL + 19-point curvature, dt=h",
                    "grid": [args.n] * 3, "interior_updates_per_step": interior,
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": SL_KERNEL,
+        "roofline": {"bound": "hbm", "kernel": kname or "dense SL (RK2 + tricubic, FP64)",
                      "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": read_traffic(args.n),
+                     "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "bytes_per_launch": interior * B["sl"],
                      "avg_launch_ms": r["sl_ms"]},
         "step_roofline_frac": r["step_roofline_frac"],

@@ -69,7 +69,9 @@ struct BL<double> {
   }
 };
 
-// node velocity (see node_vel): the two faces around the node per component
+// node velocity (see node_vel): the two faces around the node per component.
+// With 3 FP32 components per node, (u, v, w) of the node and u of its +x
+// neighbour are one 16-byte load.
 template <typename T>
 __device__ __forceinline__ void node_vel_buf(rsrc_t rv, const KVel& v, int X, int Y, int Z, T V[3]) {
   const int x = clampi(X, v.w.px, v.w.px + v.w.nx - 2) - v.w.px;
@@ -78,12 +80,22 @@ __device__ __forceinline__ void node_vel_buf(rsrc_t rv, const KVel& v, int X, in
   const int sx = v.dof, sy = v.w.nx * v.dof, sz = v.w.ny * sy;
   const int e = (int)sizeof(T);
   const int vo = (((z * v.w.ny + y) * v.w.nx + x) * v.dof + v.udof) * e;
+  if (sizeof(T) == 4 && v.dof == 3) {
+    T q[4];
+    BL<T>::four(rv, vo, 0, q);
+    V[0] = T(0.5) * q[0] + T(0.5) * q[3];
+    V[1] = T(0.5) * q[1] + T(0.5) * BL<T>::one(rv, vo + e, sy * e);
+    V[2] = T(0.5) * q[2] + T(0.5) * BL<T>::one(rv, vo + 2 * e, sz * e);
+    return;
+  }
   V[0] = T(0.5) * BL<T>::one(rv, vo, 0) + T(0.5) * BL<T>::one(rv, vo, sx * e);
   V[1] = T(0.5) * BL<T>::one(rv, vo + e, 0) + T(0.5) * BL<T>::one(rv, vo + e, sy * e);
   V[2] = T(0.5) * BL<T>::one(rv, vo + 2 * e, 0) + T(0.5) * BL<T>::one(rv, vo + 2 * e, sz * e);
 }
 
-// MAC trilinear velocity at an arbitrary point (see mac_vel)
+// MAC trilinear velocity at an arbitrary point (see mac_vel).  With 3 FP32
+// components per node, each x pair of faces of one component is the first
+// and last value of one 16-byte load.
 template <typename T>
 __device__ __forceinline__ void mac_vel_buf(rsrc_t rv, const KVel& v, const double Xin[3], T V[3]) {
   const int lo[3] = {v.w.px, v.w.py, v.w.pz};
@@ -101,15 +113,29 @@ __device__ __forceinline__ void mac_vel_buf(rsrc_t rv, const KVel& v, const doub
   }
   const int sx = v.dof, sy = v.w.nx * v.dof, sz = v.w.ny * sy;
   const int e = (int)sizeof(T);
+  const bool quad = sizeof(T) == 4 && v.dof == 3;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const int ix = c == 0 ? b1[0] : b0[0], iy = c == 1 ? b1[1] : b0[1], iz = c == 2 ? b1[2] : b0[2];
     const T x1 = c == 0 ? f1[0] : f0[0], y1 = c == 1 ? f1[1] : f0[1], z1 = c == 2 ? f1[2] : f0[2];
     const int vo = (iz * sz + iy * sy + ix * sx + v.udof + c) * e;
-    const T a000 = BL<T>::one(rv, vo, 0), a100 = BL<T>::one(rv, vo, sx * e);
-    const T a010 = BL<T>::one(rv, vo, sy * e), a110 = BL<T>::one(rv, vo, (sy + sx) * e);
-    const T a001 = BL<T>::one(rv, vo, sz * e), a101 = BL<T>::one(rv, vo, (sz + sx) * e);
-    const T a011 = BL<T>::one(rv, vo, (sz + sy) * e), a111 = BL<T>::one(rv, vo, (sz + sy + sx) * e);
+    T a000, a100, a010, a110, a001, a101, a011, a111;
+    if (quad) {
+      T q[4];
+      BL<T>::four(rv, vo, 0, q);
+      a000 = q[0], a100 = q[3];
+      BL<T>::four(rv, vo, sy * e, q);
+      a010 = q[0], a110 = q[3];
+      BL<T>::four(rv, vo, sz * e, q);
+      a001 = q[0], a101 = q[3];
+      BL<T>::four(rv, vo, (sz + sy) * e, q);
+      a011 = q[0], a111 = q[3];
+    } else {
+      a000 = BL<T>::one(rv, vo, 0), a100 = BL<T>::one(rv, vo, sx * e);
+      a010 = BL<T>::one(rv, vo, sy * e), a110 = BL<T>::one(rv, vo, (sy + sx) * e);
+      a001 = BL<T>::one(rv, vo, sz * e), a101 = BL<T>::one(rv, vo, (sz + sx) * e);
+      a011 = BL<T>::one(rv, vo, (sz + sy) * e), a111 = BL<T>::one(rv, vo, (sz + sy + sx) * e);
+    }
     const T r00 = a000 + x1 * (a100 - a000), r10 = a010 + x1 * (a110 - a010);
     const T r01 = a001 + x1 * (a101 - a001), r11 = a011 + x1 * (a111 - a011);
     const T s0 = r00 + y1 * (r10 - r00), s1 = r01 + y1 * (r11 - r01);
@@ -135,30 +161,6 @@ __device__ __forceinline__ void sl_dep3_buf(rsrc_t rv, const SLP& P, int X, int 
 #pragma unroll
     for (int a = 0; a < 3; ++a) S[a] = Xd[a] - (double)V[a] * P.c[a];
   }
-}
-
-// Catmull-Rom sum over 16 rows of 4 values; row(j, k) yields the row at
-// stencil offsets (y, z) = (j, k)
-template <typename T, typename Row>
-__device__ __forceinline__ T cubic_sum(const T wx[4], const T wy[4], const T wz[4], Row row) {
-  T sum = T(0);
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    T sy = T(0);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      T r[4];
-      row(j, k, r);
-      T sx = T(0);
-      sx += wx[0] * r[0];
-      sx += wx[1] * r[1];
-      sx += wx[2] * r[2];
-      sx += wx[3] * r[3];
-      sy += wy[j] * sx;
-    }
-    sum += wz[k] * sy;
-  }
-  return sum;
 }
 
 // Bilinear3D (Common/Grid3D.c:17-28) sum in the order i, j, k over the

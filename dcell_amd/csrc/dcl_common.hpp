@@ -57,14 +57,44 @@ __device__ __forceinline__ int cub_split(double s, int p, int n, double& f) {
   return b - p;
 }
 
-// Catmull-Rom (Keys a = -1/2) weights of the stencil offsets -1, 0, 1, 2
+// Catmull-Rom (Keys a = -1/2) weights of the stencil offsets -1, 0, 1, 2.
+// The weights and the stencil sums below are fixed sequences of IEEE
+// operations (contraction off, fused multiply-adds written out), so every
+// kernel that inlines them -- LDS-staged or gathering from memory -- gives
+// the same bits.
 template <typename T>
 __device__ __forceinline__ void cr_weights(T t, T w[4]) {
+#pragma clang fp contract(off)
   const T t2 = t * t, t3 = t2 * t;
-  w[0] = T(0.5) * (T(2) * t2 - t3 - t);
-  w[1] = T(0.5) * (T(3) * t3 - T(5) * t2 + T(2));
-  w[2] = T(0.5) * (T(4) * t2 + t - T(3) * t3);
+  w[0] = T(0.5) * (fma(T(2), t2, -t3) - t);
+  w[1] = T(0.5) * (fma(T(3), t3, -(T(5) * t2)) + T(2));
+  w[2] = T(0.5) * (fma(T(4), t2, t) - T(3) * t3);
   w[3] = T(0.5) * (t3 - t2);
+}
+
+// Catmull-Rom tricubic sum over 16 rows of 4 values; row(j, k, r) fills r
+// with the row at stencil offsets (y, z) = (j, k).  Every product-sum is
+// contracted to an fma.
+template <typename T, typename Row>
+__device__ __forceinline__ T cubic_sum(const T wx[4], const T wy[4], const T wz[4], Row row) {
+  T sum = T(0);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    T sy = T(0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      T r[4];
+      row(j, k, r);
+      T sx = T(0);
+      sx = fma(wx[0], r[0], sx);
+      sx = fma(wx[1], r[1], sx);
+      sx = fma(wx[2], r[2], sx);
+      sx = fma(wx[3], r[3], sx);
+      sy = fma(wy[j], sx, sy);
+    }
+    sum = fma(wz[k], sy, sum);
+  }
+  return sum;
 }
 
 template <typename T>
@@ -229,23 +259,13 @@ __device__ __forceinline__ T grid_interp(const T* __restrict__ phi, const KWin& 
 template <typename T>
 __device__ __forceinline__ T cubic_gather3(const T* b, int Y, int Z, const T wx[4], const T wy[4],
                                            const T wz[4]) {
-  T sum = T(0);
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    T sy = T(0);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const T* r = b + k * Z + j * Y;
-      T sx = T(0);
-      sx += wx[0] * r[0];
-      sx += wx[1] * r[1];
-      sx += wx[2] * r[2];
-      sx += wx[3] * r[3];
-      sy += wy[j] * sx;
-    }
-    sum += wz[k] * sy;
-  }
-  return sum;
+  return cubic_sum<T>(wx, wy, wz, [&](int j, int k, T r[4]) {
+    const T* q = b + k * Z + j * Y;
+    r[0] = q[0];
+    r[1] = q[1];
+    r[2] = q[2];
+    r[3] = q[3];
+  });
 }
 
 // Catmull-Rom tricubic gather over the 4x4x4 stencil around S
@@ -276,13 +296,13 @@ __device__ __forceinline__ T bicubic(const T* __restrict__ phi, const KWin& w, c
   T sum = T(0);
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    const T* r = b + j * Y;
+    const T* q = b + j * Y;
     T sx = T(0);
-    sx += wx[0] * r[0];
-    sx += wx[1] * r[1];
-    sx += wx[2] * r[2];
-    sx += wx[3] * r[3];
-    sum += wy[j] * sx;
+    sx = fma(wx[0], q[0], sx);
+    sx = fma(wx[1], q[1], sx);
+    sx = fma(wx[2], q[2], sx);
+    sx = fma(wx[3], q[3], sx);
+    sum = fma(wy[j], sx, sum);
   }
   return sum;
 }

@@ -31,6 +31,19 @@ __host__ __device__ inline CurvR<T> curv_recips(double dx, double dy, double dz)
 // (pointers at node (i, j) of each plane, row stride Y): the 7-point
 // derivatives, the 12 edge points of the cross terms, NaN -> 0.  Divisions by
 // the spacings are multiplications by their reciprocals.
+// c / sqrt(g^3), g = |grad phi|^2: in FP64 the IEEE quotient and square root
+// of the reference expression; in FP32 c * rsq(g)^3 with the hardware
+// reciprocal square root (a few ulp; 0 * inf = NaN at g = 0 as before)
+template <typename T>
+__device__ __forceinline__ T grad_pow3_div(T c, T g) {
+  if constexpr (sizeof(T) == 4) {
+    const T q = __builtin_amdgcn_rsqf(g);
+    return c * (q * q * q);
+  } else {
+    return c / sqrt(g * g * g);
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ T curv3d_planes(const T* pm, const T* p, const T* pp, int Y, const CurvR<T>& r) {
   const T c0 = p[0];
@@ -47,7 +60,7 @@ __device__ __forceinline__ T curv3d_planes(const T* pm, const T* p, const T* pp,
   const T pyz = (pm[-Y] + pp[Y] - pp[-Y] - pm[Y]) * r.yz;
   T c = px2 * (pyy + pzz) + py2 * (pxx + pzz) + pz2 * (pxx + pyy);
   c -= T(2.) * (px * py * pxy + px * pz * pxz + py * pz * pyz);
-  c /= sqrt(ppp * ppp * ppp);
+  c = grad_pow3_div<T>(c, ppp);
   if (c != c) c = T(0);
   return c;
 }
@@ -66,7 +79,7 @@ __device__ __forceinline__ T curv2d_at(const T* __restrict__ p, int Y, T dx, T d
   const T pxx = (p[-1] - T(2.) * p[0] + p[1]) / (dx * dx);
   const T pyy = (p[-Y] - T(2.) * p[0] + p[Y]) / (dy * dy);
   const T pxy = (p[-Y - 1] + p[Y + 1] - p[Y - 1] - p[-Y + 1]) / (T(4.) * dx * dy);
-  T k = (pxx * py2 - T(2.) * px * py * pxy + pyy * px2) / sqrt((px2 + py2) * (px2 + py2) * (px2 + py2));
+  T k = grad_pow3_div<T>(pxx * py2 - T(2.) * px * py * pxy + pyy * px2, px2 + py2);
   if (k != k) k = T(0);
   return k;
 }

@@ -152,8 +152,8 @@ __device__ __forceinline__ void ring_column(const T* __restrict__ phi0, T* __res
 
 // grid: tiles_x * tiles_y * nzc blocks of RG_TX x RG_TY threads (XCD-remapped
 // so each XCD walks neighbouring columns)
-template <typename T, int INTERP, int BT, int ZC>
-__global__ __launch_bounds__(RG_NT) void sl_ring3d(const T* __restrict__ phi0, T* __restrict__ phi,
+template <typename T, int INTERP, int BT, int ZC, int WPE>
+__global__ __launch_bounds__(RG_NT) __attribute__((amdgpu_waves_per_eu(WPE))) void sl_ring3d(const T* __restrict__ phi0, T* __restrict__ phi,
                                                    const T* __restrict__ vel, SLP P, unsigned tiles_x,
                                                    unsigned tiles_y, unsigned nzc, unsigned pbytes,
                                                    unsigned vbytes) {

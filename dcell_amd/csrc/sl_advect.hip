@@ -74,14 +74,15 @@ __global__ __launch_bounds__(256) void sl_band(const T* __restrict__ phi0, T* __
   }
 }
 
-// dense 3D windows: phi0 staged in an LDS ring (sl_ring3d) or gathered from
-// global memory per node (sl_dense); DCL_SL_KERNEL=ring|gather overrides the
-// default, read at every call
-static bool sl_ring_selected() {
+// dense 3D windows: phi0 staged in an LDS ring (sl_ring3d, the default for
+// the 64-value tricubic stencil) or gathered from global memory per node
+// (sl_dense, the default for the 8-value trilinear one); DCL_SL_KERNEL=
+// ring|gather overrides the default, read at every call
+static bool sl_ring_selected(int interp) {
   const char* e = getenv("DCL_SL_KERNEL");
   if (e && strcmp(e, "gather") == 0) return false;
   if (e && strcmp(e, "ring") == 0) return true;
-  return false;
+  return interp == DCL_CUBIC;
 }
 
 template <typename T, int DIM, int INTERP, int BT, bool BUF>
@@ -90,12 +91,22 @@ static int launch_sl_b(const void* phi0, void* phi, const void* vel, const SLP& 
                        unsigned vb, hipStream_t s) {
   const T* p0 = static_cast<const T*>(phi0);
   const T* v = static_cast<const T*>(vel);
-  if (!band && BUF && DIM == 3 && sl_ring_selected()) {
+  if (!band && BUF && DIM == 3 && sl_ring_selected(INTERP)) {
     constexpr int ZC = 32;
     const unsigned tx = (P.w.nx + RG_TX - 1) / RG_TX, ty = (P.w.ny + RG_TY - 1) / RG_TY;
     const unsigned nzc = (P.w.nz + ZC - 1) / ZC;
-    hipLaunchKernelGGL((sl_ring3d<T, INTERP, BT, ZC>), dim3(tx * ty * nzc), dim3(RG_TX, RG_TY), 0, s, p0,
+    const char* we = getenv("DCL_RING_WPE");
+    const int wpe = we ? atoi(we) : 1;
+#define DCL_RING(W)                                                                                        \
+  if (wpe == W)                                                                                            \
+    hipLaunchKernelGGL((sl_ring3d<T, INTERP, BT, ZC, W>), dim3(tx * ty * nzc), dim3(RG_TX, RG_TY), 0, s, p0, \
                        static_cast<T*>(phi), v, P, tx, ty, nzc, pb, vb);
+    DCL_RING(1)
+    DCL_RING(3)
+    DCL_RING(4)
+    DCL_RING(5)
+    DCL_RING(6)
+#undef DCL_RING
     DCL_LAUNCHED("sl_ring3d");
     return 0;
   }
