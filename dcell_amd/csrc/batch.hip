@@ -34,7 +34,7 @@ struct TileShape {
 };
 // tile shapes of the configuration table (threads = tx * ty).  Shapes
 // NBASE .. 2 NBASE - 1 use the same curvature tiles as shape s - NBASE with
-// the SL step over 16 x 16 x RING_ZC columns staged in an LDS ring.
+// the SL step over 32 x 8 x RING_ZC columns staged in an LDS ring.
 constexpr int NBASE = 6;
 constexpr int NSHAPES = 2 * NBASE;
 constexpr int RING_LIST = NBASE;  // index of the ring column list
@@ -87,11 +87,18 @@ __global__ __launch_bounds__(shape_of(S).tx * shape_of(S).ty) void batch_sl_k(
   }
 }
 
+// ring columns of the batch: 32 x 8 nodes, velocity staged in FP32
+constexpr int RING_TX = 32, RING_TY = 8;
+
 template <typename T, int INTERP, int BT>
-__global__ __launch_bounds__(RG_NT) void batch_ring_k(const T* __restrict__ phi0, T* __restrict__ phi,
+__global__ __launch_bounds__(256) void batch_ring_k(const T* __restrict__ phi0, T* __restrict__ phi,
                                                       const T* __restrict__ vel, const BCell* __restrict__ cells,
                                                       const int4* __restrict__ tiles, int ntiles) {
-  __shared__ T ring[RG_R * RG_PLANE];
+  typedef RingGeom<RING_TX, RING_TY> G;
+  typedef VelGeom<RING_TX, RING_TY> V;
+  constexpr bool VL = sizeof(T) == 4;
+  __shared__ T ring[RG_R * G::PLANE];
+  __shared__ T vr[VL ? VR_R * 3 * V::PLANE : 1];
   int t0, t1, slot, nslot;
   tile_range(ntiles, t0, t1, slot, nslot);
   for (int t = t0 + slot; t < t1; t += nslot) {
@@ -110,7 +117,8 @@ __global__ __launch_bounds__(RG_NT) void batch_ring_k(const T* __restrict__ phi0
     const unsigned nodes = (unsigned)c.nx * c.ny * c.nz;
     const rsrc_t rp = make_rsrc(phi0 + c.phi_off, nodes * (unsigned)sizeof(T));
     const rsrc_t rv = make_rsrc(vel + c.vel_off, 3u * nodes * (unsigned)sizeof(T));
-    ring_column<T, INTERP, BT, RING_ZC>(phi0 + c.phi_off, phi + c.phi_off, rp, rv, P, tl.y, tl.z, tl.w, ring);
+    ring_column<T, INTERP, BT, RING_ZC, G, VL>(phi0 + c.phi_off, phi + c.phi_off, rp, rv, P, tl.y, tl.z, tl.w,
+                                               ring, vr);
   }
 }
 
@@ -119,7 +127,7 @@ __global__ __launch_bounds__(shape_of(S).tx * shape_of(S).ty) void batch_curv_k(
     const T* __restrict__ phi, T* __restrict__ kappa, const BCell* __restrict__ cells,
     const int4* __restrict__ tiles, int ntiles) {
   constexpr TileShape sh = shape_of(S);
-  __shared__ T box[CurvBox<T, sh.tx, sh.ty, sh.tz>::SIZE];
+  __shared__ T pl[4 * (sh.tx + 2) * (sh.ty + 2)];
   int t0, t1, slot, nslot;
   tile_range(ntiles, t0, t1, slot, nslot);
   for (int t = t0 + slot; t < t1; t += nslot) {
@@ -128,9 +136,9 @@ __global__ __launch_bounds__(shape_of(S).tx * shape_of(S).ty) void batch_curv_k(
     KWin w;
     w.px = c.px; w.py = c.py; w.pz = c.pz;
     w.nx = c.nx; w.ny = c.ny; w.nz = c.nz;
-    curv_tile<T, sh.tx, sh.ty, sh.tz>(phi + c.phi_off, kappa + c.phi_off, w, curv_recips<T>(c.dx, c.dy, c.dz),
-                                      tl.y, tl.z, tl.w, box);
-    __syncthreads();
+    const unsigned nodes = (unsigned)c.nx * c.ny * c.nz;
+    curv_column<T, sh.tx, sh.ty, sh.tz>(make_rsrc(phi + c.phi_off, nodes * (unsigned)sizeof(T)), kappa + c.phi_off,
+                                        w, curv_recips<T>(c.dx, c.dy, c.dz), tl.y, tl.z, tl.w, pl);
   }
 }
 
@@ -196,10 +204,10 @@ struct dcl_batch_s {
   bool dt_set = false;
 };
 
-// tile list s < NBASE: tiles of shape s; s == RING_LIST: 16 x 16 x RING_ZC columns
+// tile list s < NBASE: tiles of shape s; s == RING_LIST: 32 x 8 x RING_ZC columns
 static int build_tiles(dcl_batch b, int s) {
   if (b->tiles[s]) return 0;
-  const TileShape sh = s == RING_LIST ? TileShape{RG_TX, RG_TY, RING_ZC} : shape_of(s);
+  const TileShape sh = s == RING_LIST ? TileShape{RING_TX, RING_TY, RING_ZC} : shape_of(s);
   std::vector<int4> t;
   for (int c = 0; c < b->ncells; ++c) {
     const dcl_window& w = b->wins[c];
@@ -249,7 +257,7 @@ static int make_plan(dcl_batch b, int s, int occ, Plan& p) {
   p.kcv = f64 ? curv_kernel<double>(cs) : curv_kernel<float>(cs);
   if (ring) {
     p.ksl = f64 ? ring_kernel<double>(b->interp, b->backtrace) : ring_kernel<float>(b->interp, b->backtrace);
-    p.bsl = dim3(RG_TX, RG_TY);
+    p.bsl = dim3(RING_TX, RING_TY);
     p.tsl = b->tiles[RING_LIST];
     p.nsl = b->ntiles[RING_LIST];
   } else {

@@ -10,7 +10,7 @@
 #include <string.h>
 
 #include "dcl_host.hpp"
-#include "dcl_curv.hpp"
+#include "dcl_tile.hpp"
 
 namespace dcl {
 
@@ -53,54 +53,20 @@ __global__ __launch_bounds__(256) void curv_dense(const T* __restrict__ phi, T* 
 }
 
 // Dense 3D curvature, 2.5D tiling: a block owns a 64 x 4 x 16 column of
-// nodes; it keeps the planes k-1, k, k+1 (with a one-node halo in x and y) in
-// a rolling three-slot LDS buffer, loading one new plane per step with
-// coalesced reads, so each phi value is read from memory about once per
-// block and the 19 stencil reads come from LDS.
-constexpr int CTX = 64, CTY = 4, CTZ = 16, CPX = CTX + 2, CPY = CTY + 2;
-
-template <typename T>
-__device__ __forceinline__ void curv_load_plane(T* dst, const T* __restrict__ phi, int k, int x0, int y0,
-                                                int nx, int ny, int nz, int tid) {
-  const int kk = clampi(k, 0, nz - 1);
-  const T* src = phi + (size_t)kk * nx * ny;
-  for (int e = tid; e < CPX * CPY; e += 256) {
-    const int bx = e % CPX, by = e / CPX;
-    const int gx = clampi(x0 - 1 + bx, 0, nx - 1), gy = clampi(y0 - 1 + by, 0, ny - 1);
-    dst[e] = src[gy * nx + gx];
-  }
-}
+// nodes and walks it with curv_column (a four-plane LDS ring of phi with a
+// one-node halo, one barrier per plane).
+constexpr int CTX = 64, CTY = 4, CTZ = 16;
 
 template <typename T>
 __global__ __launch_bounds__(256) void curv_tiled3d(const T* __restrict__ phi, T* __restrict__ kappa,
-                                                    CurvP P, int ntx, int nty, int ntz) {
-  __shared__ T pl[3][CPX * CPY];
-  const int nx = P.w.nx, ny = P.w.ny, nz = P.w.nz;
+                                                    CurvP P, int ntx, int nty, int ntz, unsigned pbytes) {
+  __shared__ T pl[4 * (CTX + 2) * (CTY + 2)];
   const unsigned nb = (unsigned)(ntx * nty * ntz);
   const unsigned per = nb / 8u;
   const unsigned lb = (per == 0 || blockIdx.x >= per * 8u) ? blockIdx.x : (blockIdx.x % 8u) * per + blockIdx.x / 8u;
   const int tx = lb % ntx, ty = (lb / ntx) % nty, tz = lb / (ntx * nty);
-  const int x0 = tx * CTX, y0 = ty * CTY, z0 = tz * CTZ;
-  const int tid = threadIdx.x + CTX * threadIdx.y;
-  const CurvR<T> r = curv_recips<T>(P.dx, P.dy, P.dz);
-  curv_load_plane<T>(pl[0], phi, z0 - 1, x0, y0, nx, ny, nz, tid);
-  curv_load_plane<T>(pl[1], phi, z0, x0, y0, nx, ny, nz, tid);
-  const int i = x0 + threadIdx.x, j = y0 + threadIdx.y;
-  const bool in_xy = i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2;
-  const int lo = (threadIdx.y + 1) * CPX + threadIdx.x + 1;
-  for (int kz = 0; kz < CTZ; ++kz) {
-    const int k = z0 + kz;
-    if (k >= nz) break;  // uniform across the block
-    curv_load_plane<T>(pl[(kz + 2) % 3], phi, k + 1, x0, y0, nx, ny, nz, tid);
-    __syncthreads();
-    if (i < nx && j < ny) {
-      T c = T(0);
-      if (in_xy && k >= 1 && k <= nz - 2)
-        c = curv3d_planes<T>(pl[kz % 3] + lo, pl[(kz + 1) % 3] + lo, pl[(kz + 2) % 3] + lo, CPX, r);
-      kappa[((size_t)k * ny + j) * nx + i] = c;
-    }
-    __syncthreads();
-  }
+  curv_column<T, CTX, CTY, CTZ>(make_rsrc(phi, pbytes), kappa, P.w, curv_recips<T>(P.dx, P.dy, P.dz), tx * CTX,
+                                ty * CTY, tz * CTZ, pl);
 }
 
 template <typename T, int DIM, int ST, bool COMPACT>
@@ -121,10 +87,11 @@ template <typename T, int DIM, int ST>
 static int launch_curv(const void* phi, void* kappa, const CurvP& P, const dcl_icoor* band,
                        int64_t n, void* bv, hipStream_t s) {
   const T* p = static_cast<const T*>(phi);
-  if (!band && DIM == 3 && use_tiled_curv()) {
+  const long long pbytes = (long long)P.w.nx * P.w.ny * P.w.nz * (long long)sizeof(T);
+  if (!band && DIM == 3 && use_tiled_curv() && buf_ok(pbytes, 0)) {
     const int ntx = (P.w.nx + CTX - 1) / CTX, nty = (P.w.ny + CTY - 1) / CTY, ntz = (P.w.nz + CTZ - 1) / CTZ;
     hipLaunchKernelGGL((curv_tiled3d<T>), dim3(ntx * nty * ntz), dim3(CTX, CTY), 0, s, p,
-                       static_cast<T*>(kappa), P, ntx, nty, ntz);
+                       static_cast<T*>(kappa), P, ntx, nty, ntz, (unsigned)pbytes);
     DCL_LAUNCHED("curv_tiled3d");
     return 0;
   }

@@ -83,34 +83,22 @@ __device__ __forceinline__ void node_vel_buf(rsrc_t rv, const KVel& v, int X, in
   if (sizeof(T) == 4 && v.dof == 3) {
     T q[4];
     BL<T>::four(rv, vo, 0, q);
-    V[0] = T(0.5) * q[0] + T(0.5) * q[3];
-    V[1] = T(0.5) * q[1] + T(0.5) * BL<T>::one(rv, vo + e, sy * e);
-    V[2] = T(0.5) * q[2] + T(0.5) * BL<T>::one(rv, vo + 2 * e, sz * e);
+    V[0] = vmid(q[0], q[3]);
+    V[1] = vmid(q[1], BL<T>::one(rv, vo + e, sy * e));
+    V[2] = vmid(q[2], BL<T>::one(rv, vo + 2 * e, sz * e));
     return;
   }
-  V[0] = T(0.5) * BL<T>::one(rv, vo, 0) + T(0.5) * BL<T>::one(rv, vo, sx * e);
-  V[1] = T(0.5) * BL<T>::one(rv, vo + e, 0) + T(0.5) * BL<T>::one(rv, vo + e, sy * e);
-  V[2] = T(0.5) * BL<T>::one(rv, vo + 2 * e, 0) + T(0.5) * BL<T>::one(rv, vo + 2 * e, sz * e);
+  V[0] = vmid(BL<T>::one(rv, vo, 0), BL<T>::one(rv, vo, sx * e));
+  V[1] = vmid(BL<T>::one(rv, vo + e, 0), BL<T>::one(rv, vo + e, sy * e));
+  V[2] = vmid(BL<T>::one(rv, vo + 2 * e, 0), BL<T>::one(rv, vo + 2 * e, sz * e));
 }
 
-// MAC trilinear velocity at an arbitrary point (see mac_vel).  With 3 FP32
-// components per node, each x pair of faces of one component is the first
-// and last value of one 16-byte load.
+// MAC trilinear velocity of the split (b0, b1, f0, f1) (see mac_vel).  With 3
+// FP32 components per node, each x pair of faces of one component is the
+// first and last value of one 16-byte load.
 template <typename T>
-__device__ __forceinline__ void mac_vel_buf(rsrc_t rv, const KVel& v, const double Xin[3], T V[3]) {
-  const int lo[3] = {v.w.px, v.w.py, v.w.pz};
-  const int nn[3] = {v.w.nx, v.w.ny, v.w.nz};
-  int b0[3], b1[3];
-  T f0[3], f1[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const double X = clampd(Xin[a], (double)lo[a], (double)(lo[a] + nn[a] - 2));
-    const double fl0 = floor(X), fl1 = floor(X + 0.5);
-    b0[a] = (int)fl0 - lo[a];
-    b1[a] = (int)fl1 - lo[a];
-    f0[a] = (T)(X - fl0);
-    f1[a] = (T)(X - fl1 + 0.5);
-  }
+__device__ __forceinline__ void mac_vel_buf_split(rsrc_t rv, const KVel& v, const int b0[3], const int b1[3],
+                                                  const T f0[3], const T f1[3], T V[3]) {
   const int sx = v.dof, sy = v.w.nx * v.dof, sz = v.w.ny * sy;
   const int e = (int)sizeof(T);
   const bool quad = sizeof(T) == 4 && v.dof == 3;
@@ -136,11 +124,16 @@ __device__ __forceinline__ void mac_vel_buf(rsrc_t rv, const KVel& v, const doub
       a001 = BL<T>::one(rv, vo, sz * e), a101 = BL<T>::one(rv, vo, (sz + sx) * e);
       a011 = BL<T>::one(rv, vo, (sz + sy) * e), a111 = BL<T>::one(rv, vo, (sz + sy + sx) * e);
     }
-    const T r00 = a000 + x1 * (a100 - a000), r10 = a010 + x1 * (a110 - a010);
-    const T r01 = a001 + x1 * (a101 - a001), r11 = a011 + x1 * (a111 - a011);
-    const T s0 = r00 + y1 * (r10 - r00), s1 = r01 + y1 * (r11 - r01);
-    V[c] = s0 + z1 * (s1 - s0);
+    V[c] = vtri(a000, a100, a010, a110, a001, a101, a011, a111, x1, y1, z1);
   }
+}
+
+template <typename T>
+__device__ __forceinline__ void mac_vel_buf(rsrc_t rv, const KVel& v, const double Xin[3], T V[3]) {
+  int b0[3], b1[3];
+  T f0[3], f1[3];
+  mac_split<T, 3>(v, Xin, b0, b1, f0, f1);
+  mac_vel_buf_split<T>(rv, v, b0, b1, f0, f1, V);
 }
 
 // departure point of node (X, Y, Z) (see sl_departure) with buffer loads

@@ -106,6 +106,30 @@ __device__ __forceinline__ void cr_dweights(T t, T dw[4]) {
   dw[3] = T(0.5) * (T(3) * t2 - T(2) * t);
 }
 
+// The velocity arithmetic below is written out operation by operation
+// (contraction off, fma where written), so the kernels that read the faces
+// from memory, from LDS or through buffer loads give the same bits.
+template <typename T>
+__device__ __forceinline__ T vmid(T a, T b) {  // mean of two faces
+#pragma clang fp contract(off)
+  return fma(T(0.5), a, T(0.5) * b);
+}
+
+template <typename T>
+__device__ __forceinline__ T vlerp(T a, T b, T t) {  // a + t (b - a)
+#pragma clang fp contract(off)
+  return fma(t, b - a, a);
+}
+
+// trilinear over the corners a[z][y][x] with fractions (x1, y1, z1)
+template <typename T>
+__device__ __forceinline__ T vtri(T a000, T a100, T a010, T a110, T a001, T a101, T a011, T a111, T x1, T y1,
+                                  T z1) {
+  const T r00 = vlerp(a000, a100, x1), r10 = vlerp(a010, a110, x1);
+  const T r01 = vlerp(a001, a101, x1), r11 = vlerp(a011, a111, x1);
+  return vlerp(vlerp(r00, r10, y1), vlerp(r01, r11, y1), z1);
+}
+
 // Velocity at an integer node (the first stage of every backtrace).  With
 // the fractions 1/2 along the staggered axis and 0 elsewhere, the eight terms
 // of InterpolateVelocity3D (Common/InterpolateVelocity.c:35-67) reduce
@@ -121,28 +145,24 @@ __device__ __forceinline__ void node_vel(const T* __restrict__ vel, const KVel& 
     const int z = clampi(Z, v.w.pz, v.w.pz + v.w.nz - 2) - v.w.pz;
     const int sz = v.w.ny * sy;
     const T* b = vel + (z * v.w.ny + y) * sy + x * sx + v.udof;
-    V[0] = T(0.5) * b[0] + T(0.5) * b[sx];
-    V[1] = T(0.5) * b[1] + T(0.5) * b[sy + 1];
-    V[2] = T(0.5) * b[2] + T(0.5) * b[sz + 2];
+    V[0] = vmid(b[0], b[sx]);
+    V[1] = vmid(b[1], b[sy + 1]);
+    V[2] = vmid(b[2], b[sz + 2]);
   } else {
     const T* b = vel + y * sy + x * sx + v.udof;
-    V[0] = T(0.5) * b[0] + T(0.5) * b[sx];
-    V[1] = T(0.5) * b[1] + T(0.5) * b[sy + 1];
+    V[0] = vmid(b[0], b[sx]);
+    V[1] = vmid(b[1], b[sy + 1]);
   }
 }
 
-// MAC-staggered trilinear velocity at an arbitrary point: InterpolateVelocity3D
-// / 2D (Common/InterpolateVelocity.c:7-67).  The fraction split is done in
-// double; along each axis only two splits exist (at X for the components
-// not staggered along it, at X + 1/2 for the one that is), so they are shared
-// between the components.  Weights and the trilinear sum are in T.
+// MAC split of a point: along each axis the split at X (components not
+// staggered along it) and at X + 1/2 (the one that is), in double, after
+// clamping X into [vp, vp+vn-2]; local base indices and T fractions
 template <typename T, int DIM>
-__device__ __forceinline__ void mac_vel(const T* __restrict__ vel, const KVel& v,
-                                        const double Xin[DIM], T V[DIM]) {
+__device__ __forceinline__ void mac_split(const KVel& v, const double Xin[DIM], int b0[3], int b1[3], T f0[3],
+                                          T f1[3]) {
   const int lo[3] = {v.w.px, v.w.py, v.w.pz};
   const int nn[3] = {v.w.nx, v.w.ny, v.w.nz};
-  int b0[3], b1[3];
-  T f0[3], f1[3];
 #pragma unroll
   for (int a = 0; a < DIM; ++a) {
     const double X = clampd(Xin[a], (double)lo[a], (double)(lo[a] + nn[a] - 2));
@@ -152,6 +172,18 @@ __device__ __forceinline__ void mac_vel(const T* __restrict__ vel, const KVel& v
     f0[a] = (T)(X - fl0);
     f1[a] = (T)(X - fl1 + 0.5);
   }
+}
+
+// MAC-staggered trilinear velocity at an arbitrary point: InterpolateVelocity3D
+// / 2D (Common/InterpolateVelocity.c:7-67).  The fraction split is done in
+// double (mac_split); along each axis only two splits exist, so they are
+// shared between the components.  Weights and the trilinear sum are in T.
+template <typename T, int DIM>
+__device__ __forceinline__ void mac_vel(const T* __restrict__ vel, const KVel& v,
+                                        const double Xin[DIM], T V[DIM]) {
+  int b0[3], b1[3];
+  T f0[3], f1[3];
+  mac_split<T, DIM>(v, Xin, b0, b1, f0, f1);
   const int sx = v.dof, sy = v.w.nx * v.dof, sz = v.w.ny * sy;
 #pragma unroll
   for (int c = 0; c < DIM; ++c) {
@@ -161,24 +193,14 @@ __device__ __forceinline__ void mac_vel(const T* __restrict__ vel, const KVel& v
       const int iz = c == 2 ? b1[2] : b0[2];
       const T z1 = c == 2 ? f1[2] : f0[2];
       const T* b = vel + iz * sz + iy * sy + ix * sx + v.udof + c;
-      const T r00 = b[0] + x1 * (b[sx] - b[0]);
-      const T r10 = b[sy] + x1 * (b[sy + sx] - b[sy]);
-      const T r01 = b[sz] + x1 * (b[sz + sx] - b[sz]);
-      const T r11 = b[sz + sy] + x1 * (b[sz + sy + sx] - b[sz + sy]);
-      const T s0 = r00 + y1 * (r10 - r00), s1 = r01 + y1 * (r11 - r01);
-      V[c] = s0 + z1 * (s1 - s0);
+      V[c] = vtri(b[0], b[sx], b[sy], b[sy + sx], b[sz], b[sz + sx], b[sz + sy], b[sz + sy + sx], x1, y1, z1);
     } else {
       const T* b = vel + iy * sy + ix * sx + v.udof + c;
-      const T r0 = b[0] + x1 * (b[sx] - b[0]);
-      const T r1 = b[sy] + x1 * (b[sy + sx] - b[sy]);
-      V[c] = r0 + y1 * (r1 - r0);
+      V[c] = vlerp(vlerp(b[0], b[sx], x1), vlerp(b[sy], b[sy + sx], x1), y1);
     }
   }
 }
 
-// Trilinear gather of Bilinear3D with GridFunction3D_Identity
-// (Common/Grid3D.c:4-31), order i, j, k, from a base pointer with row stride
-// Y and plane stride Z (global memory or an LDS box)
 template <typename T>
 __device__ __forceinline__ T linear_gather3(const T* b, int Y, int Z, T sx, T sy, T sz, T ox, T oy, T oz) {
   T sum = T(0);

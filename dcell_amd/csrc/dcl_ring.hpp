@@ -1,12 +1,14 @@
 // dcl_ring.hpp -- dense 3D SL with phi0 staged in LDS.
 //
-// A block owns a 16 x 16 column of nodes and walks a chunk of ZC planes in z.
+// A block owns a TX x TY column of nodes (TX TY = 256 threads, one node
+// each) and walks a chunk of ZC planes in z.
 // phi0 over the column plus a halo of H = 3 nodes per side is held in LDS as
 // a ring of 2H + 2 = 8 planes: at plane k the ring holds planes k-H .. k+H,
 // and the plane k+H+1 is fetched into registers while plane k is computed,
 // then stored into the slot of plane k-H-1, so one barrier per plane
 // suffices.  Each staged value is loaded once per block (the column's halo
-// makes it (16+6)^2 / 16^2 = 1.9 values per node, plus 2H planes per chunk),
+// makes it (TX+6)(TY+6) / (TX TY) values per node, 1.9 for 16 x 16 and 2.7
+// for 64 x 4, plus 2H planes per chunk),
 // and the 64 (tricubic) or 8 (trilinear) stencil reads of a node come from
 // LDS.  H = 3 holds every stencil of a departure less than 2 cells from its
 // node along each axis (stencil base-1 .. base+2, base = floor(S)); a node
@@ -18,39 +20,51 @@
 
 namespace dcl {
 
-constexpr int RG_TX = 16, RG_TY = 16, RG_H = 3, RG_R = 8;  // RG_R = 2 RG_H + 2 ring slots
-constexpr int RG_BX = RG_TX + 2 * RG_H, RG_BY = RG_TY + 2 * RG_H;
-constexpr int RG_PLANE = RG_BX * RG_BY;
-constexpr int RG_NT = RG_TX * RG_TY;
-constexpr int RG_PF = (RG_PLANE + RG_NT - 1) / RG_NT;  // staged values per thread and plane
+constexpr int RG_H = 3, RG_R = 8;  // halo, ring slots (2 RG_H + 2)
 
-// the box's values of plane z (x, y clamped to the window) into registers
-template <typename T>
-__device__ __forceinline__ void ring_fetch(rsrc_t rp, const KWin& w, int x0, int y0, int z, int tid,
-                                           T pf[RG_PF]) {
+// geometry of a TX x TY column
+template <int TX_, int TY_>
+struct RingGeom {
+  static constexpr int TX = TX_, TY = TY_;
+  static constexpr int BX = TX + 2 * RG_H, BY = TY + 2 * RG_H;
+  static constexpr int PLANE = BX * BY;
+  static constexpr int NT = TX * TY;
+  static constexpr int PF = (PLANE + NT - 1) / NT;  // staged values per thread and plane
+};
+
+// byte offsets within a plane of the box values this thread stages (x, y
+// clamped to the window), -1 past the box; the same for every plane
+template <typename T, typename G>
+__device__ __forceinline__ void ring_offsets(const KWin& w, int x0, int y0, int tid, int xo[G::PF]) {
 #pragma unroll
-  for (int q = 0; q < RG_PF; ++q) {
-    const int e = tid + q * RG_NT;
-    if (e < RG_PLANE) {
-      const int bx = e % RG_BX, by = e / RG_BX;
-      const int gx = clampi(x0 - RG_H + bx, 0, w.nx - 1), gy = clampi(y0 - RG_H + by, 0, w.ny - 1);
-      pf[q] = BL<T>::one(rp, ((z * w.ny + gy) * w.nx + gx) * (int)sizeof(T), 0);
-    }
+  for (int q = 0; q < G::PF; ++q) {
+    const int e = tid + q * G::NT;
+    const int bx = e % G::BX, by = e / G::BX;
+    const int gx = clampi(x0 - RG_H + bx, 0, w.nx - 1), gy = clampi(y0 - RG_H + by, 0, w.ny - 1);
+    xo[q] = e < G::PLANE ? (gy * w.nx + gx) * (int)sizeof(T) : -1;
   }
 }
 
-template <typename T>
-__device__ __forceinline__ void ring_store(T* ring, int slot, int tid, const T pf[RG_PF]) {
+// the box's values of plane z into registers; zo = z nx ny sizeof(T) (uniform)
+template <typename T, typename G>
+__device__ __forceinline__ void ring_fetch(rsrc_t rp, const int xo[G::PF], int zo, T pf[G::PF]) {
 #pragma unroll
-  for (int q = 0; q < RG_PF; ++q) {
-    const int e = tid + q * RG_NT;
-    if (e < RG_PLANE) ring[slot * RG_PLANE + e] = pf[q];
+  for (int q = 0; q < G::PF; ++q)
+    if (xo[q] >= 0) pf[q] = BL<T>::one(rp, xo[q], zo);
+}
+
+template <typename T, typename G>
+__device__ __forceinline__ void ring_store(T* ring, int slot, int tid, const T pf[G::PF]) {
+#pragma unroll
+  for (int q = 0; q < G::PF; ++q) {
+    const int e = tid + q * G::NT;
+    if (e < G::PLANE) ring[slot * G::PLANE + e] = pf[q];
   }
 }
 
 // gather of phi0 at S for the node in plane k of the column at (x0, y0); the
 // ring slot of plane z is (z - z0 + H) mod 8
-template <typename T, int INTERP>
+template <typename T, int INTERP, typename G>
 __device__ __forceinline__ T gather3_ring(rsrc_t rp, const T* ring, const KWin& w, const double S[3], int x0,
                                           int y0, int z0, int k) {
   double fx, fy, fz;
@@ -63,12 +77,12 @@ __device__ __forceinline__ T gather3_ring(rsrc_t rp, const T* ring, const KWin& 
     cr_weights<T>((T)fy, wy);
     cr_weights<T>((T)fz, wz);
     const int lx = bx - 1 - (x0 - RG_H), ly = by - 1 - (y0 - RG_H), lz = bz - 1 - (k - RG_H);
-    if ((unsigned)lx <= (unsigned)(RG_BX - 4) && (unsigned)ly <= (unsigned)(RG_BY - 4) &&
+    if ((unsigned)lx <= (unsigned)(G::BX - 4) && (unsigned)ly <= (unsigned)(G::BY - 4) &&
         (unsigned)lz <= (unsigned)(2 * RG_H - 3)) {
-      const T* b = ring + ly * RG_BX + lx;
+      const T* b = ring + ly * G::BX + lx;
       const int zs = bz - 1 - z0 + RG_H;
       return cubic_sum<T>(wx, wy, wz, [&](int j, int kk, T r[4]) {
-        const T* q = b + ((zs + kk) & (RG_R - 1)) * RG_PLANE + j * RG_BX;
+        const T* q = b + ((zs + kk) & (RG_R - 1)) * G::PLANE + j * G::BX;
         r[0] = q[0];
         r[1] = q[1];
         r[2] = q[2];
@@ -86,19 +100,19 @@ __device__ __forceinline__ T gather3_ring(rsrc_t rp, const T* ring, const KWin& 
     const T ox = (T)(1.0 - fx), oy = (T)(1.0 - fy), oz = (T)(1.0 - fz);
     const int lx = bx - (x0 - RG_H), ly = by - (y0 - RG_H), lz = bz - (k - RG_H);
     T a00[2], a01[2], a10[2], a11[2];
-    if ((unsigned)lx <= (unsigned)(RG_BX - 2) && (unsigned)ly <= (unsigned)(RG_BY - 2) &&
+    if ((unsigned)lx <= (unsigned)(G::BX - 2) && (unsigned)ly <= (unsigned)(G::BY - 2) &&
         (unsigned)lz <= (unsigned)(2 * RG_H - 1)) {
       const int zs = bz - z0 + RG_H;
-      const T* p0 = ring + (zs & (RG_R - 1)) * RG_PLANE + ly * RG_BX + lx;
-      const T* p1 = ring + ((zs + 1) & (RG_R - 1)) * RG_PLANE + ly * RG_BX + lx;
+      const T* p0 = ring + (zs & (RG_R - 1)) * G::PLANE + ly * G::BX + lx;
+      const T* p1 = ring + ((zs + 1) & (RG_R - 1)) * G::PLANE + ly * G::BX + lx;
       a00[0] = p0[0];
       a00[1] = p0[1];
-      a10[0] = p0[RG_BX];
-      a10[1] = p0[RG_BX + 1];
+      a10[0] = p0[G::BX];
+      a10[1] = p0[G::BX + 1];
       a01[0] = p1[0];
       a01[1] = p1[1];
-      a11[0] = p1[RG_BX];
-      a11[1] = p1[RG_BX + 1];
+      a11[0] = p1[G::BX];
+      a11[1] = p1[G::BX + 1];
     } else {
       const int Yr = w.nx, Zr = w.nx * w.ny, e = (int)sizeof(T);
       const int vo = (bz * Zr + by * Yr + bx) * e;
@@ -111,20 +125,149 @@ __device__ __forceinline__ T gather3_ring(rsrc_t rp, const T* ring, const KWin& 
   }
 }
 
-// the SL step of one column: nodes (x0 .. x0+15, y0 .. y0+15, z0 .. z0+ZC-1)
-// of the window, clipped to it; nodes outside [2, n-3] copy phi0.  Every
-// thread of the block calls it (it holds barriers) and it ends on a barrier,
-// so a block may run it for one column after another with the same ring.
-template <typename T, int INTERP, int BT, int ZC>
+// Velocity ring (optional): with the velocity window equal to phi's and 3
+// components per node, the faces over the column plus 1 node below and 2
+// above in x and y are staged too, one component after the other (so the x
+// pair of faces of a MAC corner is two neighbouring words), for the planes
+// k-1 .. k+2 that the node velocity and the RK2 midpoint (less than one cell
+// away) read, plus the plane being prefetched: VR_R = 5 slots, plane z in
+// slot (z - z0 + 1) mod 5.  A lane whose midpoint corners leave the box reads
+// them from memory.
+constexpr int VR_R = 5;
+
+template <int TX_, int TY_>
+struct VelGeom {
+  static constexpr int VX = TX_ + 3, VY = TY_ + 3;
+  static constexpr int PLANE = VX * VY;
+  static constexpr int NT = TX_ * TY_;
+  static constexpr int PF = (PLANE + NT - 1) / NT;
+};
+
+template <typename T, typename V>
+__device__ __forceinline__ void vring_offsets(const KWin& w, int x0, int y0, int tid, int xo[V::PF]) {
+#pragma unroll
+  for (int q = 0; q < V::PF; ++q) {
+    const int e = tid + q * V::NT;
+    const int bx = e % V::VX, by = e / V::VX;
+    const int gx = clampi(x0 - 1 + bx, 0, w.nx - 1), gy = clampi(y0 - 1 + by, 0, w.ny - 1);
+    xo[q] = e < V::PLANE ? (gy * w.nx + gx) * 3 * (int)sizeof(T) : -1;
+  }
+}
+
+// the faces (u, v, w) of the box nodes of plane z; zo = 3 z nx ny sizeof(T)
+template <typename T, typename V>
+__device__ __forceinline__ void vring_fetch(rsrc_t rv, const int xo[V::PF], int zo, T pf[V::PF][3]) {
+#pragma unroll
+  for (int q = 0; q < V::PF; ++q) {
+    if (xo[q] >= 0) {
+      if (sizeof(T) == 4) {
+        typedef float f32x3 __attribute__((ext_vector_type(3)));
+        const f32x3 x = __builtin_bit_cast(f32x3, __builtin_amdgcn_raw_buffer_load_b96(rv, xo[q], zo, 0));
+        pf[q][0] = (T)x.x;
+        pf[q][1] = (T)x.y;
+        pf[q][2] = (T)x.z;
+      } else {
+        pf[q][0] = BL<T>::one(rv, xo[q], zo);
+        pf[q][1] = BL<T>::one(rv, xo[q] + (int)sizeof(T), zo);
+        pf[q][2] = BL<T>::one(rv, xo[q] + 2 * (int)sizeof(T), zo);
+      }
+    }
+  }
+}
+
+template <typename T, typename V>
+__device__ __forceinline__ void vring_store(T* vr, int slot, int tid, const T pf[V::PF][3]) {
+#pragma unroll
+  for (int q = 0; q < V::PF; ++q) {
+    const int e = tid + q * V::NT;
+    if (e < V::PLANE) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) vr[(slot * 3 + c) * V::PLANE + e] = pf[q][c];
+    }
+  }
+}
+
+// the departure point of interior node (i, j, k) (local) with the velocity
+// read from the ring; sl[m] is the slot of plane k-1+m
+template <typename T, int BT, typename V>
+__device__ __forceinline__ void sl_dep3_ring(rsrc_t rv, const T* vr, const int sl[4], const SLP& P, int i, int j,
+                                             int k, int x0, int y0, double S[3]) {
+  // node velocity: faces at (i, i+1), (j, j+1), (k, k+1); interior nodes need no clamp
+  const int bx = i - (x0 - 1), by = j - (y0 - 1);
+  const T* p0 = vr + sl[1] * 3 * V::PLANE + by * V::VX + bx;
+  const T* p1 = vr + sl[2] * 3 * V::PLANE + by * V::VX + bx;
+  T Vn[3];
+  Vn[0] = vmid(p0[0], p0[1]);
+  Vn[1] = vmid(p0[V::PLANE], p0[V::PLANE + V::VX]);
+  Vn[2] = vmid(p0[2 * V::PLANE], p1[2 * V::PLANE]);
+  const int X = P.w.px + i, Y = P.w.py + j, Z = P.w.pz + k;
+  const double Xd[3] = {(double)X, (double)Y, (double)Z};
+  if (BT != DCL_RK2) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) S[a] = Xd[a] - (double)Vn[a] * P.c[a];
+    return;
+  }
+  double Xm[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) Xm[a] = Xd[a] - (double)Vn[a] * P.ch[a];
+  int b0[3], b1[3];
+  T f0[3], f1[3];
+  mac_split<T, 3>(P.v, Xm, b0, b1, f0, f1);
+  T V2[3];
+  const int lo[3] = {x0 - 1, y0 - 1, k - 1};
+  const int hi[3] = {x0 - 1 + V::VX - 1, y0 - 1 + V::VY - 1, k + 2};
+  bool in = true;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) in = in && b0[a] >= lo[a] && b1[a] + 1 <= hi[a];
+  if (in) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int ix = c == 0 ? b1[0] : b0[0], iy = c == 1 ? b1[1] : b0[1], iz = c == 2 ? b1[2] : b0[2];
+      const T x1 = c == 0 ? f1[0] : f0[0], y1 = c == 1 ? f1[1] : f0[1], z1 = c == 2 ? f1[2] : f0[2];
+      const int rel = iz - (k - 1);  // 0 .. 2
+      const int sa = rel == 0 ? sl[0] : rel == 1 ? sl[1] : sl[2];
+      const int sb = rel == 0 ? sl[1] : rel == 1 ? sl[2] : sl[3];
+      const int o = (iy - lo[1]) * V::VX + (ix - lo[0]) + c * V::PLANE;
+      const T* qa = vr + sa * 3 * V::PLANE + o;
+      const T* qb = vr + sb * 3 * V::PLANE + o;
+      V2[c] = vtri(qa[0], qa[1], qa[V::VX], qa[V::VX + 1], qb[0], qb[1], qb[V::VX], qb[V::VX + 1], x1, y1, z1);
+    }
+  } else {
+    mac_vel_buf_split<T>(rv, P.v, b0, b1, f0, f1, V2);
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) S[a] = Xd[a] - (double)V2[a] * P.c[a];
+}
+
+// the SL step of one column: nodes (x0 .. x0+TX-1, y0 .. y0+TY-1, z0 ..
+// z0+ZC-1) of the window, clipped to it; nodes outside [2, n-3] copy phi0.
+// Every thread of the block calls it (it holds barriers) and it ends on a
+// barrier, so a block may run it for one column after another with the same
+// ring (RG_R * G::PLANE values).
+template <typename T, int INTERP, int BT, int ZC, typename G, bool VL>
 __device__ __forceinline__ void ring_column(const T* __restrict__ phi0, T* __restrict__ phi, rsrc_t rp,
-                                            rsrc_t rv, const SLP& P, int x0, int y0, int z0, T* ring) {
+                                            rsrc_t rv, const SLP& P, int x0, int y0, int z0, T* ring, T* vr) {
+  typedef VelGeom<G::TX, G::TY> V;
   const int nx = P.w.nx, ny = P.w.ny, nz = P.w.nz;
   const int z1 = min(nz, z0 + ZC);
-  const int tid = threadIdx.x + RG_TX * threadIdx.y;
+  const int tid = threadIdx.x + G::TX * threadIdx.y;
+  // bytes per plane of phi0 (a third of velocity's); products are taken
+  // unsigned, the buffer offsets being 32-bit (windows below 4 GiB)
+  const unsigned pz = (unsigned)(nx * ny) * (unsigned)sizeof(T);
+  int xo[G::PF], vxo[V::PF];
+  ring_offsets<T, G>(P.w, x0, y0, tid, xo);
+  if (VL) vring_offsets<T, V>(P.w, x0, y0, tid, vxo);
   for (int z = max(z0 - RG_H, 0); z <= min(z0 + RG_H, nz - 1); ++z) {
-    T pf[RG_PF];
-    ring_fetch<T>(rp, P.w, x0, y0, z, tid, pf);
-    ring_store<T>(ring, (z - z0 + RG_H) & (RG_R - 1), tid, pf);
+    T pf[G::PF];
+    ring_fetch<T, G>(rp, xo, (int)((unsigned)z * pz), pf);
+    ring_store<T, G>(ring, (z - z0 + RG_H) & (RG_R - 1), tid, pf);
+  }
+  if (VL) {
+    for (int z = max(z0 - 1, 0); z <= min(z0 + 2, nz - 1); ++z) {
+      T vf[V::PF][3];
+      vring_fetch<T, V>(rv, vxo, (int)(3u * (unsigned)z * pz), vf);
+      vring_store<T, V>(vr, (z - z0 + 1) % VR_R, tid, vf);
+    }
   }
   __syncthreads();
   const int i = x0 + threadIdx.x, j = y0 + threadIdx.y;
@@ -133,31 +276,45 @@ __device__ __forceinline__ void ring_column(const T* __restrict__ phi0, T* __res
   for (int k = z0; k < z1; ++k) {
     const int zn = k + RG_H + 1;
     const bool pre = zn < nz && k + 1 < z1;
-    T pf[RG_PF];
-    if (pre) ring_fetch<T>(rp, P.w, x0, y0, zn, tid, pf);
+    T pf[G::PF];
+    if (pre) ring_fetch<T, G>(rp, xo, (int)((unsigned)zn * pz), pf);
+    const bool vpre = VL && k + 3 < nz && k + 1 < z1;
+    T vf[V::PF][3];
+    if (vpre) vring_fetch<T, V>(rv, vxo, (int)(3u * (unsigned)(k + 3) * pz), vf);
     if (in_w) {
       const int id = (k * ny + j) * nx + i;
       if (!(in_xy && k >= 2 && k <= nz - 3)) {
         phi[id] = phi0[id];
       } else {
         double S[3];
-        sl_dep3_buf<T, BT>(rv, P, P.w.px + i, P.w.py + j, P.w.pz + k, S);
-        phi[id] = gather3_ring<T, INTERP>(rp, ring, P.w, S, x0, y0, z0, k);
+        if (VL) {
+          const int b = (k - z0) % VR_R;
+          const int sl[4] = {b, (b + 1) % VR_R, (b + 2) % VR_R, (b + 3) % VR_R};
+          sl_dep3_ring<T, BT, V>(rv, vr, sl, P, i, j, k, x0, y0, S);
+        } else {
+          sl_dep3_buf<T, BT>(rv, P, P.w.px + i, P.w.py + j, P.w.pz + k, S);
+        }
+        phi[id] = gather3_ring<T, INTERP, G>(rp, ring, P.w, S, x0, y0, z0, k);
       }
     }
-    if (pre) ring_store<T>(ring, (zn - z0 + RG_H) & (RG_R - 1), tid, pf);
+    if (pre) ring_store<T, G>(ring, (zn - z0 + RG_H) & (RG_R - 1), tid, pf);
+    if (vpre) vring_store<T, V>(vr, (k + 3 - z0 + 1) % VR_R, tid, vf);
     __syncthreads();
   }
 }
 
-// grid: tiles_x * tiles_y * nzc blocks of RG_TX x RG_TY threads (XCD-remapped
-// so each XCD walks neighbouring columns)
-template <typename T, int INTERP, int BT, int ZC, int WPE>
-__global__ __launch_bounds__(RG_NT) __attribute__((amdgpu_waves_per_eu(WPE))) void sl_ring3d(const T* __restrict__ phi0, T* __restrict__ phi,
-                                                   const T* __restrict__ vel, SLP P, unsigned tiles_x,
-                                                   unsigned tiles_y, unsigned nzc, unsigned pbytes,
-                                                   unsigned vbytes) {
-  __shared__ T ring[RG_R * RG_PLANE];
+// grid: tiles_x * tiles_y * nzc blocks of TX x TY threads (XCD-remapped so
+// each XCD walks neighbouring columns).  VL: velocity staged too (velocity
+// window = phi window, 3 components, udof 0).
+template <typename T, int INTERP, int BT, int ZC, int TX, int TY, bool VL>
+__global__ __launch_bounds__(TX * TY) void sl_ring3d(const T* __restrict__ phi0, T* __restrict__ phi,
+                                                     const T* __restrict__ vel, SLP P, unsigned tiles_x,
+                                                     unsigned tiles_y, unsigned nzc, unsigned pbytes,
+                                                     unsigned vbytes) {
+  typedef RingGeom<TX, TY> G;
+  typedef VelGeom<TX, TY> V;
+  __shared__ T ring[RG_R * G::PLANE];
+  __shared__ T vr[VL ? VR_R * 3 * V::PLANE : 1];
   const unsigned nb = tiles_x * tiles_y * nzc;
   unsigned lb = blockIdx.x;
   {
@@ -165,8 +322,8 @@ __global__ __launch_bounds__(RG_NT) __attribute__((amdgpu_waves_per_eu(WPE))) vo
     if (per != 0 && lb < per * 8u) lb = (lb % 8u) * per + lb / 8u;
   }
   const unsigned tx = lb % tiles_x, ty = (lb / tiles_x) % tiles_y, zc = lb / (tiles_x * tiles_y);
-  ring_column<T, INTERP, BT, ZC>(phi0, phi, make_rsrc(phi0, pbytes), make_rsrc(vel, vbytes), P,
-                                 (int)tx * RG_TX, (int)ty * RG_TY, (int)zc * ZC, ring);
+  ring_column<T, INTERP, BT, ZC, G, VL>(phi0, phi, make_rsrc(phi0, pbytes), make_rsrc(vel, vbytes), P,
+                                        (int)tx * TX, (int)ty * TY, (int)zc * ZC, ring, vr);
 }
 
 }  // namespace dcl

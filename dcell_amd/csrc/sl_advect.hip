@@ -74,15 +74,16 @@ __global__ __launch_bounds__(256) void sl_band(const T* __restrict__ phi0, T* __
   }
 }
 
-// dense 3D windows: phi0 staged in an LDS ring (sl_ring3d, the default for
-// the 64-value tricubic stencil) or gathered from global memory per node
-// (sl_dense, the default for the 8-value trilinear one); DCL_SL_KERNEL=
-// ring|gather overrides the default, read at every call
-static bool sl_ring_selected(int interp) {
+// Dense 3D windows: phi0 staged in an LDS ring (sl_ring3d) or gathered from
+// global memory per node (sl_dense).  Measured on 256^3 / 384^3 windows, the
+// ring is faster for the 64-value tricubic stencil and for the RK2 midpoint,
+// the gather for trilinear Euler.  DCL_SL_KERNEL=ring|gather overrides the
+// choice (read at every call).
+static bool sl_ring_selected(int interp, int bt) {
   const char* e = getenv("DCL_SL_KERNEL");
   if (e && strcmp(e, "gather") == 0) return false;
   if (e && strcmp(e, "ring") == 0) return true;
-  return interp == DCL_CUBIC;
+  return interp == DCL_CUBIC || bt == DCL_RK2;
 }
 
 template <typename T, int DIM, int INTERP, int BT, bool BUF>
@@ -91,21 +92,31 @@ static int launch_sl_b(const void* phi0, void* phi, const void* vel, const SLP& 
                        unsigned vb, hipStream_t s) {
   const T* p0 = static_cast<const T*>(phi0);
   const T* v = static_cast<const T*>(vel);
-  if (!band && BUF && DIM == 3 && sl_ring_selected(INTERP)) {
+  if (!band && BUF && DIM == 3 && sl_ring_selected(INTERP, BT)) {
+    // 32 x 8 columns (DCL_RING_TILE=0|1|2 selects 16 x 16, 32 x 8, 64 x 4);
+    // velocity staged too where it can be (same window, 3 components) and
+    // measured faster: FP32, or FP64 with the trilinear stencil (the FP64
+    // tricubic kernel is register-bound with it).  DCL_RING_VEL=0|1 overrides.
     constexpr int ZC = 32;
-    const unsigned tx = (P.w.nx + RG_TX - 1) / RG_TX, ty = (P.w.ny + RG_TY - 1) / RG_TY;
-    const unsigned nzc = (P.w.nz + ZC - 1) / ZC;
-    const char* we = getenv("DCL_RING_WPE");
-    const int wpe = we ? atoi(we) : 1;
-#define DCL_RING(W)                                                                                        \
-  if (wpe == W)                                                                                            \
-    hipLaunchKernelGGL((sl_ring3d<T, INTERP, BT, ZC, W>), dim3(tx * ty * nzc), dim3(RG_TX, RG_TY), 0, s, p0, \
-                       static_cast<T*>(phi), v, P, tx, ty, nzc, pb, vb);
-    DCL_RING(1)
-    DCL_RING(3)
-    DCL_RING(4)
-    DCL_RING(5)
-    DCL_RING(6)
+    const char* ge = getenv("DCL_RING_TILE");
+    const int g = ge ? atoi(ge) : 1;
+    const char* ve = getenv("DCL_RING_VEL");
+    const bool same = P.v.w.px == P.w.px && P.v.w.py == P.w.py && P.v.w.pz == P.w.pz && P.v.w.nx == P.w.nx &&
+                      P.v.w.ny == P.w.ny && P.v.w.nz == P.w.nz && P.v.dof == 3 && P.v.udof == 0;
+    const bool vl = same && (ve ? atoi(ve) != 0 : (sizeof(T) == 4 || INTERP == DCL_LINEAR));
+#define DCL_RING(GI, TXV, TYV, VLV)                                                                          \
+  if (g == GI && vl == VLV) {                                                                                \
+    const unsigned tx = (P.w.nx + TXV - 1) / TXV, ty = (P.w.ny + TYV - 1) / TYV;                             \
+    const unsigned nzc = (P.w.nz + ZC - 1) / ZC;                                                             \
+    hipLaunchKernelGGL((sl_ring3d<T, INTERP, BT, ZC, TXV, TYV, VLV>), dim3(tx * ty * nzc), dim3(TXV, TYV), 0, \
+                       s, p0, static_cast<T*>(phi), v, P, tx, ty, nzc, pb, vb);                              \
+  }
+    DCL_RING(0, 16, 16, false)
+    DCL_RING(1, 32, 8, false)
+    DCL_RING(2, 64, 4, false)
+    DCL_RING(0, 16, 16, true)
+    DCL_RING(1, 32, 8, true)
+    DCL_RING(2, 64, 4, true)
 #undef DCL_RING
     DCL_LAUNCHED("sl_ring3d");
     return 0;

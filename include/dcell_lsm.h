@@ -229,7 +229,7 @@ int dcl_batch_layout(dcl_batch batch, int64_t *phi_offsets, int64_t *vel_offsets
 int dcl_batch_set_dt(dcl_batch batch, const double *dt);
 /* configuration of the launches: shape 0..5 = curvature and SL tile shape
  * (tile[] of dcl_batch_get_config), shape 6..11 = the curvature tiles of
- * shape - 6 with the SL step over 16 x 16 x 32 columns staged in an LDS
+ * shape - 6 with the SL step over 32 x 8 x 32 columns staged in an LDS
  * ring; occupancy 0..2 -> 2, 4, 8 resident blocks per CU */
 int dcl_batch_set_config(dcl_batch batch, int shape, int occupancy);
 int dcl_batch_get_config(dcl_batch batch, int *shape, int *occupancy, int32_t tile[3],

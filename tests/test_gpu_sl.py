@@ -89,10 +89,12 @@ def test_sl_band_parity_offset_velocity_window(dev, dtype, interp, bt, dim):
 @pytest.mark.parametrize("interp", ["linear", "cubic"])
 @pytest.mark.parametrize("shape,amp", [((37, 23, 50), 1.0), ((16, 16, 32), 0.3), ((70, 9, 33), 3.0),
                                        ((5, 6, 7), 1.0)])
-def test_sl_ring_equals_gather(dev, monkeypatch, dtype, interp, shape, amp):
+@pytest.mark.parametrize("tile", ["0", "1", "2"])   # 16 x 16, 32 x 8, 64 x 4 columns
+@pytest.mark.parametrize("vring", ["0", "1"])       # velocity from memory / staged in LDS
+def test_sl_ring_equals_gather(dev, monkeypatch, dtype, interp, shape, amp, tile, vring):
     """The LDS-ring kernel gives the per-node gather kernel's values bit for bit
-    (same arithmetic, stencil values read from LDS instead of memory) on
-    windows that are not multiples of the 16 x 16 x 32 block, several z
+    (same arithmetic, stencil values read from LDS instead of memory) for
+    every column shape, on windows that are not multiples of the column, several z
     chunks, a flow slow enough to stay in the ring, and one fast enough to
     leave it (those nodes gather from memory)."""
     rng = np.random.default_rng(sum(shape))
@@ -101,6 +103,8 @@ def test_sl_ring_equals_gather(dev, monkeypatch, dtype, interp, shape, amp):
     win = D.GridWindow((1, -4, 2), (nx, ny, nz), (h, h, h))
     phi0 = rounded(rng.normal(size=(nz, ny, nx)), dtype)
     vel = rounded(rng.uniform(-amp, amp, size=(nz, ny, nx, 3)), dtype)
+    monkeypatch.setenv("DCL_RING_TILE", tile)
+    monkeypatch.setenv("DCL_RING_VEL", vring)
     outs = {}
     for mode in ("gather", "ring"):
         monkeypatch.setenv("DCL_SL_KERNEL", mode)
