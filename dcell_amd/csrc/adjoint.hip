@@ -137,16 +137,17 @@ __global__ __launch_bounds__(256) void sl_adjoint_k(const T* __restrict__ phi0, 
       const double Xd[3] = {(double)X, (double)Yg, (double)Zg};
       T V1[3];
       node_vel<T, 3>(vel, P.v, X, Yg, Zg, V1);
-      MacState<T> m2;
-      bool cm[3] = {false, false, false};
-      double S[3];
+      // the midpoint value now (the forward's arithmetic); its derivative
+      // state only after the stencil loop, to keep it out of the loop's
+      // registers (more waves in flight hide the dependent gathers)
+      double Xm[3], S[3];
       if (BT == DCL_RK2) {
-        double Xm[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) Xm[a] = Xd[a] - (double)V1[a] * P.ch[a];
-        mac_state<T>(vel, P.v, Xm, m2, cm);
+        T V2[3];
+        mac_vel<T, 3>(vel, P.v, Xm, V2);
 #pragma unroll
-        for (int a = 0; a < 3; ++a) S[a] = Xd[a] - (double)m2.V[a] * P.c[a];
+        for (int a = 0; a < 3; ++a) S[a] = Xd[a] - (double)V2[a] * P.c[a];
       } else {
 #pragma unroll
         for (int a = 0; a < 3; ++a) S[a] = Xd[a] - (double)V1[a] * P.c[a];
@@ -183,7 +184,8 @@ __global__ __launch_bounds__(256) void sl_adjoint_k(const T* __restrict__ phi0, 
                           bz0 + NP <= LBZ;
       T G[3] = {T(0), T(0), T(0)};
       const T* src = phi0 + (b[2] + OFF) * Z + (b[1] + OFF) * Y + (b[0] + OFF);
-#pragma unroll
+      // one stencil plane at a time: bounds the loads in flight (registers)
+#pragma unroll 1
       for (int kk = 0; kk < NP; ++kk)
 #pragma unroll
         for (int jj = 0; jj < NP; ++jj) {
@@ -210,6 +212,9 @@ __global__ __launch_bounds__(256) void sl_adjoint_k(const T* __restrict__ phi0, 
       for (int a = 0; a < 3; ++a) H[a] = cs[a] ? T(0) : (T)(-P.c[a]) * g * G[a];
       T H1[3];
       if (BT == DCL_RK2) {
+        MacState<T> m2;
+        bool cm[3] = {false, false, false};
+        mac_state<T>(vel, P.v, Xm, m2, cm);
         mac_scatter<T>(gvel, P.v, m2, H);
 #pragma unroll
         for (int bb = 0; bb < 3; ++bb) {
