@@ -49,7 +49,7 @@ def parse():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-forms", action="store_true", help="only the headline workload")
-    ap.add_argument("--cpu-steps", type=int, default=2, help="full-grid steps of the CPU baseline leg")
+    ap.add_argument("--cpu-steps", type=int, default=3, help="full-grid steps of the CPU baseline leg")
     ap.add_argument("--form", choices=sorted(FORMS),
                     help="run one form only (for profiling) and print its JSON")
     ap.add_argument("--probe", action="store_true",
@@ -227,12 +227,15 @@ def cpu_baseline(args):
     nthreads = len(os.sched_getaffinity(0))
     for threads in (1, nthreads):
         O.set_threads(threads)
-        p = phi
-        t0 = time.perf_counter()
-        for s in range(args.cpu_steps):
+        p = O.sl_advect(phi, vel, w, configs.deformation_dt(n, 0), O.CUBIC, O.RK2)  # warm-up step
+        O.curvature(p, w)
+        times = []
+        for s in range(1, 1 + args.cpu_steps):
+            t0 = time.perf_counter()
             p = O.sl_advect(p, vel, w, configs.deformation_dt(n, s), O.CUBIC, O.RK2)
             O.curvature(p, w)
-        out[threads] = (n - 4) ** 3 * args.cpu_steps / (time.perf_counter() - t0)
+            times.append(time.perf_counter() - t0)
+        out[threads] = (n - 4) ** 3 / float(np.median(times))
     O.set_threads(1)
     model = "unknown"
     try:
@@ -244,8 +247,8 @@ def cpu_baseline(args):
     except OSError:
         pass
     return dict(value=out[1], unit="grid-point updates/s", cores=1, kind="port",
-                sample=f"{args.cpu_steps} full steps of the {n}^3 workload (RK2+tricubic SL + "
-                       "19-point curvature, FP64, dense) by the oracle, 1 thread",
+                sample=f"median of {args.cpu_steps} timed full steps (after 1 warm-up step) of the {n}^3 "
+                       "workload (RK2+tricubic SL + 19-point curvature, FP64, dense) by the oracle, 1 thread",
                 all_cores=dict(value=out[nthreads], cores=nthreads), cpu=model)
 
 
@@ -418,6 +421,63 @@ def adjoint_run(n=128, steps=50, every=10):
                 grad_norm=float(v.grad.norm()))
 
 
+def zalesak_run(n=128, steps=628, interp="cubic", bt="rk2"):
+    """C1: the Zalesak slotted disk (128^2 FP64) under rigid rotation for one
+    revolution (628 steps, dt = 1/628), SL + 9-point curvature every step.
+    Reports updates/s and, after the revolution, the area of {phi < 0} and
+    the max / mean |phi - phi_initial| within 3 cells of the interface."""
+    import torch
+    import dcell_amd as D
+    from dcell_amd import configs
+
+    h = 1.0 / (n - 1)
+    win = D.GridWindow((0, 0, 0), (n, n, 1), (h, h, h))
+    dev = torch.device("cuda:0")
+    phi_np = configs.zalesak_sdf(n)
+    phi = torch.from_numpy(phi_np).to(dev, torch.float64)
+    vel = torch.from_numpy(configs.rotation_velocity(n)).to(dev, torch.float64)
+    bufs = [phi, torch.empty_like(phi)]
+    kappa = torch.empty_like(phi)
+    for s in range(2):  # warm-up (no state carried over)
+        D.advect_sl(bufs[0], vel, win, 1.0 / steps, interp, bt, out=bufs[1])
+        D.curvature(bufs[1], win, out=kappa)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        D.advect_sl(bufs[s % 2], vel, win, 1.0 / steps, interp, bt, out=bufs[(s + 1) % 2])
+        D.curvature(bufs[(s + 1) % 2], win, out=kappa)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    final = bufs[steps % 2].cpu().numpy()
+    band = np.abs(phi_np) < 3
+    err = np.abs(final - phi_np)[band]
+    interior = (n - 4) ** 2
+    return dict(n=n, steps=steps, interp=interp, backtrace=bt, updates_per_s=interior * steps / wall,
+                ms_per_step=wall / steps * 1e3, area_initial=float((phi_np < 0).sum() * h * h),
+                area_final=float((final < 0).sum() * h * h), max_abs_err_band=float(err.max()),
+                mean_abs_err_band=float(err.mean()))
+
+
+def copy_ceiling():
+    """The measured HBM ceiling: a device-to-device copy of 1 GiB (torch),
+    best of 5, read + write bytes over its time."""
+    import torch
+    src = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    best = 1e30
+    for _ in range(5):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        torch.cuda.synchronize()
+        best = min(best, e0.elapsed_time(e1) * 1e-3)
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2 * (1 << 30) / best
+
+
 def _f64():
     import torch
     return torch.float64
@@ -436,6 +496,8 @@ FORMS = {
     "fp64_reference_scheme_256": lambda: dense_run(256, _f64(), "linear", "euler", 50, 2),
     "fp32_config_scheme_384": lambda: dense_run(384, _f32(), "cubic", "rk2", 50, 2),
     "fp32_reference_scheme_384": lambda: dense_run(384, _f32(), "linear", "euler", 50, 2),
+    "zalesak_2d_128_fp64_config_scheme": lambda: zalesak_run(128, 628, "cubic", "rk2"),
+    "zalesak_2d_128_fp64_reference_scheme": lambda: zalesak_run(128, 628, "linear", "euler"),
 }
 
 
@@ -488,6 +550,7 @@ def main():
                      "bytes_per_launch": interior * B["sl"],
                      "avg_launch_ms": r["sl_ms"]},
         "step_roofline_frac": r["step_roofline_frac"],
+        "measured_copy_gbs": copy_ceiling() / 1e9,
         "kernels_ms": {"sl_advect": r["sl_ms"], "curvature": r["curv_ms"]},
         "curvature_gbs": r["curv_gbs"],
     }

@@ -20,14 +20,11 @@ struct CurvP {
 };
 
 // dense 3D windows use the 2.5D LDS kernel; DCL_DENSE_KERNEL=gather selects
-// the plain per-node kernel, for comparisons
+// the plain per-node kernel that windows of 4 GiB and more take (read at
+// every call)
 static bool use_tiled_curv() {
-  static int mode = -1;
-  if (mode < 0) {
-    const char* e = getenv("DCL_DENSE_KERNEL");
-    mode = (e && strcmp(e, "gather") == 0) ? 0 : 1;
-  }
-  return mode == 1;
+  const char* e = getenv("DCL_DENSE_KERNEL");
+  return !(e && strcmp(e, "gather") == 0);
 }
 
 template <typename T, int DIM, int ST>

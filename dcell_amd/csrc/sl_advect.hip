@@ -78,10 +78,13 @@ __global__ __launch_bounds__(256) void sl_band(const T* __restrict__ phi0, T* __
 // global memory per node (sl_dense).  Measured on 256^3 / 384^3 windows, the
 // ring is faster for the 64-value tricubic stencil and for the RK2 midpoint,
 // the gather for trilinear Euler.  DCL_SL_KERNEL=ring|gather overrides the
-// choice (read at every call).
+// choice (read at every call); DCL_SL_KERNEL=plain selects the pointer-based
+// gather that windows of 4 GiB and more take.
+static const char* sl_kernel_env() { return getenv("DCL_SL_KERNEL"); }
+
 static bool sl_ring_selected(int interp, int bt) {
-  const char* e = getenv("DCL_SL_KERNEL");
-  if (e && strcmp(e, "gather") == 0) return false;
+  const char* e = sl_kernel_env();
+  if (e && (strcmp(e, "gather") == 0 || strcmp(e, "plain") == 0)) return false;
   if (e && strcmp(e, "ring") == 0) return true;
   return interp == DCL_CUBIC || bt == DCL_RK2;
 }
@@ -148,7 +151,8 @@ static int launch_sl(const void* phi0, void* phi, const void* vel, const SLP& P,
                      hipStream_t s) {
   const long long pb = (long long)P.w.nx * P.w.ny * P.w.nz * (long long)sizeof(T);
   const long long vb = (long long)P.v.w.nx * P.v.w.ny * P.v.w.nz * P.v.dof * (long long)sizeof(T);
-  if (DIM == 3 && buf_ok(pb, vb))
+  const char* e = sl_kernel_env();
+  if (DIM == 3 && buf_ok(pb, vb) && !(e && strcmp(e, "plain") == 0))
     return launch_sl_b<T, DIM, INTERP, BT, true>(phi0, phi, vel, P, band, band_len, band_values,
                                                   (unsigned)pb, (unsigned)vb, s);
   return launch_sl_b<T, DIM, INTERP, BT, false>(phi0, phi, vel, P, band, band_len, band_values, 0, 0, s);

@@ -18,8 +18,12 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-12), (torch.float32, 5e-5)])
-@pytest.mark.parametrize("dim,stencil", [(3, "std"), (2, "std"), (2, "lss")])
-def test_curvature_dense_and_band(dev, dtype, tol, dim, stencil):
+@pytest.mark.parametrize("dim,stencil,kernel", [(3, "std", "tiled"), (3, "std", "gather"), (2, "std", "tiled"),
+                                                (2, "lss", "tiled")])
+def test_curvature_dense_and_band(dev, monkeypatch, dtype, tol, dim, stencil, kernel):
+    # kernel: the 3D plane-ring kernel, or (DCL_DENSE_KERNEL=gather) the per-node
+    # kernel that windows of 4 GiB and more take
+    monkeypatch.setenv("DCL_DENSE_KERNEL", kernel)
     rng = np.random.default_rng(31 + dim)
     n = 36 if dim == 3 else 80
     h = 1.0 / (n - 1)

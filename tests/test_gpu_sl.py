@@ -31,10 +31,11 @@ def _case(dim, n, seed, p=(3, -2, 5), amp=1.0):
     return win, phi0, vel, rng
 
 
-@pytest.fixture(params=["gather", "ring"])
+@pytest.fixture(params=["gather", "ring", "plain"])
 def sl_kernel(request, monkeypatch):
-    """Both dense 3D SL kernels: per-node gathers from global memory, and phi0
-    staged in an LDS ring (selected by DCL_SL_KERNEL at every call)."""
+    """Every dense 3D SL kernel: per-node buffer-load gathers, phi0 staged in an
+    LDS ring, and the pointer-based gather that windows of 4 GiB and more take
+    (selected by DCL_SL_KERNEL at every call)."""
     monkeypatch.setenv("DCL_SL_KERNEL", request.param)
     return request.param
 
