@@ -509,7 +509,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported "
-              "(the workload is one GPU's share; DESIGN.md describes how it would shard)",
+              "(the workload is one GPU's share; a multi-GPU run would give each GPU whole cells "
+              "or a z-slab of one grid, which this build does not implement)",
               file=sys.stderr)
         sys.exit(2)
     if args.traffic_summary:
