@@ -237,6 +237,13 @@ def cpu_baseline(args):
             times.append(time.perf_counter() - t0)
         out[threads] = (n - 4) ** 3 / float(np.median(times))
     O.set_threads(1)
+    # the irregular-node list of the same sphere over its 6-cell band
+    # (irregular_run's workload), by the oracle on one thread
+    band = O.band_compact(phi, w, 6.0)
+    O.irregular_nodes(phi, w, band[:1000])
+    t0 = time.perf_counter()
+    recs = O.irregular_nodes(phi, w, band)
+    t_irr = time.perf_counter() - t0
     model = "unknown"
     try:
         with open("/proc/cpuinfo") as f:
@@ -249,7 +256,9 @@ def cpu_baseline(args):
     return dict(value=out[1], unit="grid-point updates/s", cores=1, kind="port",
                 sample=f"median of {args.cpu_steps} timed full steps (after 1 warm-up step) of the {n}^3 "
                        "workload (RK2+tricubic SL + 19-point curvature, FP64, dense) by the oracle, 1 thread",
-                all_cores=dict(value=out[nthreads], cores=nthreads), cpu=model)
+                all_cores=dict(value=out[nthreads], cores=nthreads), cpu=model,
+                irregular_nodes=dict(band_nodes_per_s=len(band) / t_irr, records=len(recs), cores=1,
+                                     sample=f"the full {len(band)}-node band of the {n}^3 sphere, one call"))
 
 
 def batch_run(ncells, seed, steps, warmup, interp="cubic", bt="rk2", autotune=True):
@@ -458,6 +467,34 @@ def zalesak_run(n=128, steps=628, interp="cubic", bt="rk2"):
                 mean_abs_err_band=float(err.mean()))
 
 
+def irregular_run(n=256, steps=20, width=6.0):
+    """The irregular-node list (LevelSetUpdateIrregularNodeList) of the
+    headline sphere (n^3 FP64, r = 0.15) over its band |phi| < width cells
+    (the reference's default band width, LevelSetMethod/LevelSet.c:8): band
+    nodes/s and records/s of one dcl_irregular_nodes call (count, scan and
+    write kernels and the count's return to the host), device-resident."""
+    import torch
+    import dcell_amd as D
+    from dcell_amd import configs
+    from dcell_amd import metrics as M
+
+    win = D.GridWindow.unit_box(n)
+    phi = torch.from_numpy(configs.sphere_sdf(n)).to("cuda:0", torch.float64)
+    band, nb = M.band_compact(phi, win, width)  # phi is in grid units
+    band = band.contiguous()
+    out = torch.empty(10 * nb * M.IRREGULAR_DTYPE.itemsize, dtype=torch.uint8, device="cuda:0")
+    count = M.irregular_nodes_into(phi, win, band, out)  # warm-up
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        M.irregular_nodes_into(phi, win, band, out)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    return dict(n=n, band_nodes=nb, records=count, ms_per_call=t * 1e3, band_nodes_per_s=nb / t,
+                records_per_s=count / t, band_width_cells=width)
+
+
 def copy_ceiling():
     """The measured HBM ceiling: a device-to-device copy of 1 GiB (torch),
     best of 5, read + write bytes over its time."""
@@ -498,6 +535,7 @@ FORMS = {
     "fp32_reference_scheme_384": lambda: dense_run(384, _f32(), "linear", "euler", 50, 2),
     "zalesak_2d_128_fp64_config_scheme": lambda: zalesak_run(128, 628, "cubic", "rk2"),
     "zalesak_2d_128_fp64_reference_scheme": lambda: zalesak_run(128, 628, "linear", "euler"),
+    "irregular_nodes_256_fp64": lambda: irregular_run(256, 20),
 }
 
 

@@ -35,6 +35,20 @@ inline unsigned grid1d(int64_t n, int block) { return (unsigned)((n + block - 1)
 // their stream before returning, so the buffer is free again on return.
 int scratch(size_t bytes, void** ptr);
 
+// Exclusive scan of per-block counts (scan.hip), in two levels over tiles of
+// SCAN_TILE counts: afterwards the exclusive prefix of count i is
+// counts[i] + sums[i / SCAN_TILE] and *total is the sum.  A tile's prefix must
+// stay below 2^32 (counts below 2^22).  Asynchronous on stream s; scans
+// narrays (1 or 2) arrays of the same length n at once.
+constexpr int SCAN_TILE = 1024;
+struct ScanArrays {
+  unsigned* counts[2];
+  unsigned long long* sums[2];   // scan_tiles(n) entries each
+  unsigned long long* total[2];  // one entry each
+};
+inline int64_t scan_tiles(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
+int scan_counts(const ScanArrays& a, int narrays, int64_t n, hipStream_t s);
+
 }  // namespace dcl
 
 #define DCL_HIP(call)                                                          \

@@ -338,12 +338,8 @@ PetscErrorCode LevelSetDestroy(LevelSet ls) {
 }
 
 /* Steps outside the level-set update: no-ops here, the rest of DCell
- * provides them (IrregularNodes*.c, FastMarching*.c, ParticleLS.c:202-295). */
-__attribute__((weak)) PetscErrorCode LevelSetUpdateIrregularNodeList(LevelSet ls) {
-  (void)ls;
-  return 0;
-}
-
+ * provides them (FastMarching*.c, ParticleLS.c:202-295).  The irregular-node
+ * list is the drop-in's (libdcell_levelset.so). */
 __attribute__((weak)) PetscErrorCode LevelSetReinitialize(LevelSet ls) {
   (void)ls;
   return 0;

@@ -22,6 +22,7 @@ enum {
   DL_PART,
   DL_NODES,
   DL_KAPPA,
+  DL_IRREG,
   DL_NSLOTS
 };
 

@@ -106,35 +106,10 @@ __global__ __launch_bounds__(RB) void band_count_k(const T* __restrict__ phi, in
   if (threadIdx.x == 0) counts[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
-// exclusive scan of nb block counts in place (one block of 1024 threads), total into *total
-__global__ __launch_bounds__(1024) void band_scan_k(unsigned* __restrict__ counts, int64_t nb,
-                                                    unsigned long long* __restrict__ total) {
-  __shared__ unsigned long long sh[1024];
-  const int t = threadIdx.x;
-  const int64_t chunk = (nb + 1023) / 1024;
-  const int64_t lo = t * chunk, hi = lo + chunk < nb ? lo + chunk : nb;
-  unsigned long long s = 0;
-  for (int64_t i = lo; i < hi; ++i) s += counts[i];
-  sh[t] = s;
-  __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {  // inclusive Hillis-Steele scan of the chunk sums
-    const unsigned long long add = t >= o ? sh[t - o] : 0ull;
-    __syncthreads();
-    sh[t] += add;
-    __syncthreads();
-  }
-  unsigned long long run = sh[t] - s;  // exclusive prefix of this chunk
-  for (int64_t i = lo; i < hi; ++i) {
-    const unsigned c = counts[i];
-    counts[i] = (unsigned)run;
-    run += c;
-  }
-  if (t == 1023) *total = sh[1023];
-}
-
 template <typename T>
 __global__ __launch_bounds__(RB) void band_write_k(const T* __restrict__ phi, KWin w, int64_t n, T width,
                                                    const unsigned* __restrict__ offsets,
+                                                   const unsigned long long* __restrict__ tile_sums,
                                                    dcl_icoor* __restrict__ out, int64_t cap) {
   __shared__ unsigned sh[RB / 64];
   const int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x;
@@ -148,7 +123,7 @@ __global__ __launch_bounds__(RB) void band_write_k(const T* __restrict__ phi, KW
   unsigned wave_pre = 0;
   for (int k = 0; k < wv; ++k) wave_pre += sh[k];
   if (!pred) return;
-  const int64_t pos = (int64_t)offsets[blockIdx.x] + wave_pre + lane_pre;
+  const int64_t pos = (int64_t)(offsets[blockIdx.x] + tile_sums[blockIdx.x / SCAN_TILE]) + wave_pre + lane_pre;
   if (pos >= cap) return;
   const int64_t plane = (int64_t)w.nx * w.ny;
   const int z = (int)(i / plane), r = (int)(i - (int64_t)z * plane);
@@ -268,17 +243,20 @@ static int band_compact_t(const void* phi, const dcl_window* win, double width, 
                           int64_t cap, int64_t* count_out, hipStream_t s) {
   const int64_t n = win_nodes(*win);
   const int64_t nb = grid1d(n, RB);
+  // scratch: total | tile sums | block counts
+  const int64_t ntiles = scan_tiles(nb);
   void* sc = nullptr;
-  DCL_TRY(scratch(sizeof(unsigned) * nb + 16, &sc));
+  DCL_TRY(scratch(16 + sizeof(unsigned long long) * ntiles + sizeof(unsigned) * nb, &sc));
   unsigned long long* total = static_cast<unsigned long long*>(sc);
-  unsigned* counts = reinterpret_cast<unsigned*>(static_cast<char*>(sc) + 16);
+  unsigned long long* sums = total + 2;
+  unsigned* counts = reinterpret_cast<unsigned*>(sums + ntiles);
   const T* p = static_cast<const T*>(phi);
   hipLaunchKernelGGL((band_count_k<T>), dim3(nb), dim3(RB), 0, s, p, n, (T)width, counts);
   DCL_LAUNCHED("band_count");
-  hipLaunchKernelGGL(band_scan_k, dim3(1), dim3(1024), 0, s, counts, nb, total);
-  DCL_LAUNCHED("band_scan");
+  const ScanArrays sa = {{counts, nullptr}, {sums, nullptr}, {total, nullptr}};
+  DCL_TRY(scan_counts(sa, 1, nb, s));
   if (cap > 0) {
-    hipLaunchKernelGGL((band_write_k<T>), dim3(nb), dim3(RB), 0, s, p, kwin(*win), n, (T)width, counts,
+    hipLaunchKernelGGL((band_write_k<T>), dim3(nb), dim3(RB), 0, s, p, kwin(*win), n, (T)width, counts, sums,
                        out, cap);
     DCL_LAUNCHED("band_write");
   }

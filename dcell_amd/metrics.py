@@ -1,10 +1,13 @@
 """Reductions over the C ABI: max velocity (LevelSet_MaxVelocity,
-LevelSetMethod/Advect.c:70-111), band compaction, interface metrics
-(LevelSetDiracDelta3D/2D, NormalDirection.c:36-57) and error norms."""
+LevelSetMethod/Advect.c:70-111), band compaction, the irregular-node list
+(LevelSetUpdateIrregularNodeList, LevelSetMethod/IrregularNodes.c:8-28),
+interface metrics (LevelSetDiracDelta3D/2D, NormalDirection.c:36-57) and
+error norms."""
 from __future__ import annotations
 
 import ctypes as C
 
+import numpy as np
 import torch
 
 from . import _lib as L
@@ -52,3 +55,36 @@ def error_norms(a, b):
     L.call("dcl_error_norms", _p(a), precision_of(a), _p(b), precision_of(b), a.numel(), out,
            stream_handle())
     return tuple(out)
+
+
+# one dcl_irregular_node record (88 bytes)
+IRREGULAR_DTYPE = np.dtype([("pos", np.int32, 3), ("axis", np.int32), ("shift", np.int32),
+                            ("sign_center", np.int32), ("sign_face", np.int32), ("pad", np.int32),
+                            ("d", np.float64), ("X", np.float64, 3), ("op", np.float64, 3)])
+assert IRREGULAR_DTYPE.itemsize == 88
+
+
+def irregular_nodes_into(phi, win: GridWindow, band, out) -> int:
+    """Device-resident form: the records of the band nodes into `out` (a
+    uint8 device tensor of capacity * 88 bytes), in the reference's order;
+    returns the total count (records beyond the capacity are not written)."""
+    _dev(phi, "phi")
+    _dev(band, "band")
+    _dev(out, "out")
+    cap = out.numel() // IRREGULAR_DTYPE.itemsize
+    count = C.c_int64()
+    L.call("dcl_irregular_nodes", _p(phi), C.byref(win.c()), _p(band), band.shape[0], precision_of(phi),
+           _p(out), cap, C.byref(count), stream_handle())
+    return count.value
+
+
+def irregular_nodes(phi, win: GridWindow, band, capacity: int | None = None):
+    """The irregular-node records of the band nodes (an (n, 3) int32 device
+    tensor), in the reference's order, as a numpy structured array
+    (IRREGULAR_DTYPE) on the host, and the total count."""
+    cap = 10 * band.shape[0] if capacity is None else capacity
+    out = torch.empty(max(cap, 1) * IRREGULAR_DTYPE.itemsize, dtype=torch.uint8, device=phi.device)
+    count = irregular_nodes_into(phi, win, band, out[:cap * IRREGULAR_DTYPE.itemsize])
+    n = min(count, cap)
+    host = out[:n * IRREGULAR_DTYPE.itemsize].cpu().numpy()
+    return host.view(IRREGULAR_DTYPE), count

@@ -196,6 +196,34 @@ int dcl_band_compact(const void *phi, const dcl_window *win, double width,
                      int precision, dcl_icoor *band_out, int64_t capacity,
                      int64_t *count_out, void *stream);
 
+/* One record of the irregular-node list: the fields that
+ * LevelSetUpdateIrregularNodeList sets (LevelSetMethod/IrregularNodes3D.c:
+ * 53-113, IrregularNodes.c:69-137); the reference's IrregularNode
+ * (LevelSetMethod.h:30-54) carries them among fields filled later.  shift -1
+ * and axis -1: the orthogonal projection of the node (op, X = pos + op);
+ * shift 0: a crossing towards -x/-y/-z (axis 0..2) at fraction d; shift 1..3:
+ * a crossing of the U/V/W face averages along axis.  88 bytes. */
+typedef struct {
+  dcl_icoor pos;
+  int32_t axis, shift, sign_center, sign_face, pad;
+  double d;
+  dcl_coor X;  /* global index units */
+  dcl_coor op;
+} dcl_irregular_node;
+
+/* LevelSetUpdateIrregularNodeList (LevelSetMethod/IrregularNodes.c:8-28): the
+ * records of every band node in the reference's append order, with the
+ * orthogonal projections of OrthogonalProjection3D.c:6-208 /
+ * OrthogonalProjection.c:6-184 (Newton on the tricubic / bicubic of the
+ * cells around the node, quadratic and linear fallbacks) computed in double.
+ * phi: device window (DCL_F64 or DCL_F32, read as double); band: device
+ * nodes; out: device, capacity records.  Synchronous: *count_out (host) is
+ * the total, which may exceed capacity (then only capacity are written).
+ * Stencil reads clamp into the window.  band_len < 2^31. */
+int dcl_irregular_nodes(const void *phi, const dcl_window *win, const dcl_icoor *band,
+                        int64_t band_len, int precision, dcl_irregular_node *out,
+                        int64_t capacity, int64_t *count_out, void *stream);
+
 /* Interface measure with LevelSetDiracDelta3D / 2D
  * (LevelSetMethod/NormalDirection.c:36-57) at the nodes [p+1, p+n-2]:
  * out[0] = sum delta dV (area; length in 2D), out[1] = sum delta kappa dV

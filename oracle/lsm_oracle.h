@@ -109,6 +109,28 @@ int orc_sl_adjoint(const double *phi0, const double *vel, const double *lam_next
 void orc_error_norms(const double *a, const double *b, const uint8_t *mask,
                      int64_t n, double out[4]);
 
+/* Irregular-node list (LevelSetMethod/IrregularNodes.c:8-140,
+ * IrregularNodes3D.c:6-117); irregular_oracle.c.  One record per node the
+ * reference appends, in its order; the fields the reference sets, others 0.
+ * Returns the number of records (records past capacity are not written). */
+typedef struct {
+  orc_icoor pos;
+  int32_t axis;        /* 0..2, -1 for the orthogonal projection record */
+  int32_t shift;       /* -1 projection, 0 cell centre, 1..3 U / V / W face */
+  int32_t sign_center;
+  int32_t sign_face;
+  int32_t pad;
+  double d;
+  double X[3];
+  double op[3];
+} orc_irregular_node;
+
+int64_t orc_irregular_nodes(const double *phi, const orc_win *w, const orc_icoor *band, int64_t band_len,
+                            orc_irregular_node *out, int64_t capacity);
+/* the evaluators of the projections' Newton iterations: p and gradient */
+void orc_irr_tricubic(const double *st64, double x, double y, double z, double out[4]);
+void orc_irr_bicubic(const double *st16, double x, double y, double out[3]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -90,6 +90,10 @@ def lib():
         L.orc_sl_adjoint.argtypes = [vp, vp, vp, vp, vp, W, W, C.c_int, C.c_int, C.c_double,
                                      C.c_int, C.c_int]
         L.orc_error_norms.argtypes = [vp, vp, vp, C.c_int64, vp]
+        L.orc_irregular_nodes.argtypes = [vp, W, vp, C.c_int64, vp, C.c_int64]
+        L.orc_irregular_nodes.restype = C.c_int64
+        L.orc_irr_tricubic.argtypes = [vp, C.c_double, C.c_double, C.c_double, vp]
+        L.orc_irr_bicubic.argtypes = [vp, C.c_double, C.c_double, vp]
         _lib = L
     return _lib
 
@@ -262,6 +266,39 @@ def band_compact(phi, win: Win, width):
     out = np.zeros((cap, 3), dtype=np.int32)
     n = lib().orc_band_compact(_ptr(phi), C.byref(win.c()), float(width), _ptr(out), cap)
     return out[:n]
+
+
+# one irregular-node record (orc_irregular_node, 88 bytes)
+IRREG_DTYPE = np.dtype([("pos", np.int32, 3), ("axis", np.int32), ("shift", np.int32),
+                        ("sign_center", np.int32), ("sign_face", np.int32), ("pad", np.int32),
+                        ("d", np.float64), ("X", np.float64, 3), ("op", np.float64, 3)])
+assert IRREG_DTYPE.itemsize == 88
+
+
+def irregular_nodes(phi, win: Win, band):
+    """LevelSetUpdateIrregularNodeList (LevelSetMethod/IrregularNodes.c:8-140,
+    IrregularNodes3D.c:6-117): the records, in the reference's order."""
+    phi = _f64(phi)
+    band = np.ascontiguousarray(band, dtype=np.int32).reshape(-1, 3)
+    cap = 10 * len(band)
+    out = np.zeros(cap, dtype=IRREG_DTYPE)
+    n = lib().orc_irregular_nodes(_ptr(phi), C.byref(win.c()), _ptr(band), len(band), _ptr(out), cap)
+    return out[:n]
+
+
+def irr_tricubic(st64, x, y, z):
+    """p, px, py, pz of the projections' tricubic on a 4x4x4 stencil."""
+    st = _f64(st64).reshape(-1)
+    out = np.zeros(4)
+    lib().orc_irr_tricubic(_ptr(st), float(x), float(y), float(z), _ptr(out))
+    return out
+
+
+def irr_bicubic(st16, x, y):
+    st = _f64(st16).reshape(-1)
+    out = np.zeros(3)
+    lib().orc_irr_bicubic(_ptr(st), float(x), float(y), _ptr(out))
+    return out
 
 
 def interface_metrics(phi, win: Win, kappa=None, eps=1.5):

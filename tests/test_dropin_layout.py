@@ -67,5 +67,6 @@ def test_dropin_libraries_export_reference_names():
                  "ParticleLS_AdvectParticles2D", "ParticleLS_AdvectParticles3D",
                  "ParticleLS_ErrorCorrection2D", "ParticleLS_ErrorCorrection3D", "ParticleLS_AdjustRadii",
                  "LevelSetCFLIncrement", "LevelSet_MaxVelocity", "LevelSetCurvature",
-                 "LevelSetAdvectSLSetScheme", "LevelSetPLSSetReferenceCornerZ", "LevelSetGPUFinalize"):
+                 "LevelSetUpdateIrregularNodeList", "LevelSetUpdateIrregularNodeList_2D",
+                 "LevelSetUpdateIrregularNodeList_3D", "LevelSetAdvectSLSetScheme", "LevelSetPLSSetReferenceCornerZ", "LevelSetGPUFinalize"):
         assert f" T {name}\n" in syms, name

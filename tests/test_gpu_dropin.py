@@ -108,6 +108,10 @@ def test_advect_and_reinit_3d_offset_velocity_window(dev, scheme):
         if cfl > 0.25:              # threshold crossed: counters reset (Advect.c:42-48)
             cfl = 0.0
         assert ls.contents.CFLcount == pytest.approx(cfl, rel=1e-12, abs=1e-15)
+        # the node list is rebuilt from the advected phi (Advect.c:41), and the
+        # next step's max |V| runs over it
+        pick = O.irregular_nodes(ref, w, band)["pos"]
+        assert L.ArrayLength(ls.contents.irregularNodes) == len(pick)
     DI.check(L.LevelSetAdvectSLSetScheme(0, 0), "scheme")
     L.GridDestroy(vg)
     L.LevelSetDestroy(ls)
@@ -225,3 +229,66 @@ def test_advect_pls_rk2_2d(dev):
     L.GridDestroy(vg)
     L.LevelSetDestroy(ls)
     DI.check(L.LevelSetGPUFinalize(), "finalize")
+
+
+def _check_irregular(got, ref):
+    assert len(got) == len(ref)
+    np.testing.assert_array_equal(np.stack([got["pos"][c] for c in "xyz"], axis=1), ref["pos"])
+    for f, g in (("axis", "axis"), ("shift", "shift"), ("signCenter", "sign_center"),
+                 ("signFace", "sign_face"), ("d", "d")):
+        np.testing.assert_array_equal(got[f], ref[g])
+    for f in ("X", "op"):
+        np.testing.assert_array_equal(np.stack([got[f][c] for c in "xyz"], axis=1), ref[f])
+    assert np.all(got["k"] == 0) and np.all(got["numNei"] == 0)
+
+
+def _irregular_of(ls):
+    size = C.sizeof(DI.IrregularNodeS)
+    raw = DI.get_array(ls.contents.irregularNodes, np.uint8, size)
+    return np.frombuffer(raw.tobytes(), dtype=np.dtype(DI.IrregularNodeS))
+
+
+def test_update_irregular_node_list(dev):
+    """LevelSetUpdateIrregularNodeList (IrregularNodes.c:8-28): the records
+    land in ls->irregularNodes in the reference's order, with the fields the
+    reference sets equal to the oracle's bit for bit and the rest zero."""
+    L = DI.lib()
+    h = 1.0 / 40
+    p, n = (-3, 4, 1), (36, 34, 30)
+    ls = DI.create_levelset((h, h, h), p, n)
+    phi = DI.grid_array(ls.contents.phi)
+    K, J, I = np.meshgrid(*(np.arange(n[a]) for a in (2, 1, 0)), indexing="ij")
+    phi[:] = np.sqrt((I - 17.3) ** 2 + (J - 16.1) ** 2 + (K - 15.2) ** 2) - 9.0
+    phi += 0.05 * np.sin(0.7 * I) * np.cos(0.5 * J)
+    band = _band_of(phi, p, 3.0, 3)
+    band = band[np.random.default_rng(8).permutation(len(band))]
+    DI.set_array(ls.contents.band, band, 12)
+    DI.check(L.LevelSetUpdateIrregularNodeList(ls), "LevelSetUpdateIrregularNodeList")
+    ref = O.irregular_nodes(phi, O.Win(p, n, (h, h, h)), band)
+    assert len(ref) > 1000
+    _check_irregular(_irregular_of(ls), ref)
+    # the 3D half appends to the list, as IrregularNodes3D.c:6-117 does
+    DI.check(L.LevelSetUpdateIrregularNodeList_3D(ls), "LevelSetUpdateIrregularNodeList_3D")
+    _check_irregular(_irregular_of(ls), np.concatenate([ref, ref]))
+    DI.set_array(ls.contents.band, np.zeros((0, 3), np.int32), 12)
+    DI.check(L.LevelSetUpdateIrregularNodeList(ls), "LevelSetUpdateIrregularNodeList")
+    assert L.ArrayLength(ls.contents.irregularNodes) == 0
+    L.LevelSetDestroy(ls)
+
+
+def test_update_irregular_node_list_2d(dev):
+    """The 2D list (IrregularNodes.c:30-140): the star of advectSLRK2.c's
+    circle of radius 10 at dx = 1, bicubic projections."""
+    L = DI.lib()
+    p, n = (-22, -22, 0), (44, 44, 0)
+    ls = DI.create_levelset((1.0, 1.0, 0.0), p, n)
+    phi = DI.grid_array(ls.contents.phi)
+    J, I = np.meshgrid(np.arange(n[1]) + p[1], np.arange(n[0]) + p[0], indexing="ij")
+    phi[:] = np.sqrt(I * I + J * J) - 10.0 + 0.3 * np.sin(0.9 * I)
+    band = _band_of(phi, p, 4.0, 2)
+    DI.set_array(ls.contents.band, band, 12)
+    DI.check(L.LevelSetUpdateIrregularNodeList(ls), "LevelSetUpdateIrregularNodeList")
+    ref = O.irregular_nodes(phi, O.Win(p, (n[0], n[1], 1), (1.0, 1.0, 1.0)), band)
+    assert len(ref) > 100
+    _check_irregular(_irregular_of(ls), ref)
+    L.LevelSetDestroy(ls)

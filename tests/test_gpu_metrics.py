@@ -51,6 +51,25 @@ def test_band_compact_raster_order(dev, dtype):
     assert count2 == len(ref) and np.array_equal(to_np(b2).astype(np.int32), ref[:100])
 
 
+def test_band_compact_many_scan_tiles(dev):
+    """A window of 288M nodes: 1.1M count blocks, so the scan of the block
+    counts runs over more than 1024 tiles of 1024 (the carry across chunks of
+    the tile sums).  Checked against torch.nonzero, whose indices come in the
+    same raster order."""
+    nx, ny, nz = 1024, 1024, 275
+    win = D.GridWindow((3, -7, 11), (nx, ny, nz), (1, 1, 1))
+    z = torch.arange(nz, device="cuda:0", dtype=torch.float32).view(nz, 1, 1)
+    x = torch.arange(nx, device="cuda:0", dtype=torch.float32).view(1, 1, nx)
+    phi = (z - 137.3) + 0.8 * torch.sin(0.05 * x) * torch.ones((1, ny, 1), device="cuda:0")
+    b, count = M.band_compact(phi.contiguous(), win, 1.5, capacity=8_000_000)
+    ref = torch.nonzero(phi.abs() < 1.5)  # (z, y, x)
+    assert count == ref.shape[0]
+    ref_xyz = ref.flip(1).to(torch.int32) + torch.tensor([3, -7, 11], device="cuda:0", dtype=torch.int32)
+    assert torch.equal(b, ref_xyz)
+    del phi, b, ref, ref_xyz
+    torch.cuda.empty_cache()
+
+
 def test_interface_metrics_and_norms(dev):
     m = 129
     win = D.GridWindow.unit_box(m)
