@@ -472,7 +472,8 @@ def irregular_run(n=256, steps=20, width=6.0):
     headline sphere (n^3 FP64, r = 0.15) over its band |phi| < width cells
     (the reference's default band width, LevelSetMethod/LevelSet.c:8): band
     nodes/s and records/s of one dcl_irregular_nodes call (count, scan and
-    write kernels and the count's return to the host), device-resident."""
+    write kernels and the count's return to the host), device-resident; then
+    the surface quantities (dcl_surface_quantities) at every record."""
     import torch
     import dcell_amd as D
     from dcell_amd import configs
@@ -491,8 +492,20 @@ def irregular_run(n=256, steps=20, width=6.0):
         M.irregular_nodes_into(phi, win, band, out)
         times.append(time.perf_counter() - t0)
     t = float(np.median(times))
+    # the surface quantities (IIMUpdateSurfaceQuantities_3D) at every record
+    surf = torch.empty(max(count, 1) * M.SURFACE_DTYPE.itemsize, dtype=torch.uint8, device="cuda:0")
+    M.surface_quantities_into(phi, win, out, count, surf)
+    torch.cuda.synchronize()
+    stimes = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        M.surface_quantities_into(phi, win, out, count, surf)
+        torch.cuda.synchronize()
+        stimes.append(time.perf_counter() - t0)
+    ts = float(np.median(stimes))
     return dict(n=n, band_nodes=nb, records=count, ms_per_call=t * 1e3, band_nodes_per_s=nb / t,
-                records_per_s=count / t, band_width_cells=width)
+                records_per_s=count / t, band_width_cells=width,
+                surface_quantities=dict(ms_per_call=ts * 1e3, records_per_s=count / ts))
 
 
 def copy_ceiling():

@@ -16,6 +16,7 @@
 #include "LSM_private.h"
 
 PetscLogEvent EVENT_LevelSetUpdateIrregularNodeList;
+PetscLogEvent EVENT_IIMUpdateSurfaceQuantities;
 PetscLogEvent EVENT_LevelSetReinitialize;
 PetscLogEvent EVENT_LevelSetWriteIrregularNodeList;
 PetscLogEvent EVENT_LevelSetGetVelocity;

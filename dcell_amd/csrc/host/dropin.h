@@ -23,6 +23,7 @@ enum {
   DL_NODES,
   DL_KAPPA,
   DL_IRREG,
+  DL_SURF,
   DL_NSLOTS
 };
 

@@ -81,6 +81,18 @@ class IrregularNodeS(C.Structure):
                 + [("X", Coor), ("op", Coor), ("numNei", C.c_int), ("uj", C.c_double)])
 
 
+IrregularNodeP = C.POINTER(IrregularNodeS)
+# InterfacialForce (IIM/ImmersedInterfaceMethod.h:15)
+InterfacialForce = C.CFUNCTYPE(None, IrregularNodeP, C.c_void_p)
+
+
+class IIMS(C.Structure):
+    """struct _IIM (IIM/ImmersedInterfaceMethod.h:20-34)."""
+    _fields_ = [("lc", C.c_void_p), ("lsq", C.c_void_p), ("Np", C.c_int), ("eps", C.c_double), ("mu", C.c_double),
+                ("F", InterfacialForce), ("context", C.c_void_p), ("idx", C.c_void_p), ("coor", C.c_void_p),
+                ("val", C.c_void_p), ("debug", C.c_void_p), ("dh", Coor), ("sidx", C.c_void_p)]
+
+
 _L = None
 
 
@@ -123,6 +135,8 @@ def lib():
         L.ArrayGetData.argtypes = [vp]
         L.ArrayGetData.restype = vp
         L.ArrayLength.argtypes = [vp]
+        for f in ("IIMUpdateSurfaceQuantities_2D", "IIMUpdateSurfaceQuantities_3D"):
+            getattr(L, f).argtypes = [C.POINTER(IIMS), LevelSetP]
         for f in ("LevelSetAdvectSL", "LevelSetAdvectSLRK2HalfStep", "LevelSetAdvectSLRK2FullStep",
                   "LevelSetAdvectAndReinit", "LevelSetAdvectPLS", "LevelSetAdvectPLSRK2HalfStep",
                   "LevelSetAdvectPLSRK2FullStep", "LevelSetCFLIncrement"):

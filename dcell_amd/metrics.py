@@ -1,6 +1,8 @@
 """Reductions over the C ABI: max velocity (LevelSet_MaxVelocity,
 LevelSetMethod/Advect.c:70-111), band compaction, the irregular-node list
-(LevelSetUpdateIrregularNodeList, LevelSetMethod/IrregularNodes.c:8-28),
+(LevelSetUpdateIrregularNodeList, LevelSetMethod/IrregularNodes.c:8-28) and
+its surface quantities (IIMUpdateSurfaceQuantities_3D / _2D,
+IIM/SurfaceQuantities.c:31-118),
 interface metrics (LevelSetDiracDelta3D/2D, NormalDirection.c:36-57) and
 error norms."""
 from __future__ import annotations
@@ -88,3 +90,34 @@ def irregular_nodes(phi, win: GridWindow, band, capacity: int | None = None):
     n = min(count, cap)
     host = out[:n * IRREGULAR_DTYPE.itemsize].cpu().numpy()
     return host.view(IRREGULAR_DTYPE), count
+
+
+# the surface quantities of one record (dcl_surface, 104 bytes)
+SURFACE_DTYPE = np.dtype([("n", np.float64, 3), ("s", np.float64, 3), ("r", np.float64, 3), ("k", np.float64),
+                          ("k_nn", np.float64), ("k_tt", np.float64), ("k_nt", np.float64)])
+assert SURFACE_DTYPE.itemsize == 104
+
+
+def surface_quantities_into(phi, win: GridWindow, records, count: int, out) -> None:
+    """Device-resident form: the surface quantities at the X of the first
+    `count` records in `records` (a uint8 device tensor of 88-byte records, as
+    irregular_nodes_into writes them) into `out` (a uint8 device tensor of at
+    least count * 104 bytes), asynchronously on the current stream."""
+    _dev(phi, "phi")
+    _dev(records, "records")
+    _dev(out, "out")
+    if records.numel() < count * IRREGULAR_DTYPE.itemsize or out.numel() < count * SURFACE_DTYPE.itemsize:
+        raise ValueError("records or out holds fewer than count entries")
+    L.call("dcl_surface_quantities", _p(phi), C.byref(win.c()), _p(records), int(count), precision_of(phi),
+           _p(out), stream_handle())
+
+
+def surface_quantities(phi, win: GridWindow, records: np.ndarray) -> np.ndarray:
+    """The surface quantities (SURFACE_DTYPE) at the X of host records
+    (IRREGULAR_DTYPE), computed on phi's device."""
+    recs = np.ascontiguousarray(records, dtype=IRREGULAR_DTYPE)
+    n = len(recs)
+    dev_recs = torch.from_numpy(recs.view(np.uint8).reshape(-1).copy()).to(phi.device)
+    out = torch.empty(max(n, 1) * SURFACE_DTYPE.itemsize, dtype=torch.uint8, device=phi.device)
+    surface_quantities_into(phi, win, dev_recs, n, out)
+    return out[:n * SURFACE_DTYPE.itemsize].cpu().numpy().view(SURFACE_DTYPE)

@@ -224,6 +224,29 @@ int dcl_irregular_nodes(const void *phi, const dcl_window *win, const dcl_icoor 
                         int64_t band_len, int precision, dcl_irregular_node *out,
                         int64_t capacity, int64_t *count_out, void *stream);
 
+/* The surface quantities of one irregular-node record: the fields that
+ * IIMUpdateSurfaceQuantities_3D / _2D (IIM/SurfaceQuantities.c:31-118) set
+ * in the reference's IrregularNode (LevelSetMethod.h:30-54): unit normal n
+ * (nx, ny, nz), tangents s and r (LocalCoor3DTangential,
+ * IIM/LocalCoor.c:38-65), curvature k and the principal curvatures k_nn,
+ * k_tt, k_nt.  In 2D: s = (ny, -nx), r = 0, k_nn = k, k_tt = k_nt = 0.
+ * 104 bytes. */
+typedef struct {
+  dcl_coor n, s, r;
+  double k, k_nn, k_tt, k_nt;
+} dcl_surface;
+
+/* IIMUpdateSurfaceQuantities_3D / _2D (IIM/SurfaceQuantities.c:31-118)
+ * without the user's force callback (iim->F): at the X of each of the count
+ * records in nodes (device, as written by dcl_irregular_nodes), the
+ * Bilinear3D / Bilinear2D (Common/Grid3D.c:4-31, Common/Grid.c:190-212)
+ * interpolations of the central-difference functionals
+ * (Common/Grid3D.c:38-108, Common/Grid.c:219-254), in double.  phi: device
+ * window (DCL_F64, or DCL_F32 read as double); out: device, count records.
+ * Asynchronous on stream.  Reads clamp into the window.  count < 2^31. */
+int dcl_surface_quantities(const void *phi, const dcl_window *win, const dcl_irregular_node *nodes,
+                           int64_t count, int precision, dcl_surface *out, void *stream);
+
 /* Interface measure with LevelSetDiracDelta3D / 2D
  * (LevelSetMethod/NormalDirection.c:36-57) at the nodes [p+1, p+n-2]:
  * out[0] = sum delta dV (area; length in 2D), out[1] = sum delta kappa dV

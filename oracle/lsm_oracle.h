@@ -131,6 +131,18 @@ int64_t orc_irregular_nodes(const double *phi, const orc_win *w, const orc_icoor
 void orc_irr_tricubic(const double *st64, double x, double y, double z, double out[4]);
 void orc_irr_bicubic(const double *st16, double x, double y, double out[3]);
 
+/* Surface quantities at irregular nodes (IIM/SurfaceQuantities.c:31-118);
+ * surface_oracle.c.  At each record's X: unit normal n, tangents s, r (2D:
+ * s = (ny, -nx), r = 0), curvature k, principal curvatures k_nn, k_tt, k_nt
+ * (2D: k_nn = k, k_tt = k_nt = 0). */
+typedef struct {
+  double n[3], s[3], r[3];
+  double k, k_nn, k_tt, k_nt;
+} orc_surface;
+
+void orc_surface_quantities(const double *phi, const orc_win *w, const orc_irregular_node *nodes, int64_t count,
+                            orc_surface *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -94,6 +94,7 @@ def lib():
         L.orc_irregular_nodes.restype = C.c_int64
         L.orc_irr_tricubic.argtypes = [vp, C.c_double, C.c_double, C.c_double, vp]
         L.orc_irr_bicubic.argtypes = [vp, C.c_double, C.c_double, vp]
+        L.orc_surface_quantities.argtypes = [vp, W, vp, C.c_int64, vp]
         _lib = L
     return _lib
 
@@ -327,3 +328,27 @@ def error_norms(a, b, mask=None):
     out = np.zeros(4)
     lib().orc_error_norms(_ptr(a), _ptr(b), _ptr(m), a.size, _ptr(out))
     return out
+
+
+# surface quantities of one record (orc_surface, 104 bytes)
+SURF_DTYPE = np.dtype([("n", np.float64, 3), ("s", np.float64, 3), ("r", np.float64, 3), ("k", np.float64),
+                       ("k_nn", np.float64), ("k_tt", np.float64), ("k_nt", np.float64)])
+assert SURF_DTYPE.itemsize == 104
+
+
+def surface_quantities(phi, win: Win, records):
+    """IIMUpdateSurfaceQuantities_3D / _2D (IIM/SurfaceQuantities.c:31-118) at
+    the X of each irregular-node record (IRREG_DTYPE)."""
+    phi = _f64(phi)
+    recs = np.ascontiguousarray(records, dtype=IRREG_DTYPE)
+    out = np.zeros(len(recs), dtype=SURF_DTYPE)
+    lib().orc_surface_quantities(_ptr(phi), C.byref(win.c()), _ptr(recs), len(recs), _ptr(out))
+    return out
+
+
+def surface_at(phi, win: Win, X):
+    """The surface quantities at arbitrary points X (n, 3) (records with only X set)."""
+    X = np.asarray(X, dtype=np.float64).reshape(-1, 3)
+    recs = np.zeros(len(X), dtype=IRREG_DTYPE)
+    recs["X"] = X
+    return surface_quantities(phi, win, recs)

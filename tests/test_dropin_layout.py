@@ -16,17 +16,19 @@ PROBE = r"""
 #include <stddef.h>
 #include "LevelSetMethod.h"
 #include "LSM_private.h"
+#include "ImmersedInterfaceMethod.h"
 #define F(T, f) printf("%s.%s %zu\n", #T, #f, offsetof(struct T, f))
 int main(void) {
-  printf("Grid %zu\nLevelSet %zu\nIrregularNode %zu\nParticle %zu\nParticleLS %zu\n",
+  printf("Grid %zu\nLevelSet %zu\nIrregularNode %zu\nParticle %zu\nParticleLS %zu\nIIM %zu\n",
          sizeof(struct _Grid), sizeof(struct _LevelSet), sizeof(struct _IrregularNode),
-         sizeof(struct _Particle), sizeof(struct _ParticleLS));
+         sizeof(struct _Particle), sizeof(struct _ParticleLS), sizeof(struct _IIM));
   F(_Grid, p); F(_Grid, n); F(_Grid, d); F(_Grid, is2D); F(_Grid, v1); F(_Grid, dof);
   F(_Grid, SIZE); F(_Grid, Interpolate);
   F(_LevelSet, band); F(_LevelSet, CFLcount); F(_LevelSet, AdvectCount); F(_LevelSet, maxVel);
   F(_LevelSet, pls); F(_LevelSet, Advect);
   F(_IrregularNode, nx); F(_IrregularNode, k); F(_IrregularNode, X); F(_IrregularNode, uj);
   F(_ParticleLS, S_INIT); F(_ParticleLS, AdvectParticles); F(_ParticleLS, particles0);
+  F(_IIM, Np); F(_IIM, eps); F(_IIM, F); F(_IIM, context); F(_IIM, dh); F(_IIM, sidx);
   return 0;
 }
 """
@@ -44,7 +46,7 @@ def test_struct_layouts(tmp_path):
     assert out["Grid"] == 4096 + 8 + 24 + 24 + 8 + 8 + 8 + 12 + 4 + 16 + 8
     assert out["Particle"] == 32
     assert out["IrregularNode"] == C.sizeof(DI.IrregularNodeS)
-    mirror = {"Grid": DI.GridS, "LevelSet": DI.LevelSetS, "ParticleLS": DI.ParticleLSS}
+    mirror = {"Grid": DI.GridS, "LevelSet": DI.LevelSetS, "ParticleLS": DI.ParticleLSS, "IIM": DI.IIMS}
     for name, cls in mirror.items():
         assert out[name] == C.sizeof(cls), name
     for key, off in out.items():
@@ -52,7 +54,7 @@ def test_struct_layouts(tmp_path):
             continue
         struct, field = key.split(".")
         cls = {"_Grid": DI.GridS, "_LevelSet": DI.LevelSetS, "_IrregularNode": DI.IrregularNodeS,
-               "_ParticleLS": DI.ParticleLSS}[struct]
+               "_ParticleLS": DI.ParticleLSS, "_IIM": DI.IIMS}[struct]
         assert getattr(cls, field).offset == off, key
 
 
@@ -68,5 +70,6 @@ def test_dropin_libraries_export_reference_names():
                  "ParticleLS_ErrorCorrection2D", "ParticleLS_ErrorCorrection3D", "ParticleLS_AdjustRadii",
                  "LevelSetCFLIncrement", "LevelSet_MaxVelocity", "LevelSetCurvature",
                  "LevelSetUpdateIrregularNodeList", "LevelSetUpdateIrregularNodeList_2D",
-                 "LevelSetUpdateIrregularNodeList_3D", "LevelSetAdvectSLSetScheme", "LevelSetPLSSetReferenceCornerZ", "LevelSetGPUFinalize"):
+                 "LevelSetUpdateIrregularNodeList_3D", "IIMUpdateSurfaceQuantities_2D",
+                 "IIMUpdateSurfaceQuantities_3D", "LevelSetAdvectSLSetScheme", "LevelSetPLSSetReferenceCornerZ", "LevelSetGPUFinalize"):
         assert f" T {name}\n" in syms, name

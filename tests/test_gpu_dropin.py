@@ -292,3 +292,60 @@ def test_update_irregular_node_list_2d(dev):
     assert len(ref) > 100
     _check_irregular(_irregular_of(ls), ref)
     L.LevelSetDestroy(ls)
+
+
+@pytest.mark.parametrize("dim", [3, 2])
+def test_iim_update_surface_quantities(dev, dim):
+    """IIMUpdateSurfaceQuantities_3D / _2D (IIM/SurfaceQuantities.c:31-118)
+    after LevelSetUpdateIrregularNodeList: the nodes' normal, tangents and
+    curvatures equal the oracle's bit for bit, and the interfacial force runs
+    once per node, in list order, after the node's quantities are set."""
+    L = DI.lib()
+    if dim == 3:
+        h = 1.0 / 40
+        p, n = (-3, 4, 1), (36, 34, 30)
+        ls = DI.create_levelset((h, h, h), p, n)
+        phi = DI.grid_array(ls.contents.phi)
+        K, J, I = np.meshgrid(*(np.arange(n[a]) for a in (2, 1, 0)), indexing="ij")
+        phi[:] = (np.sqrt((I - 17.3) ** 2 + (J - 16.1) ** 2 + (K - 15.2) ** 2) - 9.0) * h
+        w = O.Win(p, n, (h, h, h))
+    else:
+        h = 0.05
+        p, n = (-22, -22, 0), (44, 44, 0)
+        ls = DI.create_levelset((h, h, 0.0), p, n)
+        phi = DI.grid_array(ls.contents.phi)
+        J, I = np.meshgrid(np.arange(n[1]) + p[1], np.arange(n[0]) + p[0], indexing="ij")
+        phi[:] = (np.sqrt(I * I + J * J) - 10.0 + 0.3 * np.sin(0.9 * I)) * h
+        w = O.Win(p, (n[0], n[1], 1), (h, h, 1.0))
+    band = _band_of(phi, p, 3.0 * h, dim)
+    DI.set_array(ls.contents.band, band, 12)
+    DI.check(L.LevelSetUpdateIrregularNodeList(ls), "LevelSetUpdateIrregularNodeList")
+    seen = []
+
+    def force(node, ctx):
+        nd = node.contents
+        seen.append((nd.pos.x, nd.pos.y, nd.k))
+        nd.F1 = 2.0 * nd.k
+        nd.F2 = nd.nx
+
+    cb = DI.InterfacialForce(force)
+    iim = DI.IIMS()
+    iim.F = cb
+    fn = L.IIMUpdateSurfaceQuantities_3D if dim == 3 else L.IIMUpdateSurfaceQuantities_2D
+    DI.check(fn(C.byref(iim), ls), "IIMUpdateSurfaceQuantities")
+    got = _irregular_of(ls)
+    recs = O.irregular_nodes(phi, w, band)
+    ref = O.surface_quantities(phi, w, recs)
+    assert len(got) == len(ref) > 100
+    for f, (g, c) in {"nx": ("n", 0), "ny": ("n", 1), "nz": ("n", 2), "sx": ("s", 0), "sy": ("s", 1),
+                      "sz": ("s", 2), "rx": ("r", 0), "ry": ("r", 1), "rz": ("r", 2)}.items():
+        np.testing.assert_array_equal(got[f], ref[g][:, c])
+    for f in ("k", "k_nn", "k_tt", "k_nt"):
+        np.testing.assert_array_equal(got[f], ref[f])
+    assert len(seen) == len(got)
+    np.testing.assert_array_equal(np.array([s[2] for s in seen]), ref["k"])
+    np.testing.assert_array_equal(got["F1"], 2.0 * ref["k"])
+    np.testing.assert_array_equal(got["F2"], ref["n"][:, 0])
+    if dim == 2:
+        assert iim.dh.x == h and iim.dh.y == h
+    L.LevelSetDestroy(ls)
