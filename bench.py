@@ -508,6 +508,31 @@ def irregular_run(n=256, steps=20, width=6.0):
                 surface_quantities=dict(ms_per_call=ts * 1e3, records_per_s=count / ts))
 
 
+def get_velocity_run(n=256, steps=20):
+    """LevelSetGetVelocity from a device-resident fluid field: the (u, v, w)
+    window of an n^3 level set out of an n^3 FP64 fluid array of (p, u, v, w)
+    records.  Kernel time from HIP events on the current stream; 48 B per node
+    (24 read, 24 written) against the HBM peak."""
+    import torch
+    import dcell_amd as D
+
+    fluid = torch.rand((n, n, n, 4), dtype=torch.float64, device="cuda:0")
+    win = D.GridWindow((0, 0, 0), (n, n, n), (1.0 / (n - 1),) * 3)
+    out = torch.empty((n, n, n, 3), dtype=torch.float64, device="cuda:0")
+    D.get_velocity(fluid, win, out=out)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        D.get_velocity(fluid, win, out=out)
+    e1.record()
+    torch.cuda.synchronize()
+    t = e0.elapsed_time(e1) * 1e-3 / steps
+    gbs = n ** 3 * 48 / t / 1e9
+    return dict(n=n, ms_per_call=t * 1e3, nodes_per_s=n ** 3 / t, achieved_gbs=gbs,
+                roofline_frac=gbs * 1e9 / HBM_PEAK, bytes_per_node=48)
+
+
 def copy_ceiling():
     """The measured HBM ceiling: a device-to-device copy of 1 GiB (torch),
     best of 5, read + write bytes over its time."""
@@ -549,6 +574,7 @@ FORMS = {
     "zalesak_2d_128_fp64_config_scheme": lambda: zalesak_run(128, 628, "cubic", "rk2"),
     "zalesak_2d_128_fp64_reference_scheme": lambda: zalesak_run(128, 628, "linear", "euler"),
     "irregular_nodes_256_fp64": lambda: irregular_run(256, 20),
+    "get_velocity_256_fp64": lambda: get_velocity_run(256, 20),
 }
 
 

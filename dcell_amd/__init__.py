@@ -6,7 +6,7 @@ Python package is the host-side mirror used by tests, the benchmark and the
 autograd wrapper of the adjoint.
 """
 from . import _lib  # noqa: F401
-from .levelset import (GridWindow, advect_sl, curvature, levelset_step,  # noqa: F401
+from .levelset import (GridWindow, advect_sl, curvature, get_velocity, levelset_step,  # noqa: F401
                        precision_of, velocity)
 
-__all__ = ["GridWindow", "advect_sl", "curvature", "levelset_step", "precision_of", "velocity"]
+__all__ = ["GridWindow", "advect_sl", "curvature", "get_velocity", "levelset_step", "precision_of", "velocity"]

@@ -165,3 +165,27 @@ def levelset_step(phi0, vel, win: GridWindow, dt: float, interp="linear", backtr
     L.call("dcl_levelset_step", _p(phi0), _p(out_phi), _p(out_kappa), C.byref(win.c()),
            C.byref(V), float(dt), prec, INTERP[interp], BACKTRACE[backtrace], stream_handle())
     return out_phi, out_kappa
+
+
+def get_velocity(fluid, win: GridWindow, out=None):
+    """LevelSetGetVelocity (LevelSetMethod/Advect.c:145-192) from a device
+    fluid field: `fluid` is the fluid's (Nz, Ny, Nx, 4) array of (p, u, v, w)
+    (2D: (Ny, Nx, 3) of (p, u, v)) over its domain with node (0, 0, 0) at the
+    global origin.  Returns the velocity window (win's shape + (3,) or (2,)):
+    (u, v, w) at the nodes of win inside the domain; the other nodes of `out`
+    keep their values (zeros for a new tensor)."""
+    _dev(fluid, "fluid")
+    d3 = fluid.dim() == 4
+    if fluid.shape[-1] != (4 if d3 else 3):
+        raise ValueError("fluid must be (Nz, Ny, Nx, 4) or (Ny, Nx, 3)")
+    dim = 3 if d3 else 2
+    shape = win.shape + (dim,)
+    if out is None:
+        out = torch.zeros(shape, dtype=fluid.dtype, device=fluid.device)
+    _dev(out, "out", fluid.dtype)
+    if tuple(out.shape) != shape:
+        raise ValueError(f"out has shape {tuple(out.shape)}, expected {shape}")
+    fn = L.ICoor(fluid.shape[-2], fluid.shape[-3], fluid.shape[0] if d3 else 1)
+    L.call("dcl_get_velocity", _p(fluid), C.byref(fn), C.byref(win.c()), precision_of(fluid), _p(out),
+           stream_handle())
+    return out

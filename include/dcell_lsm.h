@@ -224,6 +224,18 @@ int dcl_irregular_nodes(const void *phi, const dcl_window *win, const dcl_icoor 
                         int64_t band_len, int precision, dcl_irregular_node *out,
                         int64_t capacity, int64_t *count_out, void *stream);
 
+/* LevelSetGetVelocity (LevelSetMethod/Advect.c:145-192) from a fluid field
+ * resident on the device.  fluid: the fluid's [z][y][x][4] array (p, u, v,
+ * w) over its domain of fluid_n nodes, node (0,0,0) at global (0,0,0), 3D
+ * records aligned to their size (32 bytes in DCL_F64, 16 in DCL_F32); in 2D
+ * (fluid_n->z < 2) [y][x][3] (p, u, v).  Copies the velocity components
+ * (U_FACE..W_FACE) at the nodes of win inside the domain into vel, the
+ * [z][y][x][3] ([y][x][2]) array over win; nodes of win outside the domain
+ * are not written (the reference's get is clipped to the domain,
+ * Advect.c:160-186).  Asynchronous on stream. */
+int dcl_get_velocity(const void *fluid, const dcl_icoor *fluid_n, const dcl_window *win, int precision,
+                     void *vel, void *stream);
+
 /* The surface quantities of one irregular-node record: the fields that
  * IIMUpdateSurfaceQuantities_3D / _2D (IIM/SurfaceQuantities.c:31-118) set
  * in the reference's IrregularNode (LevelSetMethod.h:30-54): unit normal n

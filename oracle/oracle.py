@@ -352,3 +352,29 @@ def surface_at(phi, win: Win, X):
     recs = np.zeros(len(X), dtype=IRREG_DTYPE)
     recs["X"] = X
     return surface_quantities(phi, win, recs)
+
+
+def get_velocity(fluid, win: Win, vel):
+    """LevelSetGetVelocity (LevelSetMethod/Advect.c:145-192): the velocity
+    components (U_FACE..W_FACE) of the fluid's [z][y][x][p,u,v,w] array
+    ([y][x][p,u,v] in 2D), node (0,0,0) at the global origin, copied into a
+    copy of vel (win's shape + (dim,)) over the window's box clipped to the
+    domain (:160-186); the rest of vel is kept."""
+    fluid = np.asarray(fluid)
+    out = np.array(vel, copy=True)
+    if win.is2d:
+        N = fluid.shape[:2]
+        lo = [max(win.p[1], 0), max(win.p[0], 0)]
+        hi = [min(win.p[1] + win.n[1], N[0]), min(win.p[0] + win.n[0], N[1])]
+        if hi[0] > lo[0] and hi[1] > lo[1]:
+            out[lo[0] - win.p[1]:hi[0] - win.p[1], lo[1] - win.p[0]:hi[1] - win.p[0], :] = \
+                fluid[lo[0]:hi[0], lo[1]:hi[1], 1:3]
+        return out
+    N = fluid.shape[:3]
+    p, n = (win.p[2], win.p[1], win.p[0]), (win.n[2], win.n[1], win.n[0])
+    lo = [max(p[a], 0) for a in range(3)]
+    hi = [min(p[a] + n[a], N[a]) for a in range(3)]
+    if all(hi[a] > lo[a] for a in range(3)):
+        out[tuple(slice(lo[a] - p[a], hi[a] - p[a]) for a in range(3))] = \
+            fluid[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2], 1:4]
+    return out
