@@ -25,7 +25,7 @@ EXPORTS = (
     "dcl_sl_advect", "dcl_curvature", "dcl_levelset_step",
     "dcl_pls_advect", "dcl_pls_ws_create", "dcl_pls_ws_destroy", "dcl_pls_correct",
     "dcl_pls_adjust_radii", "dcl_max_velocity", "dcl_band_compact", "dcl_irregular_nodes",
-    "dcl_surface_quantities", "dcl_get_velocity",
+    "dcl_surface_quantities", "dcl_get_velocity", "dcl_reinitialize",
     "dcl_interface_metrics", "dcl_error_norms", "dcl_sl_adjoint",
     "dcl_batch_create", "dcl_batch_destroy", "dcl_batch_layout", "dcl_batch_set_dt",
     "dcl_batch_set_config", "dcl_batch_get_config", "dcl_batch_step", "dcl_batch_autotune",
@@ -49,6 +49,13 @@ class Window(C.Structure):
 
 class Velocity(C.Structure):
     _fields_ = [("data", C.c_void_p), ("win", Window), ("dof", C.c_int32), ("udof", C.c_int32)]
+
+
+class ReinitCell(C.Structure):
+    """dcl_reinit_cell (include/dcell_lsm.h)."""
+    _fields_ = [("phi", C.c_void_p), ("win", Window), ("nodes", C.c_void_p), ("count", C.c_int64),
+                ("band_width", C.c_double), ("phi_inf", C.c_double), ("band", C.c_void_p),
+                ("band_capacity", C.c_int64), ("band_count", C.c_int64)]
 
 
 class DclError(RuntimeError):
@@ -86,6 +93,7 @@ def _declare(L):
         "dcl_irregular_nodes": ([vp, W, vp, i64, i32, vp, i64, C.POINTER(i64), vp], i32),
         "dcl_surface_quantities": ([vp, W, vp, i64, i32, vp, vp], i32),
         "dcl_get_velocity": ([vp, C.POINTER(ICoor), W, i32, vp, vp], i32),
+        "dcl_reinitialize": ([C.POINTER(ReinitCell), i32, i64, vp], i32),
         "dcl_interface_metrics": ([vp, vp, W, dbl, i32, C.POINTER(dbl), vp], i32),
         "dcl_error_norms": ([vp, i32, vp, i32, i64, C.POINTER(dbl), vp], i32),
         "dcl_sl_adjoint": ([vp, W, V, vp, vp, vp, dbl, i32, i32, i32, vp], i32),

@@ -2,7 +2,8 @@
 LevelSetMethod/Advect.c:70-111), band compaction, the irregular-node list
 (LevelSetUpdateIrregularNodeList, LevelSetMethod/IrregularNodes.c:8-28) and
 its surface quantities (IIMUpdateSurfaceQuantities_3D / _2D,
-IIM/SurfaceQuantities.c:31-118),
+IIM/SurfaceQuantities.c:31-118), the reinitialisation that rebuilds phi
+from it (LevelSetReinitialize, LevelSetMethod/FastMarching.c:10-146),
 interface metrics (LevelSetDiracDelta3D/2D, NormalDirection.c:36-57) and
 error norms."""
 from __future__ import annotations
@@ -121,3 +122,30 @@ def surface_quantities(phi, win: GridWindow, records: np.ndarray) -> np.ndarray:
     out = torch.empty(max(n, 1) * SURFACE_DTYPE.itemsize, dtype=torch.uint8, device=phi.device)
     surface_quantities_into(phi, win, dev_recs, n, out)
     return out[:n * SURFACE_DTYPE.itemsize].cpu().numpy().view(SURFACE_DTYPE)
+
+
+def reinitialize(cells, band_width: float = 6.0, phi_inf: float = 7.5, heap_capacity: int = 0):
+    """LevelSetReinitialize of a batch: `cells` is a list of (phi, win,
+    records, count) with phi an FP64 device tensor (rewritten in place) and
+    records a uint8 device tensor of irregular-node records (as
+    irregular_nodes_into writes them).  Returns, per level set, the new band
+    as an (n, 3) int32 device tensor in the reference's order."""
+    arr = (L.ReinitCell * len(cells))()
+    bands = []
+    for i, (phi, win, records, count) in enumerate(cells):
+        _dev(phi, "phi", torch.float64)
+        _dev(records, "records")
+        if records.numel() < count * IRREGULAR_DTYPE.itemsize:
+            raise ValueError("records holds fewer than count entries")
+        band = torch.empty((max(win.nodes, 1), 3), dtype=torch.int32, device=phi.device)
+        bands.append(band)
+        arr[i].phi = phi.data_ptr()
+        arr[i].win = win.c()
+        arr[i].nodes = records.data_ptr() if count else None
+        arr[i].count = int(count)
+        arr[i].band_width = float(band_width)
+        arr[i].phi_inf = float(phi_inf)
+        arr[i].band = band.data_ptr()
+        arr[i].band_capacity = win.nodes
+    L.call("dcl_reinitialize", arr, len(cells), int(heap_capacity), stream_handle())
+    return [b[:arr[i].band_count] for i, b in enumerate(bands)]

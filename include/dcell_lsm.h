@@ -224,6 +224,33 @@ int dcl_irregular_nodes(const void *phi, const dcl_window *win, const dcl_icoor 
                         int64_t band_len, int precision, dcl_irregular_node *out,
                         int64_t capacity, int64_t *count_out, void *stream);
 
+/* One level set of a reinitialisation batch (see dcl_reinitialize). */
+typedef struct {
+  void *phi;                       /* device FP64 window, rewritten in place */
+  dcl_window win;
+  const dcl_irregular_node *nodes; /* device: phi's irregular-node records */
+  int64_t count;
+  double band_width, phi_inf;      /* grid units; the reference's are 6 and 7.5 */
+  dcl_icoor *band;                 /* device: the new band, band_capacity entries */
+  int64_t band_capacity;
+  int64_t band_count;              /* out: the new band's length (may exceed capacity) */
+} dcl_reinit_cell;
+
+/* LevelSetReinitialize (LevelSetMethod/FastMarching.c:10-146) of ncells
+ * level sets: phi reset to +-phi_inf, the signed distance to the orthogonal
+ * projection at every projection record (axis -1) as the boundary values,
+ * then the second-order fast marching (FastMarching3D.c:14-101,
+ * FastMarching2D.c:33-185) out to band_width on each side, with the
+ * reference's heap order (Common/Heap.c:59-139), so phi and the band come
+ * out as the reference's: band = [projection nodes in record order | - side
+ * in acceptance order | + side in acceptance order].  Neighbours outside the
+ * window are not pushed (the reference reads them unchecked).
+ * heap_capacity: heap entries per side and level set (0: 2 * count + 4096;
+ * a full heap is an error).  All level sets of a call are 2D or all 3D.
+ * Synchronous: band_count is set on return.  Each side of each level set is
+ * one wave, so a batch runs its level sets side by side. */
+int dcl_reinitialize(dcl_reinit_cell *cells, int ncells, int64_t heap_capacity, void *stream);
+
 /* LevelSetGetVelocity (LevelSetMethod/Advect.c:145-192) from a fluid field
  * resident on the device.  fluid: the fluid's [z][y][x][4] array (p, u, v,
  * w) over its domain of fluid_n nodes, node (0,0,0) at global (0,0,0), 3D

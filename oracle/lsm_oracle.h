@@ -143,6 +143,14 @@ typedef struct {
 void orc_surface_quantities(const double *phi, const orc_win *w, const orc_irregular_node *nodes, int64_t count,
                             orc_surface *out);
 
+/* Reinitialisation (LevelSetMethod/FastMarching.c:10-146); reinit_oracle.c.
+ * phi (in/out): reset to +-phi_inf, then the boundary values at the
+ * projection records and the second-order fast marching out to band_width
+ * on each side.  band: the new band in the reference's order.  Returns its
+ * length (entries past band_cap are not written), or -1 on a NaN update. */
+int64_t orc_reinitialize(double *phi, const orc_win *w, const orc_irregular_node *nodes, int64_t count,
+                         double band_width, double phi_inf, orc_icoor *band, int64_t band_cap);
+
 #ifdef __cplusplus
 }
 #endif

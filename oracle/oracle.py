@@ -95,6 +95,8 @@ def lib():
         L.orc_irr_tricubic.argtypes = [vp, C.c_double, C.c_double, C.c_double, vp]
         L.orc_irr_bicubic.argtypes = [vp, C.c_double, C.c_double, vp]
         L.orc_surface_quantities.argtypes = [vp, W, vp, C.c_int64, vp]
+        L.orc_reinitialize.argtypes = [vp, W, vp, C.c_int64, C.c_double, C.c_double, vp, C.c_int64]
+        L.orc_reinitialize.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -378,3 +380,18 @@ def get_velocity(fluid, win: Win, vel):
         out[tuple(slice(lo[a] - p[a], hi[a] - p[a]) for a in range(3))] = \
             fluid[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2], 1:4]
     return out
+
+
+def reinitialize(phi, win: Win, records, band_width=6.0, phi_inf=7.5):
+    """LevelSetReinitialize (LevelSetMethod/FastMarching.c:10-146): returns the
+    new phi (a copy) and the new band, (n, 3) int32, in the reference's
+    order, from the irregular-node records (IRREG_DTYPE) of phi."""
+    out = np.array(_f64(phi), copy=True)
+    recs = np.ascontiguousarray(records, dtype=IRREG_DTYPE)
+    cap = out.size
+    band = np.zeros((cap, 3), dtype=np.int32)
+    n = lib().orc_reinitialize(_ptr(out), C.byref(win.c()), _ptr(recs), len(recs), float(band_width),
+                               float(phi_inf), _ptr(band), cap)
+    if n < 0:
+        raise RuntimeError("reinitialisation: NaN update")
+    return out, band[:n]
