@@ -244,6 +244,31 @@ def cpu_baseline(args):
     t0 = time.perf_counter()
     recs = O.irregular_nodes(phi, w, band)
     t_irr = time.perf_counter() - t0
+    # the reinitialisation form's inputs, by the oracle: the 256^3 sphere on
+    # one thread, and the 64-cell batch one cell per thread
+    import concurrent.futures as cf
+    wr = O.Win((0, 0, 0), (n, n, n), (1.0, 1.0, 1.0))
+    phir = 1.5 * configs.sphere_sdf(n)
+    recr = O.irregular_nodes(phir, wr, O.band_compact(phir, wr, 8.0))
+    t0 = time.perf_counter()
+    _, bandr = O.reinitialize(phir, wr, recr)
+    t_re = time.perf_counter() - t0
+    jobs = []
+    for e, c in reinit_cells():
+        we = O.Win((0, 0, 0), (e, e, e), (1.0, 1.0, 1.0))
+        pe = 1.5 * configs.sphere_sdf(e, c, 0.25)
+        jobs.append((pe, we, O.irregular_nodes(pe, we, O.band_compact(pe, we, 8.0))))
+
+    def one(job):
+        return len(O.reinitialize(*job)[1])
+
+    t0 = time.perf_counter()
+    nbatch = sum(one(j) for j in jobs)
+    t_b1 = time.perf_counter() - t0
+    with cf.ThreadPoolExecutor(nthreads) as ex:
+        t0 = time.perf_counter()
+        list(ex.map(one, jobs))
+        t_bn = time.perf_counter() - t0
     model = "unknown"
     try:
         with open("/proc/cpuinfo") as f:
@@ -258,7 +283,12 @@ def cpu_baseline(args):
                        "workload (RK2+tricubic SL + 19-point curvature, FP64, dense) by the oracle, 1 thread",
                 all_cores=dict(value=out[nthreads], cores=nthreads), cpu=model,
                 irregular_nodes=dict(band_nodes_per_s=len(band) / t_irr, records=len(recs), cores=1,
-                                     sample=f"the full {len(band)}-node band of the {n}^3 sphere, one call"))
+                                     sample=f"the full {len(band)}-node band of the {n}^3 sphere, one call"),
+                reinitialize=dict(single_ms=t_re * 1e3, single_band_nodes=len(bandr), cores=1,
+                                  batch_ms=t_b1 * 1e3, batch_band_nodes=nbatch,
+                                  batch_all_cores_ms=t_bn * 1e3, all_cores=nthreads,
+                                  sample=f"the reinitialisation form's inputs: the {n}^3 sphere (one call), "
+                                         "and the 64-cell batch, cells one after another and one per thread"))
 
 
 def batch_run(ncells, seed, steps, warmup, interp="cubic", bt="rk2", autotune=True):
@@ -533,6 +563,57 @@ def get_velocity_run(n=256, steps=20):
                 roofline_frac=gbs * 1e9 / HBM_PEAK, bytes_per_node=48)
 
 
+def _reinit_inputs(n, scale, center, radius, torch, D, M, dev="cuda:0"):
+    """phi = scale x the distance (grid units) to a sphere in an n^3 window,
+    its 8-cell band and irregular-node records, all on the device."""
+    from dcell_amd import configs
+    win = D.GridWindow((0, 0, 0), (n, n, n), (1.0, 1.0, 1.0))
+    phi = torch.from_numpy(scale * configs.sphere_sdf(n, center, radius)).to(dev, torch.float64)
+    band, nb = M.band_compact(phi, win, 8.0)
+    recs = torch.empty(max(10 * nb, 1) * M.IRREGULAR_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+    count = M.irregular_nodes_into(phi, win, band.contiguous(), recs)
+    return win, phi, recs, count
+
+
+def reinit_cells(ncells=64, seed=4):
+    """The batch of the reinitialisation form: ncells spheres with edges
+    log-uniform in [32, 96] (seed 4), radius n/4, centres jittered, phi a
+    1.5x scaled distance."""
+    rng = np.random.default_rng(seed)
+    edges = np.round(np.exp(rng.uniform(np.log(32), np.log(96), ncells))).astype(int)
+    jit = rng.uniform(-0.02, 0.02, (ncells, 3))
+    return [(int(e), (0.5 + j[0], 0.5 + j[1], 0.5 + j[2])) for e, j in zip(edges, jit)]
+
+
+def reinit_run(n=256, runs=5):
+    """LevelSetReinitialize on the GPU: the 256^3 sphere of the headline (r
+    0.15, phi a 1.5x scaled distance) as one level set (two waves, one per
+    sign), and a batch of 64 level sets (128 waves).  Median wall time of one
+    dcl_reinitialize call; inputs (phi, records) resident on the device."""
+    import torch
+    import dcell_amd as D
+    from dcell_amd import metrics as M
+
+    def timed(cells_in):
+        saved = [c[1].clone() for c in cells_in]
+        ts, bands = [], None
+        for _ in range(runs + 1):
+            for c, s in zip(cells_in, saved):
+                c[1].copy_(s)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            bands = M.reinitialize([(c[1], c[0], c[2], c[3]) for c in cells_in])
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts[1:])), sum(int(b.shape[0]) for b in bands)
+
+    single = [_reinit_inputs(n, 1.5, (0.35, 0.35, 0.35), 0.15, torch, D, M)]
+    t1, nb1 = timed(single)
+    batch = [_reinit_inputs(e, 1.5, c, 0.25, torch, D, M) for e, c in reinit_cells()]
+    t2, nb2 = timed(batch)
+    return dict(single=dict(n=n, band_nodes=nb1, ms_per_call=t1 * 1e3, band_nodes_per_s=nb1 / t1),
+                batch=dict(cells=len(batch), band_nodes=nb2, ms_per_call=t2 * 1e3, band_nodes_per_s=nb2 / t2))
+
+
 def copy_ceiling():
     """The measured HBM ceiling: a device-to-device copy of 1 GiB (torch),
     best of 5, read + write bytes over its time."""
@@ -575,6 +656,7 @@ FORMS = {
     "zalesak_2d_128_fp64_reference_scheme": lambda: zalesak_run(128, 628, "linear", "euler"),
     "irregular_nodes_256_fp64": lambda: irregular_run(256, 20),
     "get_velocity_256_fp64": lambda: get_velocity_run(256, 20),
+    "reinitialize_fp64": lambda: reinit_run(256, 5),
 }
 
 

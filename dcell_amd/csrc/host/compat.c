@@ -339,9 +339,11 @@ PetscErrorCode LevelSetDestroy(LevelSet ls) {
 }
 
 /* Steps outside the level-set update: no-ops here, the rest of DCell
- * provides them (FastMarching*.c, ParticleLS.c:202-295).  The irregular-node
- * list is the drop-in's (libdcell_levelset.so). */
-__attribute__((weak)) PetscErrorCode LevelSetReinitialize(LevelSet ls) {
+ * provides them (LevelSetMethod/Resize.c:5-120, ParticleLS.c:202-295).  The
+ * irregular-node list and the reinitialisation are the drop-in's
+ * (libdcell_levelset.so); standalone, the window is taken as large enough
+ * (DCell's LevelSetResize grows it when the band nears its edge). */
+__attribute__((weak)) PetscErrorCode LevelSetResize(LevelSet ls) {
   (void)ls;
   return 0;
 }

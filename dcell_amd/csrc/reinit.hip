@@ -22,20 +22,47 @@
 // the - pass also lists the boundary nodes in record order.  The band is
 // assembled as [boundary nodes | - pass | + pass] by copies.
 //
-// Departures from the reference, both at the window's edge or worse: a
-// neighbour outside the window is not pushed (the reference reads it
-// unchecked; LevelSetResize keeps the band away from the edge), and a NaN
-// update or a full heap stops the pass with an error instead of exiting.
+// Departures from the reference, where it reads out of bounds or leaves a
+// value unset: a neighbour outside the window is not pushed and a record
+// outside it is skipped (the reference reads them unchecked; LevelSetResize
+// keeps the band away from the edge), a pushed 2D node's z is the window's
+// (the reference leaves it unset), and a NaN update or a full heap stops the
+// pass with an error instead of exiting.
+//
+// The heap's first 4096 slots (12 levels) sit in LDS, where nearly every pop
+// and push spends most of its steps; deeper levels are in memory.
 #include <math.h>
 
 #include "dcl_host.hpp"
 
 namespace dcl {
 
+// a heap entry: the tentative value and the node's window-local
+// coordinates packed 21 bits each (windows are below 2^21 per axis)
 struct RiEntry {
   double v;
-  int x, y, z, pad;
+  unsigned long long key;
 };
+
+__device__ __forceinline__ unsigned long long ri_pack(int lx, int ly, int lz) {
+  return (unsigned long long)lx | ((unsigned long long)ly << 21) | ((unsigned long long)lz << 42);
+}
+
+// the heap's first RI_LDS slots (12 levels) live in LDS, the rest in memory
+constexpr int RI_LDS = 4096;
+
+struct RiHeap {
+  RiEntry* lds;
+  RiEntry* mem;
+};
+
+__device__ __forceinline__ RiEntry ri_get(const RiHeap& h, int64_t i) { return i < RI_LDS ? h.lds[i] : h.mem[i]; }
+__device__ __forceinline__ void ri_set(const RiHeap& h, int64_t i, const RiEntry& e) {
+  if (i < RI_LDS)
+    h.lds[i] = e;
+  else
+    h.mem[i] = e;
+}
 
 struct RiCell {
   double* phi;
@@ -82,11 +109,14 @@ __global__ __launch_bounds__(256) void reinit_bc_k(RiCell c) {
   c.phi[ri_index(c.w, c.d3, n.pos.x, n.pos.y, n.pos.z)] = n.sign_center * r;
 }
 
-// phi as this pass sees it; the other pass may write (its own sign's) nodes
-// concurrently and lane 0 of this wave writes accepted nodes, so reads go to
-// the device-coherent level
+// phi as this pass sees it.  Lane 0 of the wave writes the accepted nodes
+// and the other lanes read them back after a workgroup-scope fence (one
+// wave, one CU: its L1 holds the writes).  The other pass may write nodes of
+// the opposite sign at any time; whether a read sees the old -+PHI_INF or
+// the new value, it is of the opposite sign, which this pass treats the same
+// either way, so no wider coherence is needed.
 __device__ __forceinline__ double ri_load(const double* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // AccumulateCoef (FastMarching2D.c:33-94): one axis of max(D+, D-, 0)^2
@@ -145,48 +175,48 @@ __device__ __forceinline__ void ri_accumulate(double bw, const double* phi7, dou
 
 // HeapInsert (Common/Heap.c:59-91), lane 0 only; the item is stored once at
 // its final place (the reference moves it level by level; same final tree)
-__device__ void ri_heap_insert(RiEntry* __restrict__ H, int64_t& len, const RiEntry& item) {
+__device__ __forceinline__ void ri_heap_insert(const RiHeap& H, int64_t& len, const RiEntry& item) {
   int64_t c = len;
   len = c + 1;
   while (c != 1) {
     const int64_t p = c >> 1;
-    const RiEntry parent = H[p];
+    const RiEntry parent = ri_get(H, p);
     if (parent.v - item.v < 0) break;
-    H[c] = parent;
+    ri_set(H, c, parent);
     c = p;
   }
-  H[c] = item;
+  ri_set(H, c, item);
 }
 
 // HeapPop (Common/Heap.c:94-139), lane 0 only
-__device__ RiEntry ri_heap_pop(RiEntry* __restrict__ H, int64_t& hlen) {
-  const RiEntry top = H[1];
+__device__ __forceinline__ RiEntry ri_heap_pop(const RiHeap& H, int64_t& hlen) {
+  const RiEntry top = ri_get(H, 1);
   int64_t len = hlen;
   if (len == 2) {
     hlen = 1;
     return top;
   }
   len--;
-  const RiEntry bubble = H[len];
+  const RiEntry bubble = ri_get(H, len);
   hlen = len;
   int64_t p = 1, l = 2;
   while (l < len) {
     const int64_t r = l + 1;
     int64_t s = l;
-    RiEntry child = H[l];
+    RiEntry child = ri_get(H, l);
     if (r < len) {
-      const RiEntry right = H[r];
+      const RiEntry right = ri_get(H, r);
       if (right.v - child.v < 0) {
         s = r;
         child = right;
       }
     }
     if (bubble.v - child.v <= 0) break;
-    H[p] = child;
+    ri_set(H, p, child);
     p = s;
     l = 2 * p;
   }
-  H[p] = bubble;
+  ri_set(H, p, bubble);
   return top;
 }
 
@@ -197,7 +227,8 @@ __device__ __forceinline__ int ri_star(int n, int a) { return (n >> 1) == a ? ((
 // it; lanes below NN compute one neighbour each, lane 0 inserts them in
 // STAR order.  Returns an error code (uniform).
 template <int NN>
-__device__ int ri_push(const RiCell& c, int sign, int X, int Y, int Z, RiEntry* __restrict__ H, int64_t& hlen) {
+__device__ __forceinline__ int ri_push(const RiCell& c, int sign, int X, int Y, int Z, const RiHeap& H,
+                                      int64_t& hlen) {
 #pragma clang fp contract(off)
   const int lane = threadIdx.x;
   const int i = X + ri_star(lane, 0), j = Y + ri_star(lane, 1), k = c.d3 ? Z + ri_star(lane, 2) : Z;
@@ -230,10 +261,7 @@ __device__ int ri_push(const RiCell& c, int sign, int X, int Y, int Z, RiEntry* 
     if (!((vm >> n) & 1ull)) continue;
     RiEntry e;
     e.v = __shfl(d1, n);
-    e.x = __shfl(i, n);
-    e.y = __shfl(j, n);
-    e.z = __shfl(k, n);
-    e.pad = 0;
+    e.key = ri_pack(__shfl(i, n) - c.w.px, __shfl(j, n) - c.w.py, c.d3 ? __shfl(k, n) - c.w.pz : 0);
     if (hlen >= c.heap_cap) return RI_HEAP;
     if (lane == 0) ri_heap_insert(H, hlen, e);
     else hlen++;
@@ -250,7 +278,8 @@ __global__ __launch_bounds__(64) void reinit_fmm_k(const RiCell* __restrict__ ce
   const int pass = blockIdx.x & 1;
   const int sign = pass ? 1 : -1;
   const int lane = threadIdx.x;
-  RiEntry* H = pass ? c.heap[1] : c.heap[0];
+  __shared__ RiEntry lds_heap[RI_LDS];
+  const RiHeap H = {lds_heap, pass ? c.heap[1] : c.heap[0]};
   dcl_icoor* accepted = pass ? c.list[2] : c.list[1];
   int64_t hlen = 1, nbc = 0, nacc = 0;
   int err = RI_OK;
@@ -298,7 +327,9 @@ __global__ __launch_bounds__(64) void reinit_fmm_k(const RiCell* __restrict__ ce
     if (lane == 0) top = ri_heap_pop(H, hlen);
     else hlen--;
     const double v = __shfl(top.v, 0);
-    const int x = __shfl(top.x, 0), y = __shfl(top.y, 0), z = __shfl(top.z, 0);
+    const unsigned long long key = __shfl(top.key, 0);
+    const int x = c.w.px + (int)(key & 0x1fffffull), y = c.w.py + (int)((key >> 21) & 0x1fffffull);
+    const int z = c.d3 ? c.w.pz + (int)(key >> 42) : c.w.pz;  // 2D: the window's z (see orc_reinitialize)
     if (v > c.bw) break;
     double* q = c.phi + ri_index(c.w, c.d3, x, y, z);
     double cur = 0.0;
@@ -311,10 +342,10 @@ __global__ __launch_bounds__(64) void reinit_fmm_k(const RiCell* __restrict__ ce
         p.x = x, p.y = y, p.z = z;
         accepted[nacc] = p;
       }
-      __hip_atomic_store(q, sign * v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(q, sign * v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
     ++nacc;
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
     err = ri_push<NN>(c, sign, x, y, z, H, hlen);
   }

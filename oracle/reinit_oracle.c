@@ -13,11 +13,12 @@
  * Every step keeps the reference's order: the heap's sift-up and sift-down
  * with its comparisons (so equal keys pop in the same order), the neighbour
  * order of STAR, the axis order x, y, z, and each expression's operand order
- * (built with -ffp-contract=off).  Two departures from the reference, both
- * at the window's edge: a neighbour outside the window is not pushed (the
- * reference reads it unchecked; LevelSetResize keeps the band away from the
- * edge, LevelSetMethod/Resize.c:13-14), and a NaN update is reported
- * (return -1) instead of exiting the process.
+ * (built with -ffp-contract=off).  Departures from the reference, all where
+ * it reads out of bounds or leaves a value unset: a neighbour outside the
+ * window is not pushed and a record outside it is skipped (the reference
+ * reads them unchecked; LevelSetResize keeps the band away from the edge,
+ * LevelSetMethod/Resize.c:13-14), a pushed 2D node's z is the window's, and
+ * a NaN update is reported (return -1) instead of exiting the process.
  *
  * The sign passes: the - pass only writes nodes whose phi is -PHI_INF and
  * only reads non-negative values of -phi, and the + pass the mirror image,
@@ -170,7 +171,9 @@ static const int STAR[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0,
 static int push_neighbors(const fmm *f, heap *h, int sign, int X, int Y, int Z) {
   const int nn = f->d3 ? 6 : 4;
   for (int n = 0; n < nn; ++n) {
-    const int i = X + STAR[n][0], j = Y + STAR[n][1], k = f->d3 ? Z + STAR[n][2] : Z;
+    /* in 2D the reference leaves a pushed node's pos.z unset
+     * (FastMarching2D.c:177-180); here it is the window's z */
+    const int i = X + STAR[n][0], j = Y + STAR[n][1], k = f->d3 ? Z + STAR[n][2] : f->w->pz;
     if (!inside(f, i, j, k)) continue;
     if (sign * *at(f, i, j, k) < f->band_width) continue;
     double a = 0, b = 0, c = -1, loc[7];
@@ -210,7 +213,7 @@ int64_t orc_reinitialize(double *phi, const orc_win *w, const orc_irregular_node
   /* the boundary condition: the distance to the orthogonal projection */
   for (int64_t i = 0; i < count; ++i) {
     const orc_irregular_node *n = &nodes[i];
-    if (n->axis != -1) continue;
+    if (n->axis != -1 || !inside(&f, n->pos.x, n->pos.y, n->pos.z)) continue;
     APPEND(n->pos.x, n->pos.y, n->pos.z);
     const double r = f.d3 ? sqrt(n->op[0] * n->op[0] + n->op[1] * n->op[1] + n->op[2] * n->op[2])
                           : sqrt(n->op[0] * n->op[0] + n->op[1] * n->op[1]);
@@ -221,7 +224,7 @@ int64_t orc_reinitialize(double *phi, const orc_win *w, const orc_irregular_node
   for (int sign = -1; sign <= 1 && rc == 0; sign += 2) {
     for (int64_t i = 0; i < count && rc == 0; ++i) {
       const orc_irregular_node *n = &nodes[i];
-      if (n->axis != -1) continue;
+      if (n->axis != -1 || !inside(&f, n->pos.x, n->pos.y, n->pos.z)) continue;
       if (sign * *at(&f, n->pos.x, n->pos.y, n->pos.z) >= 0.)
         rc = push_neighbors(&f, &h, sign, n->pos.x, n->pos.y, n->pos.z);
     }

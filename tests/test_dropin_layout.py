@@ -69,7 +69,7 @@ def test_dropin_libraries_export_reference_names():
                  "ParticleLS_AdvectParticles2D", "ParticleLS_AdvectParticles3D",
                  "ParticleLS_ErrorCorrection2D", "ParticleLS_ErrorCorrection3D", "ParticleLS_AdjustRadii",
                  "LevelSetCFLIncrement", "LevelSet_MaxVelocity", "LevelSetCurvature",
-                 "LevelSetUpdateIrregularNodeList", "LevelSetUpdateIrregularNodeList_2D",
+                 "LevelSetUpdateIrregularNodeList", "LevelSetUpdateIrregularNodeList_2D", "LevelSetReinitialize",
                  "LevelSetUpdateIrregularNodeList_3D", "IIMUpdateSurfaceQuantities_2D",
                  "IIMUpdateSurfaceQuantities_3D", "LevelSetAdvectSLSetScheme", "LevelSetPLSSetReferenceCornerZ", "LevelSetGPUFinalize"):
         assert f" T {name}\n" in syms, name
