@@ -463,8 +463,9 @@ def adjoint_run(n=128, steps=50, every=10):
 def zalesak_run(n=128, steps=628, interp="cubic", bt="rk2"):
     """C1: the Zalesak slotted disk (128^2 FP64) under rigid rotation for one
     revolution (628 steps, dt = 1/628), SL + 9-point curvature every step.
-    Reports updates/s and, after the revolution, the area of {phi < 0} and
-    the max / mean |phi - phi_initial| within 3 cells of the interface."""
+    Reports updates/s launched step by step and as one replayed HIP graph,
+    and, after the revolution, the area of {phi < 0} and the max / mean
+    |phi - phi_initial| within 3 cells of the interface."""
     import torch
     import dcell_amd as D
     from dcell_amd import configs
@@ -488,11 +489,36 @@ def zalesak_run(n=128, steps=628, interp="cubic", bt="rk2"):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     final = bufs[steps % 2].cpu().numpy()
+    # the same revolution as one HIP graph (1256 kernel nodes), replayed: the
+    # two 16k-node kernels of a step are launch-bound one by one
+    start = torch.from_numpy(phi_np).to(dev, torch.float64)
+    g = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        D.advect_sl(bufs[0], vel, win, 1.0 / steps, interp, bt, out=bufs[1])
+        D.curvature(bufs[1], win, out=kappa)
+    torch.cuda.current_stream().wait_stream(side)
+    with torch.cuda.graph(g):
+        for s in range(steps):
+            D.advect_sl(bufs[s % 2], vel, win, 1.0 / steps, interp, bt, out=bufs[(s + 1) % 2])
+            D.curvature(bufs[(s + 1) % 2], win, out=kappa)
+    bufs[0].copy_(start)
+    g.replay()  # warm-up replay
+    bufs[0].copy_(start)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g.replay()
+    torch.cuda.synchronize()
+    gwall = time.perf_counter() - t0
+    gfinal = bufs[steps % 2].cpu().numpy()
     band = np.abs(phi_np) < 3
     err = np.abs(final - phi_np)[band]
     interior = (n - 4) ** 2
     return dict(n=n, steps=steps, interp=interp, backtrace=bt, updates_per_s=interior * steps / wall,
-                ms_per_step=wall / steps * 1e3, area_initial=float((phi_np < 0).sum() * h * h),
+                ms_per_step=wall / steps * 1e3, graph_updates_per_s=interior * steps / gwall,
+                graph_ms_per_step=gwall / steps * 1e3, graph_equals_eager=bool(np.array_equal(gfinal, final)),
+                area_initial=float((phi_np < 0).sum() * h * h),
                 area_final=float((final < 0).sum() * h * h), max_abs_err_band=float(err.max()),
                 mean_abs_err_band=float(err.mean()))
 
