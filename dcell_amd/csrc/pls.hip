@@ -15,6 +15,7 @@
 // values, so no full-grid pass is needed per call.
 #include <new>
 
+#include "dcl_buf.hpp"
 #include "dcl_host.hpp"
 
 namespace dcl {
@@ -32,21 +33,32 @@ __device__ __forceinline__ double dist2(double dx, double dy, double dz, bool th
 }
 
 // --------------------------------------------------------------- advect
-template <typename T, int DIM>
+// BUF (3D, velocity window below 4 GiB): the MAC interpolations read with
+// buffer loads (32-bit offsets; with 3 FP32 components an x pair of faces is
+// one 16-byte load), the same arithmetic as the pointer path
+template <typename T, int DIM, bool BUF>
 __global__ __launch_bounds__(256) void pls_advect_k(T* __restrict__ P, int64_t n,
                                                     const T* __restrict__ vel, KVel v,
-                                                    double dhx, double dhy, double dhz, double dt) {
+                                                    double dhx, double dhy, double dhz, double dt,
+                                                    unsigned vbytes) {
   const int64_t m = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (m >= n) return;
   T* p = P + 4 * m;
   const double dh[3] = {dhx, dhy, dhz};
   double X[3] = {(double)p[0], (double)p[1], (double)p[2]};
-  T V1[DIM], V2[DIM];
-  mac_vel<T, DIM>(vel, v, X, V1);
+  T V1[3], V2[3];
+  const rsrc_t rv = make_rsrc(vel, vbytes);
+  if (BUF && DIM == 3)
+    mac_vel_buf<T>(rv, v, X, V1);
+  else
+    mac_vel<T, DIM>(vel, v, X, V1);
   double Xm[3] = {X[0], X[1], X[2]};
 #pragma unroll
   for (int a = 0; a < DIM; ++a) Xm[a] = X[a] + dt / 2 * (double)V1[a] / dh[a];
-  mac_vel<T, DIM>(vel, v, Xm, V2);
+  if (BUF && DIM == 3)
+    mac_vel_buf<T>(rv, v, Xm, V2);
+  else
+    mac_vel<T, DIM>(vel, v, Xm, V2);
 #pragma unroll
   for (int a = 0; a < DIM; ++a) p[a] = (T)(X[a] + dt * (double)V2[a] / dh[a]);
 }
@@ -229,9 +241,15 @@ extern "C" int dcl_pls_advect(void* particles, int64_t count, const dcl_velocity
   v.udof = vel->udof;
   hipStream_t s = as_stream(stream);
   const unsigned g = grid1d(count, 256);
-#define DCL_ADV(T, D)                                                                          \
-  hipLaunchKernelGGL((pls_advect_k<T, D>), dim3(g), dim3(256), 0, s, static_cast<T*>(particles), \
-                     count, static_cast<const T*>(vel->data), v, dh.x, dh.y, dh.z, dt)
+  const long long vb = win_nodes(vel->win) * vel->dof * (long long)(precision == DCL_F64 ? 8 : 4);
+  const bool buf = dim == 3 && buf_ok(0, vb);
+#define DCL_ADV(T, D)                                                                                     \
+  if (buf)                                                                                              \
+    hipLaunchKernelGGL((pls_advect_k<T, D, true>), dim3(g), dim3(256), 0, s, static_cast<T*>(particles),  \
+                       count, static_cast<const T*>(vel->data), v, dh.x, dh.y, dh.z, dt, (unsigned)vb);   \
+  else                                                                                                  \
+    hipLaunchKernelGGL((pls_advect_k<T, D, false>), dim3(g), dim3(256), 0, s, static_cast<T*>(particles), \
+                       count, static_cast<const T*>(vel->data), v, dh.x, dh.y, dh.z, dt, 0u)
   if (precision == DCL_F64) {
     if (dim == 3) DCL_ADV(double, 3); else DCL_ADV(double, 2);
   } else {
