@@ -244,6 +244,14 @@ def cpu_baseline(args):
     t0 = time.perf_counter()
     recs = O.irregular_nodes(phi, w, band)
     t_irr = time.perf_counter() - t0
+    # the drop-in form's band step (Euler + trilinear over the 6-cell band,
+    # the drop-in's default scheme) by the oracle on one thread
+    vel_full = configs.deformation_velocity(n)
+    dt0 = configs.deformation_dt(n, 0)
+    O.sl_advect(phi, vel_full, w, dt0, O.LINEAR, O.EULER, band=band[:1000])
+    t0 = time.perf_counter()
+    O.sl_advect(phi, vel_full, w, dt0, O.LINEAR, O.EULER, band=band)
+    t_band = time.perf_counter() - t0
     # the reinitialisation form's inputs, by the oracle: the 256^3 sphere on
     # one thread, and the 64-cell batch one cell per thread
     import concurrent.futures as cf
@@ -284,6 +292,9 @@ def cpu_baseline(args):
                 all_cores=dict(value=out[nthreads], cores=nthreads), cpu=model,
                 irregular_nodes=dict(band_nodes_per_s=len(band) / t_irr, records=len(recs), cores=1,
                                      sample=f"the full {len(band)}-node band of the {n}^3 sphere, one call"),
+                band_step=dict(ms=t_band * 1e3, band_nodes=len(band), cores=1,
+                               sample=f"one Euler + trilinear SL step over the {len(band)}-node 6-cell band of "
+                                      f"the {n}^3 sphere"),
                 reinitialize=dict(single_ms=t_re * 1e3, single_band_nodes=len(bandr), cores=1,
                                   batch_ms=t_b1 * 1e3, batch_band_nodes=nbatch,
                                   batch_all_cores_ms=t_bn * 1e3, all_cores=nthreads,
@@ -640,6 +651,61 @@ def reinit_run(n=256, runs=5):
                 batch=dict(cells=len(batch), band_nodes=nb2, ms_per_call=t2 * 1e3, band_nodes_per_s=nb2 / t2))
 
 
+def dropin_run(n=256, calls=10):
+    """The C drop-in as DCell calls it: LevelSetAdvectSL on an n^3 FP64 level
+    set (the headline sphere) over its 6-cell band, with a full n^3 velocity
+    grid in host memory.  Each call uploads phi0 and the velocity window,
+    advects the band nodes on the GPU and writes them back into the host grid,
+    so this is the PCIe-inclusive rate; the same band update through the C ABI
+    with everything resident in HBM is timed beside it."""
+    import ctypes as C
+    import torch
+    import dcell_amd as D
+    from dcell_amd import configs
+    from dcell_amd import dropin as DI
+    from dcell_amd import metrics as M
+
+    L = DI.lib()
+    h = 1.0 / (n - 1)
+    ls = DI.create_levelset((h, h, h), (0, 0, 0), (n, n, n))
+    phi = DI.grid_array(ls.contents.phi)
+    phi[:] = configs.sphere_sdf(n)
+    win = D.GridWindow.unit_box(n)
+    dphi = torch.from_numpy(phi.copy()).to("cuda:0")
+    band, nb = M.band_compact(dphi, win, 6.0)
+    band_np = band.cpu().numpy()
+    DI.set_array(ls.contents.band, band_np, 12)
+    vg = DI.create_grid((h, h, h), (0, 0, 0), (n, n, n), 3)
+    vel = DI.grid_array(vg)
+    vel[:] = configs.deformation_velocity(n)
+    dt = configs.deformation_dt(n, 0)
+    DI.check(L.LevelSetAdvectSL(ls, vg, C.c_double(dt)), "LevelSetAdvectSL")  # warm-up
+    ts = []
+    for _ in range(calls):
+        t0 = time.perf_counter()
+        DI.check(L.LevelSetAdvectSL(ls, vg, C.c_double(dt)), "LevelSetAdvectSL")
+        ts.append(time.perf_counter() - t0)
+    t_drop = float(np.median(ts))
+    # device-resident: the same band update through the C ABI
+    dvel = torch.from_numpy(vel.copy()).to("cuda:0")
+    out = dphi.clone()
+    D.advect_sl(dphi, dvel, win, dt, band=band, out=out)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(calls):
+        D.advect_sl(dphi, dvel, win, dt, band=band, out=out)
+    e1.record()
+    torch.cuda.synchronize()
+    t_dev = e0.elapsed_time(e1) * 1e-3 / calls
+    L.GridDestroy(vg)
+    L.LevelSetDestroy(ls)
+    moved = (n ** 3) * 8 * 4  # phi0 and the 3-component velocity uploaded per call
+    return dict(n=n, band_nodes=nb, dropin_ms_per_call=t_drop * 1e3, dropin_band_updates_per_s=nb / t_drop,
+                host_bytes_uploaded_per_call=moved, device_resident_ms_per_call=t_dev * 1e3,
+                device_resident_band_updates_per_s=nb / t_dev)
+
+
 def copy_ceiling():
     """The measured HBM ceiling: a device-to-device copy of 1 GiB (torch),
     best of 5, read + write bytes over its time."""
@@ -683,6 +749,7 @@ FORMS = {
     "irregular_nodes_256_fp64": lambda: irregular_run(256, 20),
     "get_velocity_256_fp64": lambda: get_velocity_run(256, 20),
     "reinitialize_fp64": lambda: reinit_run(256, 5),
+    "dropin_band_256_fp64": lambda: dropin_run(256, 10),
 }
 
 
