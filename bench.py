@@ -631,24 +631,51 @@ def reinit_run(n=256, runs=5):
     import dcell_amd as D
     from dcell_amd import metrics as M
 
-    def timed(cells_in):
+    def timed(cells_in, parallel=False):
         saved = [c[1].clone() for c in cells_in]
-        ts, bands = [], None
+        ts, bands, sweeps = [], None, None
         for _ in range(runs + 1):
             for c, s in zip(cells_in, saved):
                 c[1].copy_(s)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            bands = M.reinitialize([(c[1], c[0], c[2], c[3]) for c in cells_in])
+            args = [(c[1], c[0], c[2], c[3]) for c in cells_in]
+            if parallel:
+                bands, sweeps = M.reinitialize_parallel(args)
+            else:
+                bands = M.reinitialize(args)
             ts.append(time.perf_counter() - t0)
-        return float(np.median(ts[1:])), sum(int(b.shape[0]) for b in bands)
+        phis = [c[1].clone() for c in cells_in]
+        for c, s in zip(cells_in, saved):
+            c[1].copy_(s)
+        return float(np.median(ts[1:])), sum(int(b.shape[0]) for b in bands), phis, sweeps
+
+    def agree(exact, par):
+        """The parallel result against the exact one over the band nodes of both."""
+        nb = eq = nx = 0
+        dmax = 0.0
+        for a, b in zip(exact, par):
+            ma, mb = a.abs() < 7.5, b.abs() < 7.5
+            m = ma & mb
+            nb += int(m.sum())
+            eq += int((a[m] == b[m]).sum())
+            nx += int((ma ^ mb).sum())
+            if m.any():
+                dmax = max(dmax, float((a[m] - b[m]).abs().max()))
+        return dict(bit_equal_frac=eq / max(nb, 1), max_abs_diff=dmax, band_nodes_in_one_only=nx)
 
     single = [_reinit_inputs(n, 1.5, (0.35, 0.35, 0.35), 0.15, torch, D, M)]
-    t1, nb1 = timed(single)
+    t1, nb1, e1, _ = timed(single)
+    p1, pb1, q1, sw1 = timed(single, True)
     batch = [_reinit_inputs(e, 1.5, c, 0.25, torch, D, M) for e, c in reinit_cells()]
-    t2, nb2 = timed(batch)
+    t2, nb2, e2, _ = timed(batch)
+    p2, pb2, q2, sw2 = timed(batch, True)
     return dict(single=dict(n=n, band_nodes=nb1, ms_per_call=t1 * 1e3, band_nodes_per_s=nb1 / t1),
-                batch=dict(cells=len(batch), band_nodes=nb2, ms_per_call=t2 * 1e3, band_nodes_per_s=nb2 / t2))
+                batch=dict(cells=len(batch), band_nodes=nb2, ms_per_call=t2 * 1e3, band_nodes_per_s=nb2 / t2),
+                parallel_single=dict(n=n, band_nodes=pb1, ms_per_call=p1 * 1e3, band_nodes_per_s=pb1 / p1,
+                                     sweeps=sw1[0], vs_exact=agree(e1, q1)),
+                parallel_batch=dict(cells=len(batch), band_nodes=pb2, ms_per_call=p2 * 1e3,
+                                    band_nodes_per_s=pb2 / p2, sweeps_max=max(sw2), vs_exact=agree(e2, q2)))
 
 
 def dropin_run(n=256, calls=10):

@@ -25,7 +25,7 @@ EXPORTS = (
     "dcl_sl_advect", "dcl_curvature", "dcl_levelset_step",
     "dcl_pls_advect", "dcl_pls_ws_create", "dcl_pls_ws_destroy", "dcl_pls_correct",
     "dcl_pls_adjust_radii", "dcl_max_velocity", "dcl_band_compact", "dcl_irregular_nodes",
-    "dcl_surface_quantities", "dcl_get_velocity", "dcl_reinitialize",
+    "dcl_surface_quantities", "dcl_get_velocity", "dcl_reinitialize", "dcl_reinitialize_parallel",
     "dcl_interface_metrics", "dcl_error_norms", "dcl_sl_adjoint",
     "dcl_batch_create", "dcl_batch_destroy", "dcl_batch_layout", "dcl_batch_set_dt",
     "dcl_batch_set_config", "dcl_batch_get_config", "dcl_batch_step", "dcl_batch_autotune",
@@ -94,6 +94,7 @@ def _declare(L):
         "dcl_surface_quantities": ([vp, W, vp, i64, i32, vp, vp], i32),
         "dcl_get_velocity": ([vp, C.POINTER(ICoor), W, i32, vp, vp], i32),
         "dcl_reinitialize": ([C.POINTER(ReinitCell), i32, i64, vp], i32),
+        "dcl_reinitialize_parallel": ([C.POINTER(ReinitCell), i32, i32, C.POINTER(C.c_int), vp], i32),
         "dcl_interface_metrics": ([vp, vp, W, dbl, i32, C.POINTER(dbl), vp], i32),
         "dcl_error_norms": ([vp, i32, vp, i32, i64, C.POINTER(dbl), vp], i32),
         "dcl_sl_adjoint": ([vp, W, V, vp, vp, vp, dbl, i32, i32, i32, vp], i32),

@@ -130,6 +130,12 @@ def reinitialize(cells, band_width: float = 6.0, phi_inf: float = 7.5, heap_capa
     records a uint8 device tensor of irregular-node records (as
     irregular_nodes_into writes them).  Returns, per level set, the new band
     as an (n, 3) int32 device tensor in the reference's order."""
+    arr, bands = _reinit_cells(cells, band_width, phi_inf)
+    L.call("dcl_reinitialize", arr, len(cells), int(heap_capacity), stream_handle())
+    return [b[:arr[i].band_count] for i, b in enumerate(bands)]
+
+
+def _reinit_cells(cells, band_width, phi_inf):
     arr = (L.ReinitCell * len(cells))()
     bands = []
     for i, (phi, win, records, count) in enumerate(cells):
@@ -147,5 +153,15 @@ def reinitialize(cells, band_width: float = 6.0, phi_inf: float = 7.5, heap_capa
         arr[i].phi_inf = float(phi_inf)
         arr[i].band = band.data_ptr()
         arr[i].band_capacity = win.nodes
-    L.call("dcl_reinitialize", arr, len(cells), int(heap_capacity), stream_handle())
-    return [b[:arr[i].band_count] for i, b in enumerate(bands)]
+    return arr, bands
+
+
+def reinitialize_parallel(cells, band_width: float = 6.0, phi_inf: float = 7.5, max_sweeps: int = 0):
+    """The parallel reinitialisation of a batch (dcl_reinitialize_parallel):
+    same arguments as reinitialize.  Returns (bands, sweeps): per level set
+    the new band ([projection nodes in record order | - side | + side], each
+    side in window order) and the sweeps it took."""
+    arr, bands = _reinit_cells(cells, band_width, phi_inf)
+    sweeps = (C.c_int * len(cells))()
+    L.call("dcl_reinitialize_parallel", arr, len(cells), int(max_sweeps), sweeps, stream_handle())
+    return [b[:arr[i].band_count] for i, b in enumerate(bands)], list(sweeps)

@@ -251,6 +251,26 @@ typedef struct {
  * one wave, so a batch runs its level sets side by side. */
 int dcl_reinitialize(dcl_reinit_cell *cells, int ncells, int64_t heap_capacity, void *stream);
 
+/* The same reinitialisation computed in parallel sweeps, for level sets too
+ * large for the fast marching's one-node-at-a-time order (it replaces
+ * LevelSetReinitialize, LevelSetMethod/FastMarching.c:10-146, where a
+ * result within the fast marching's own discretisation error will do).
+ * Same boundary values and the same second-order update
+ * (FastMarching2D.c:33-94); each sweep gives every node the value the fast
+ * marching would give it if its neighbours were accepted in the order of
+ * their values (its pushes replayed in that order), from the previous
+ * sweep's values, until no node moves by more than band_width * 1e-9.
+ * Where the fast marching's acceptance order is its value order, phi comes
+ * out bit for bit as dcl_reinitialize's; elsewhere it differs by far less
+ * than either's distance error.  band = [projection nodes in record order |
+ * - side | + side], each side in window order (z, y, x).  max_sweeps: the
+ * sweeps allowed per level set (0: 256; not settling is an error); sweeps
+ * (host, ncells entries, may be NULL) gets the sweeps each level set took.
+ * Synchronous; band_count is set on return.  Uses device scratch of 9 bytes
+ * per window node. */
+int dcl_reinitialize_parallel(dcl_reinit_cell *cells, int ncells, int max_sweeps, int *sweeps,
+                              void *stream);
+
 /* LevelSetGetVelocity (LevelSetMethod/Advect.c:145-192) from a fluid field
  * resident on the device.  fluid: the fluid's [z][y][x][4] array (p, u, v,
  * w) over its domain of fluid_n nodes, node (0,0,0) at global (0,0,0), 3D

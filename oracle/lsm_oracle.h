@@ -151,6 +151,15 @@ void orc_surface_quantities(const double *phi, const orc_win *w, const orc_irreg
 int64_t orc_reinitialize(double *phi, const orc_win *w, const orc_irregular_node *nodes, int64_t count,
                          double band_width, double phi_inf, orc_icoor *band, int64_t band_cap);
 
+/* The parallel reinitialisation's sweeps (reinit_oracle.c): same inputs;
+ * the band is [boundary nodes in record order | - pass nodes | + pass nodes],
+ * each pass in window order.  *sweeps: the sweeps run, the last one changing
+ * nothing.  Returns the band length, -1 on a NaN update, -3 when max_sweeps
+ * sweeps did not settle. */
+int64_t orc_reinitialize_sweep(double *phi, const orc_win *w, const orc_irregular_node *nodes, int64_t count,
+                               double band_width, double phi_inf, orc_icoor *band, int64_t band_cap,
+                               int max_sweeps, int *sweeps);
+
 #ifdef __cplusplus
 }
 #endif

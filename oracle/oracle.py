@@ -97,6 +97,9 @@ def lib():
         L.orc_surface_quantities.argtypes = [vp, W, vp, C.c_int64, vp]
         L.orc_reinitialize.argtypes = [vp, W, vp, C.c_int64, C.c_double, C.c_double, vp, C.c_int64]
         L.orc_reinitialize.restype = C.c_int64
+        L.orc_reinitialize_sweep.argtypes = [vp, W, vp, C.c_int64, C.c_double, C.c_double, vp, C.c_int64,
+                                             C.c_int, C.POINTER(C.c_int)]
+        L.orc_reinitialize_sweep.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -395,3 +398,21 @@ def reinitialize(phi, win: Win, records, band_width=6.0, phi_inf=7.5):
     if n < 0:
         raise RuntimeError("reinitialisation: NaN update")
     return out, band[:n]
+
+
+def reinitialize_sweep(phi, win: Win, records, band_width=6.0, phi_inf=7.5, max_sweeps=1000):
+    """The parallel reinitialisation (dcl_reinitialize_parallel) restated:
+    returns (phi copy, band, sweeps); the band is [boundary nodes in record
+    order | - pass | + pass], each pass in window order."""
+    out = np.array(_f64(phi), copy=True)
+    recs = np.ascontiguousarray(records, dtype=IRREG_DTYPE)
+    cap = out.size
+    band = np.zeros((cap, 3), dtype=np.int32)
+    sw = C.c_int(0)
+    n = lib().orc_reinitialize_sweep(_ptr(out), C.byref(win.c()), _ptr(recs), len(recs), float(band_width),
+                                     float(phi_inf), _ptr(band), cap, int(max_sweeps), C.byref(sw))
+    if n == -3:
+        raise RuntimeError(f"reinitialisation sweeps did not settle in {max_sweeps}")
+    if n < 0:
+        raise RuntimeError("reinitialisation: NaN update")
+    return out, band[:n], sw.value

@@ -244,3 +244,190 @@ int64_t orc_reinitialize(double *phi, const orc_win *w, const orc_irregular_node
   free(h.t);
   return rc ? rc : nb;
 }
+
+/* ------------------------------------------------------------------------
+ * The parallel reinitialisation (dcl_reinitialize_parallel), restated: the
+ * checker of the GPU sweeps, itself checked against orc_reinitialize above.
+ *
+ * Each node of a sign pass gets, from its neighbours' current values, the
+ * value the fast marching above would give it if its neighbours were
+ * accepted in the order of their values: the pushes it receives are replayed
+ * in that order (the boundary nodes' pushes first, then one push per
+ * neighbour value), each with the neighbours accepted by then, and the node
+ * keeps the least update made before its own value comes up.  Sweeps repeat
+ * this at every node from the previous sweep's values (Jacobi) until a sweep
+ * changes nothing.  Where the fast marching accepts nodes in value order
+ * (no ties decided by the heap's layout, no update below an accepted
+ * neighbour) the result is its result bit for bit.
+ *
+ * mask: 0 a - pass node, 1 a + pass node, 2 a boundary node.
+ * ------------------------------------------------------------------------ */
+
+/* test hook, so the tests can show their pins catch a plausible mistake:
+ * 1 counts every usable neighbour as accepted before the node whatever its
+ * value; 2 replays pushes past the node's own value; 3 keeps the last push
+ * instead of the least; 0 in every other use */
+int orc_sweep_plant = 0;
+
+/* one neighbour as pass `sign` sees it: its value q (sign * phi) and the
+ * time it is accepted (-1: before the pass's pops; its value for a node of
+ * this pass; INFINITY: never) */
+typedef struct {
+  double q, t;
+  int pushes; /* accepted in this pass, so it pushes its neighbours */
+} sw_nb;
+
+static sw_nb sw_view(const fmm *f, const double *in, const uint8_t *mask, int sign, int x, int y, int z) {
+  sw_nb r = {f->phi_inf, INFINITY, 0};
+  if (!inside(f, x, y, z)) return r;
+  const int64_t i = at(f, x, y, z) - f->phi;
+  const double q = sign * in[i];
+  const int m = mask[i];
+  r.q = q;
+  if (m == 2) {
+    /* a boundary node: pushes in this pass when sign * phi >= 0 */
+    if (0 <= q && q < f->phi_inf) {
+      r.t = -1;
+      r.pushes = 1;
+    }
+  } else if (m == (sign > 0)) {
+    if (q < f->phi_inf) {
+      r.t = q;
+      r.pushes = 1;
+    }
+  } else if (sign > 0 && 0 <= q && q < f->phi_inf) {
+    r.t = -1; /* a - pass node, final when the + pass starts */
+  }
+  if (orc_sweep_plant == 1 && r.t < INFINITY) r.t = -1;
+  return r;
+}
+
+/* returns the node's new value, or NAN for a NaN update */
+#define SW_TOL 1e-9
+
+static double sw_node(const fmm *f, const double *in, const uint8_t *mask, int sign, int X, int Y, int Z) {
+  const int na = f->d3 ? 3 : 2;
+  sw_nb nb[3][4]; /* per axis: -2, -1, +1, +2 */
+  for (int a = 0; a < na; ++a)
+    for (int k = 0; k < 4; ++k) {
+      const int s = k < 2 ? k - 2 : k - 1;
+      nb[a][k] = sw_view(f, in, mask, sign, X + (a == 0 ? s : 0), Y + (a == 1 ? s : 0), Z + (a == 2 ? s : 0));
+    }
+  double cand = INFINITY, last = -INFINITY;
+  for (;;) {
+    /* the next push: the least acceptance time above `last` among the
+     * neighbours that push */
+    double T = INFINITY;
+    for (int a = 0; a < na; ++a)
+      for (int k = 1; k <= 2; ++k)
+        if (nb[a][k].pushes && nb[a][k].t > last && nb[a][k].t < T) T = nb[a][k].t;
+    if (T == INFINITY || (!(T < cand) && orc_sweep_plant != 2)) break;
+    double A = 0, B = 0, Cc = -1, loc[7];
+    for (int a = 0; a < na; ++a) {
+      loc[0] = loc[6] = loc[3] = f->phi_inf;
+      loc[1] = nb[a][0].t <= T ? nb[a][0].q : f->phi_inf;
+      loc[2] = nb[a][1].t <= T ? nb[a][1].q : f->phi_inf;
+      loc[4] = nb[a][2].t <= T ? nb[a][2].q : f->phi_inf;
+      loc[5] = nb[a][3].t <= T ? nb[a][3].q : f->phi_inf;
+      accumulate(f->phi_inf, loc, &A, &B, &Cc);
+    }
+    const double disc = sqrt(B * B - 4 * A * Cc);
+    double d1 = (-B + disc) / (2 * A);
+    if (d1 != d1) d1 = -B / (2 * A);
+    if (d1 != d1) return NAN;
+    if (d1 < cand || orc_sweep_plant == 3) cand = d1;
+    last = T;
+  }
+  return cand <= f->band_width ? sign * cand : sign * f->phi_inf;
+}
+
+int64_t orc_reinitialize_sweep(double *phi, const orc_win *w, const orc_irregular_node *nodes, int64_t count,
+                               double band_width, double phi_inf, orc_icoor *band, int64_t band_cap,
+                               int max_sweeps, int *sweeps) {
+  fmm f = {phi, w, w->nz >= 2, band_width, phi_inf};
+  const int64_t N = (int64_t)w->nx * w->ny * (f.d3 ? w->nz : 1);
+  uint8_t *mask = malloc((size_t)N);
+  double *other = malloc(sizeof(double) * (size_t)N);
+  if (!mask || !other) {
+    free(mask);
+    free(other);
+    return -2;
+  }
+  for (int64_t i = 0; i < N; ++i) {
+    mask[i] = phi[i] > 0;
+    phi[i] = phi[i] > 0 ? phi_inf : -phi_inf;
+  }
+  int64_t nb = 0;
+  for (int64_t i = 0; i < count; ++i) {
+    const orc_irregular_node *n = &nodes[i];
+    if (n->axis != -1 || !inside(&f, n->pos.x, n->pos.y, n->pos.z)) continue;
+    if (nb < band_cap) {
+      band[nb].x = n->pos.x;
+      band[nb].y = n->pos.y;
+      band[nb].z = n->pos.z;
+    }
+    ++nb;
+    const double r = f.d3 ? sqrt(n->op[0] * n->op[0] + n->op[1] * n->op[1] + n->op[2] * n->op[2])
+                          : sqrt(n->op[0] * n->op[0] + n->op[1] * n->op[1]);
+    double *q = at(&f, n->pos.x, n->pos.y, n->pos.z);
+    *q = n->sign_center * r;
+    mask[q - phi] = 2;
+  }
+  /* settled: no node moves by more than tol in a sweep (nodes whose order
+   * is decided by their last bits can trade rounding errors forever) */
+  const double tol = band_width * SW_TOL;
+  double *in = phi, *out = other;
+  int it = 0, rc = 0, settle = 0, falling = 0;
+  int64_t changed = 1, last = -1;
+  while (changed && rc == 0) {
+    if (it >= max_sweeps) {
+      rc = -3;
+      break;
+    }
+    changed = 0;
+#pragma omp parallel for schedule(static) reduction(+ : changed) reduction(| : rc)
+    for (int64_t i = 0; i < N; ++i) {
+      double v = in[i];
+      if (mask[i] != 2) {
+        const int X = w->px + (int)(i % w->nx), Y = w->py + (int)((i / w->nx) % w->ny);
+        const int Z = f.d3 ? w->pz + (int)(i / ((int64_t)w->nx * w->ny)) : w->pz;
+        fmm g = f;
+        g.phi = in;
+        const int sg = mask[i] ? 1 : -1;
+        const double u = sw_node(&g, in, mask, sg, X, Y, Z);
+        if (u != u) rc |= 1;
+        if (!settle || sg * u < sg * v) v = u;
+      }
+      changed += fabs(v - in[i]) > tol;
+      out[i] = v;
+    }
+    if (rc) rc = -1;
+    /* a sweep that moves no fewer nodes than the one before, once the count
+     * has started to fall, leaves only nodes trading places in a cycle: from
+     * then on a node's value only goes down, which ends every cycle */
+    if (last >= 0 && changed < last) falling = 1;
+    else if (last >= 0 && falling) settle = 1;
+    last = changed;
+    double *t = in;
+    in = out;
+    out = t;
+    ++it;
+  }
+  if (in != phi) memcpy(phi, in, sizeof(double) * (size_t)N);
+  *sweeps = it;
+  if (rc == 0) {
+    for (int p = 0; p < 2; ++p)
+      for (int64_t i = 0; i < N; ++i)
+        if (mask[i] == p && fabs(phi[i]) < phi_inf) {
+          if (nb < band_cap) {
+            band[nb].x = w->px + (int)(i % w->nx);
+            band[nb].y = w->py + (int)((i / w->nx) % w->ny);
+            band[nb].z = f.d3 ? w->pz + (int)(i / ((int64_t)w->nx * w->ny)) : w->pz;
+          }
+          ++nb;
+        }
+  }
+  free(mask);
+  free(other);
+  return rc ? rc : nb;
+}

@@ -1,0 +1,108 @@
+"""Parity of the gfx950 parallel reinitialisation (dcl_reinitialize_parallel)
+with the oracle's restatement of its sweeps (reinitialize_sweep, itself
+pinned against the fast marching in test_oracle_reinit.py): phi bit for bit
+over the whole window, the band in the same order and the same number of
+sweeps.  Cases: a scaled sphere in an offset window, a noisy sphere (the
+values-only-go-down phase), a torus (cycles), an axis plane reaching the
+window's faces, a 2D circle, a batch of level sets of different sizes that
+settle after different numbers of sweeps, an empty record list, and too few
+sweeps allowed (an error)."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+import dcell_amd as D
+from dcell_amd import metrics as M
+from helpers import owin, to_dev, to_np
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(phi, win, width=8.0):
+    recs = O.irregular_nodes(phi, owin(win), O.band_compact(phi, owin(win), width))
+    dev_recs = torch.from_numpy(recs.view(np.uint8).reshape(-1).copy()).to("cuda:0")
+    return recs, dev_recs
+
+
+def _check(phi, win, recs, got_phi, got_band, got_sweeps):
+    ref_phi, ref_band, ref_sweeps = O.reinitialize_sweep(phi, owin(win), recs)
+    assert got_sweeps == ref_sweeps
+    assert to_np(got_phi).tobytes() == ref_phi.tobytes()
+    np.testing.assert_array_equal(to_np(got_band), ref_band)
+
+
+def _sphere(n, c, r, scale, noise=0.0, seed=0):
+    z, y, x = np.meshgrid(*(np.arange(n, dtype=float),) * 3, indexing="ij")
+    sdf = np.sqrt((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2) - r
+    return scale * sdf + noise * np.random.default_rng(seed).normal(size=sdf.shape)
+
+
+def _run_one(phi, win):
+    recs, drecs = _case(phi, win)
+    dphi = to_dev(phi)
+    print(f"reinitialize_parallel window {win.n} records {len(recs)}", flush=True)
+    (band,), (sweeps,) = M.reinitialize_parallel([(dphi, win, drecs, len(recs))])
+    _check(phi, win, recs, dphi, band, sweeps)
+    return sweeps
+
+
+@pytest.mark.parametrize("noise", [0.0, 0.3])
+def test_parallel_sphere_bit_exact(dev, noise):
+    n = 48
+    win = D.GridWindow((2, -3, 5), (n, n, n), (1.0, 1.0, 1.0))
+    _run_one(_sphere(n, (23.3, 22.6, 24.2), 13.4, 2.0, noise, seed=4), win)
+
+
+def test_parallel_torus(dev):
+    n = 64
+    win = D.GridWindow((0, 0, 0), (n, n, n), (1.0, 1.0, 1.0))
+    z, y, x = np.meshgrid(*(np.arange(n, dtype=float),) * 3, indexing="ij")
+    phi = 1.7 * (np.hypot(np.hypot(x - 31.7, y - 32.4) - 18.0, z - 31.1) - 7.3)
+    assert _run_one(phi, win) > 20
+
+
+def test_parallel_plane_and_edge(dev):
+    n = 40
+    win = D.GridWindow((0, 0, 0), (n, n, 24), (1.0, 1.0, 1.0))
+    z, y, x = np.meshgrid(np.arange(24.0), np.arange(float(n)), np.arange(float(n)), indexing="ij")
+    _run_one(3.0 * (y - 20.3), win)
+
+
+def test_parallel_circle_2d(dev):
+    n = 96
+    win = D.GridWindow((0, 0, 0), (n, n, 1), (1.0, 1.0, 1.0))
+    y, x = np.meshgrid(np.arange(float(n)), np.arange(float(n)), indexing="ij")
+    _run_one(2.0 * (np.hypot(x - 47.3, y - 49.1) - 30.2), win)
+
+
+def test_parallel_batch(dev):
+    cells, host = [], []
+    for i, n in enumerate((24, 40, 33, 70)):
+        win = D.GridWindow((i, -i, 2 * i), (n, n + 1, n - 1), (1.0, 1.0, 1.0))
+        phi = np.ascontiguousarray(_sphere(n, (n / 2 + 0.3, n / 2 - 0.2, n / 2), n / 4 + 0.1, 1.5, 0.1, seed=i)
+                                   [: n - 1, :, :])
+        phi = np.concatenate([phi, phi[:, :1, :]], axis=1)  # (n-1, n+1, n)
+        recs, drecs = _case(phi, win)
+        dphi = to_dev(phi)
+        cells.append((dphi, win, drecs, len(recs)))
+        host.append((phi, win, recs))
+    print("reinitialize_parallel batch of 4", flush=True)
+    bands, sweeps = M.reinitialize_parallel(cells)
+    assert len(set(sweeps)) > 1  # the level sets settle after different sweeps
+    for (phi, win, recs), (dphi, _, _, _), band, sw in zip(host, cells, bands, sweeps):
+        _check(phi, win, recs, dphi, band, sw)
+
+
+def test_parallel_empty_and_sweep_limit(dev):
+    n = 20
+    win = D.GridWindow((0, 0, 0), (n, n, n), (1.0, 1.0, 1.0))
+    phi = _sphere(n, (9.5, 9.5, 9.5), 5.2, 1.0)
+    dphi = to_dev(phi)
+    empty = torch.zeros(0, dtype=torch.uint8, device="cuda:0")
+    (band,), (sweeps,) = M.reinitialize_parallel([(dphi, win, empty, 0)])
+    assert band.shape[0] == 0 and sweeps == 1
+    assert np.array_equal(to_np(dphi), np.where(phi > 0, 7.5, -7.5))
+    recs, drecs = _case(phi, win)
+    with pytest.raises(RuntimeError, match="did not settle"):
+        M.reinitialize_parallel([(to_dev(phi), win, drecs, len(recs))], max_sweeps=3)
