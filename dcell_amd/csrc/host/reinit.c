@@ -4,7 +4,12 @@
  * irregular nodes go to the GPU, dcl_reinitialize runs the second-order fast
  * marching in the reference's heap order, and phi (the whole window: +-PHI_INF
  * outside the new band) and the new band come back into the level set.
+ * With DCELL_GPU_REINIT=parallel in the environment the parallel sweeps
+ * (dcl_reinitialize_parallel) run instead: the same band, phi equal to the
+ * fast marching's at most nodes and within its discretisation error
+ * elsewhere, and the band's two sides in window order.
  */
+#include <stdlib.h>
 #include <string.h>
 
 #include "LevelSetMethod.h"
@@ -49,7 +54,12 @@ PetscErrorCode LevelSetReinitialize(LevelSet ls) {
   cell.phi_inf = ls->PHI_INF;
   cell.band = (dcl_icoor *)dband;
   cell.band_capacity = nodes_w;
-  DL_CHECK(where, dcl_reinitialize(&cell, 1, 0, NULL));
+  const char *mode = getenv("DCELL_GPU_REINIT");
+  if (mode && strcmp(mode, "parallel") == 0) {
+    DL_CHECK(where, dcl_reinitialize_parallel(&cell, 1, 0, NULL, NULL));
+  } else {
+    DL_CHECK(where, dcl_reinitialize(&cell, 1, 0, NULL));
+  }
   DL_CHECK(where, dcl_memcpy_dtoh(ls->phi->v1, dphi, sizeof(double) * (size_t)nodes_w, NULL));
   ierr = ArraySetSize(ls->band, (int)cell.band_count); CHKERRQ(ierr);
   if (cell.band_count > 0) {

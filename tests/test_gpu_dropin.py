@@ -366,11 +366,18 @@ def test_iim_update_surface_quantities(dev, dim):
 
 
 
+@pytest.mark.parametrize("mode", ["exact", "parallel"])
 @pytest.mark.parametrize("dim", [3, 2])
-def test_levelset_reinitialize(dev, dim):
+def test_levelset_reinitialize(dev, dim, mode, monkeypatch):
     """LevelSetReinitialize (FastMarching.c:10-41) after
     LevelSetUpdateIrregularNodeList: phi over the whole window and the new
-    band equal the oracle's bit for bit, band order included."""
+    band equal the oracle's bit for bit, band order included; with
+    DCELL_GPU_REINIT=parallel, equal to the oracle's restatement of the
+    parallel sweeps."""
+    if mode == "parallel":
+        monkeypatch.setenv("DCELL_GPU_REINIT", "parallel")
+    else:
+        monkeypatch.delenv("DCELL_GPU_REINIT", raising=False)
     L = DI.lib()
     if dim == 3:
         p, n = (-3, 4, 1), (36, 34, 30)
@@ -390,7 +397,10 @@ def test_levelset_reinitialize(dev, dim):
     DI.set_array(ls.contents.band, band, 12)
     DI.check(L.LevelSetUpdateIrregularNodeList(ls), "LevelSetUpdateIrregularNodeList")
     recs = O.irregular_nodes(phi, w, band)
-    ref_phi, ref_band = O.reinitialize(phi, w, recs, ls.contents.bandWidth, ls.contents.PHI_INF)
+    if mode == "parallel":
+        ref_phi, ref_band, _ = O.reinitialize_sweep(phi, w, recs, ls.contents.bandWidth, ls.contents.PHI_INF)
+    else:
+        ref_phi, ref_band = O.reinitialize(phi, w, recs, ls.contents.bandWidth, ls.contents.PHI_INF)
     DI.check(L.LevelSetReinitialize(ls), "LevelSetReinitialize")
     assert phi.tobytes() == ref_phi.tobytes()
     np.testing.assert_array_equal(DI.get_array(ls.contents.band, np.int32, 3), ref_band)
