@@ -306,6 +306,31 @@ typedef struct {
 int dcl_surface_quantities(const void *phi, const dcl_window *win, const dcl_irregular_node *nodes,
                            int64_t count, int precision, dcl_surface *out, void *stream);
 
+/* ---------------------------------------------------------------- formats */
+/* The bytes of the .Real64 file GridWrite writes for a grid's values
+ * (Common/Serialize.c:23-55, VecView to a PETSc binary viewer): a big-endian
+ * int32 class id 1211214 and int32 count, then count big-endian doubles --
+ * the layout Visualization/DCellVTK.py:24-32 reads.  src: device, count
+ * values (DCL_F64, or DCL_F32 widened to double); dst: device, 8 + 8 * count
+ * bytes, 8-byte aligned.  Asynchronous on stream.  count < 2^31. */
+int dcl_real64_pack(const void *src, int precision, int64_t count, void *dst, void *stream);
+
+/* The rows LevelSetWriteIrregularNodeList writes for a node list
+ * (LevelSetMethod/IrregularNodes.c:145-219), big-endian doubles.  3D: one
+ * row of 23 per record, every record: X (3), pos (3), n (3), k, f1, f1_n,
+ * f1_t, f1_nn, f1_tt, f1_nt, s (3), r (3), numNei.  2D: one row of 11 per
+ * crossing record (axis != -1): X.x, X.y, pos.x, pos.y, n.x, n.y, f1, fa1, k,
+ * f1_n, f1_nn.  nodes and surf (device, count each) as dcl_irregular_nodes
+ * and dcl_surface_quantities write them; extra (device, may be NULL for
+ * zeros): the IIM fields of each record that this library does not compute,
+ * 7 per record in 3D (f1, f1_n, f1_t, f1_nn, f1_tt, f1_nt, numNei), 4 in 2D
+ * (f1, fa1, f1_n, f1_nn).  dst: device, capacity rows.  Synchronous:
+ * *rows_out (host) is the number of rows, which may exceed capacity (then
+ * only capacity are written).  count < 2^31. */
+int dcl_irregular_node_rows(const dcl_irregular_node *nodes, const dcl_surface *surf, const double *extra,
+                            int64_t count, int dim, void *dst, int64_t capacity, int64_t *rows_out,
+                            void *stream);
+
 /* Interface measure with LevelSetDiracDelta3D / 2D
  * (LevelSetMethod/NormalDirection.c:36-57) at the nodes [p+1, p+n-2]:
  * out[0] = sum delta dV (area; length in 2D), out[1] = sum delta kappa dV

@@ -416,3 +416,38 @@ def reinitialize_sweep(phi, win: Win, records, band_width=6.0, phi_inf=7.5, max_
     if n < 0:
         raise RuntimeError("reinitialisation: NaN update")
     return out, band[:n], sw.value
+
+
+# ---------------------------------------------------------------- formats
+REAL64_CLASSID = 1211214  # PETSc's VEC_FILE_CLASSID
+
+
+def real64_bytes(values):
+    """The .Real64 file GridWrite writes (Common/Serialize.c:23-55, VecView to
+    a PETSc binary viewer): big-endian int32 class id and int32 count, then
+    the values as big-endian doubles -- the layout
+    Visualization/DCellVTK.py:24-32 reads (2 * 4 header bytes, '>f8')."""
+    v = np.asarray(values, dtype=np.float64).ravel()
+    return np.array([REAL64_CLASSID, v.size], dtype=">i4").tobytes() + v.astype(">f8").tobytes()
+
+
+def irregular_rows(records, surf, extra=None, dim=3):
+    """The rows of LevelSetWriteIrregularNodeList_3D / _2D
+    (LevelSetMethod/IrregularNodes.c:178-219 / 145-172), as big-endian bytes:
+    3D, every record, X, pos, n, k, f1, f1_n, f1_t, f1_nn, f1_tt, f1_nt, s,
+    r, numNei (23 doubles); 2D, the records with axis != -1, X.x, X.y,
+    pos.x, pos.y, n.x, n.y, f1, fa1, k, f1_n, f1_nn (11 doubles).  extra:
+    the IIM fields per record (7 in 3D, 4 in 2D, in that order), zeros when
+    None."""
+    recs = np.ascontiguousarray(records, dtype=IRREG_DTYPE)
+    sq = np.ascontiguousarray(surf, dtype=SURF_DTYPE)
+    nx = 7 if dim == 3 else 4
+    ex = np.zeros((len(recs), nx)) if extra is None else np.asarray(extra, dtype=np.float64).reshape(len(recs), nx)
+    pos = recs["pos"].astype(np.float64)
+    if dim == 3:
+        rows = np.column_stack([recs["X"], pos, sq["n"], sq["k"], ex[:, :6], sq["s"], sq["r"], ex[:, 6]])
+    else:
+        keep = recs["axis"] != -1
+        rows = np.column_stack([recs["X"][:, :2], pos[:, :2], sq["n"][:, :2], ex[:, 0], ex[:, 1], sq["k"],
+                                ex[:, 2], ex[:, 3]])[keep]
+    return np.ascontiguousarray(rows, dtype=np.float64).astype(">f8").tobytes()
