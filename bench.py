@@ -600,6 +600,54 @@ def get_velocity_run(n=256, steps=20):
                 roofline_frac=gbs * 1e9 / HBM_PEAK, bytes_per_node=48)
 
 
+def formats_run(n=256, steps=20):
+    """The output formats from device-resident data: the .Real64 bytes of an
+    n^3 FP64 grid (16 B per value: 8 read, 8 written) with HIP events on the
+    current stream against the HBM peak, the whole grid_write (pack, one copy
+    to the host, the file write), and the node-list rows of the n^3 sphere's
+    6-cell band (23 doubles per record)."""
+    import tempfile
+    import torch
+    import dcell_amd as D
+    from dcell_amd import configs
+    from dcell_amd import formats as F
+    from dcell_amd import metrics as M
+
+    win = D.GridWindow.unit_box(n)
+    phi = torch.from_numpy(configs.sphere_sdf(n)).to("cuda:0")
+    out = torch.empty(8 + 8 * n ** 3, dtype=torch.uint8, device="cuda:0")
+    F.real64_pack(phi, out)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        F.real64_pack(phi, out)
+    e1.record()
+    torch.cuda.synchronize()
+    t = e0.elapsed_time(e1) * 1e-3 / steps
+    gbs = n ** 3 * 16 / t / 1e9
+    with tempfile.TemporaryDirectory() as d:
+        F.grid_write(d, "phi", 0, phi, win)
+        t0 = time.perf_counter()
+        F.grid_write(d, "phi", 1, phi, win)
+        t_write = time.perf_counter() - t0
+    band, nb = M.band_compact(phi, win, 6.0)
+    recs = torch.empty(10 * nb * M.IRREGULAR_DTYPE.itemsize, dtype=torch.uint8, device="cuda:0")
+    count = M.irregular_nodes_into(phi, win, band.contiguous(), recs)
+    surf = torch.empty(count * M.SURFACE_DTYPE.itemsize, dtype=torch.uint8, device="cuda:0")
+    M.surface_quantities_into(phi, win, recs, count, surf)
+    F.irregular_node_rows(recs, count, surf, 3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        F.irregular_node_rows(recs, count, surf, 3)
+    t_rows = (time.perf_counter() - t0) / steps
+    return dict(real64=dict(n=n, ms_per_call=t * 1e3, values_per_s=n ** 3 / t, achieved_gbs=gbs,
+                            roofline_frac=gbs * 1e9 / HBM_PEAK, bytes_per_value=16),
+                grid_write_ms=t_write * 1e3,
+                irregular_rows=dict(records=count, ms_per_call=t_rows * 1e3, records_per_s=count / t_rows))
+
+
 def _reinit_inputs(n, scale, center, radius, torch, D, M, dev="cuda:0"):
     """phi = scale x the distance (grid units) to a sphere in an n^3 window,
     its 8-cell band and irregular-node records, all on the device."""
@@ -781,6 +829,7 @@ FORMS = {
     "get_velocity_256_fp64": lambda: get_velocity_run(256, 20),
     "reinitialize_fp64": lambda: reinit_run(256, 5),
     "reinitialize_parallel_fp64": lambda: reinit_run(256, 5, exact=False),
+    "formats_256_fp64": lambda: formats_run(256, 20),
     "dropin_band_256_fp64": lambda: dropin_run(256, 10),
 }
 
