@@ -834,8 +834,12 @@ FORMS = {
 }
 
 
+# forms run on their own only (the full line already holds their figures)
+FORM_ONLY = {"reinitialize_parallel_fp64"}
+
+
 def forms(args):
-    return {name: run() for name, run in FORMS.items()}
+    return {name: run() for name, run in FORMS.items() if name not in FORM_ONLY}
 
 
 def main():
