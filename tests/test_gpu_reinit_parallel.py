@@ -5,8 +5,10 @@ over the whole window, the band in the same order and the same number of
 sweeps.  Cases: a scaled sphere in an offset window, a noisy sphere (the
 values-only-go-down phase), a torus (cycles), an axis plane reaching the
 window's faces, a 2D circle, a batch of level sets of different sizes that
-settle after different numbers of sweeps, an empty record list, and too few
-sweeps allowed (an error)."""
+settle after different numbers of sweeps, a 2D batch holding a level set
+without records, an empty record list, a band capacity below the band's
+length (the count still whole, the prefix written), too few sweeps allowed
+and a batch mixing 2D and 3D (errors)."""
 import numpy as np
 import pytest
 import torch
@@ -106,3 +108,51 @@ def test_parallel_empty_and_sweep_limit(dev):
     recs, drecs = _case(phi, win)
     with pytest.raises(RuntimeError, match="did not settle"):
         M.reinitialize_parallel([(to_dev(phi), win, drecs, len(recs))], max_sweeps=3)
+
+
+def test_parallel_batch_2d_with_empty_cell(dev):
+    """A 2D batch: circles of different sizes and a level set without
+    records (it settles after one sweep, all +-PHI_INF)."""
+    cells, host = [], []
+    for i, n in enumerate((40, 72, 30, 57)):
+        win = D.GridWindow((i - 3, 2 * i, 0), (n, n + 3, 1), (1.0, 1.0, 1.0))
+        y, x = np.meshgrid(np.arange(float(n + 3)), np.arange(float(n)), indexing="ij")
+        phi = 1.5 * (np.hypot(x - n / 2 + 0.3, y - n / 2 - 0.4) - n / 4 - 0.2)
+        if i == 2:
+            drecs = torch.zeros(0, dtype=torch.uint8, device="cuda:0")
+            recs = np.zeros(0, dtype=O.IRREG_DTYPE)
+        else:
+            recs, drecs = _case(phi, win)
+        dphi = to_dev(phi)
+        cells.append((dphi, win, drecs, len(recs)))
+        host.append((phi, win, recs))
+    bands, sweeps = M.reinitialize_parallel(cells)
+    assert sweeps[2] == 1 and bands[2].shape[0] == 0
+    for (phi, win, recs), (dphi, _, _, _), band, sw in zip(host, cells, bands, sweeps):
+        _check(phi, win, recs, dphi, band, sw)
+
+
+def test_parallel_band_capacity_and_mixed_dims(dev):
+    import ctypes as C
+    from dcell_amd import _lib as L
+    n = 36
+    win = D.GridWindow((0, 0, 0), (n, n, n), (1.0, 1.0, 1.0))
+    phi = _sphere(n, (17.5, 18.2, 17.1), 9.3, 2.0)
+    recs, drecs = _case(phi, win)
+    ref_phi, ref_band, ref_sw = O.reinitialize_sweep(phi, owin(win), recs)
+    cap = len(ref_band) // 3
+    dphi = to_dev(phi)
+    band = torch.full((cap, 3), -7, dtype=torch.int32, device="cuda:0")
+    cell = L.ReinitCell()
+    cell.phi, cell.win, cell.nodes, cell.count = dphi.data_ptr(), win.c(), drecs.data_ptr(), len(recs)
+    cell.band_width, cell.phi_inf, cell.band, cell.band_capacity = 6.0, 7.5, band.data_ptr(), cap
+    sw = (C.c_int * 1)()
+    L.call("dcl_reinitialize_parallel", C.byref(cell), 1, 0, sw, None)
+    assert cell.band_count == len(ref_band) and sw[0] == ref_sw
+    np.testing.assert_array_equal(to_np(band), ref_band[:cap])
+    assert to_np(dphi).tobytes() == ref_phi.tobytes()
+    win2 = D.GridWindow((0, 0, 0), (20, 20, 1), (1.0, 1.0, 1.0))
+    phi2 = torch.zeros((20, 20), dtype=torch.float64, device="cuda:0")
+    empty = torch.zeros(0, dtype=torch.uint8, device="cuda:0")
+    with pytest.raises(RuntimeError, match="mixes 2D and 3D"):
+        M.reinitialize_parallel([(to_dev(phi), win, drecs, len(recs)), (phi2, win2, empty, 0)])
