@@ -51,6 +51,7 @@ constexpr int RS_CH = 1024;      // nodes per chunk of the mark scan (four per t
 constexpr int RS_STAGE = 2048;   // list entries a block gathers before publishing them
 constexpr int RS_NEED_GRID = 2048;  // blocks of the mark scan
 constexpr int RS_GRID = 4096;       // blocks of the update, each taking listed nodes in turn
+constexpr int RS_GROUP = 4;         // sweeps queued between looks at the count of level sets still sweeping
 
 struct RsCell {
   double* buf[2];        // buf[0]: the caller's phi, buf[1]: scratch
@@ -727,31 +728,52 @@ extern "C" int dcl_reinitialize_parallel(dcl_reinit_cell* cells, int ncells, int
     }
     if (hipGetLastError() != hipSuccess) rc_ = set_error(DCL_ERR_LIB, "reinit reset / boundary launch failed");
   }
-  // the sweeps; the count of level sets still sweeping is read every few
-  // sweeps (a level set that has settled skips the sweeps after)
+  // the sweeps, in groups of RS_GROUP: once a group is queued, the count of
+  // level sets still sweeping after the group before it is awaited (through
+  // pinned memory), so the GPU always holds the next group; a level set
+  // that has settled skips the sweeps queued after it
+  static thread_local int* pin = nullptr;
+  if (!pin && !rc_) {
+    hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&pin), 2 * sizeof(int), hipHostMallocDefault);
+    if (e != hipSuccess) {
+      pin = nullptr;
+      rc_ = hip_fail(e, "reinit pinned count", __FILE__, __LINE__);
+    }
+  }
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  for (int q = 0; q < 2 && !rc_; ++q) {
+    hipError_t e = hipEventCreateWithFlags(&ev[q], hipEventDisableTiming);
+    if (e != hipSuccess) rc_ = hip_fail(e, "reinit events", __FILE__, __LINE__);
+  }
   const unsigned ng = (unsigned)(nch < RS_NEED_GRID ? nch : RS_NEED_GRID);
-  for (int j = 0; j < max_sweeps && !rc_; ++j) {
-    hipLaunchKernelGGL(rs_need_k, dim3(ng), dim3(RS_B), 0, s, dcells, ncells, dstate, nch, nlist, ncount + j);
-    if (d3)
-      hipLaunchKernelGGL(rs_update_k<3>, dim3(RS_GRID), dim3(RS_B), 0, s, dcells, dstate, nlist, ncount + j, parts,
-                         nparts, pstride);
-    else
-      hipLaunchKernelGGL(rs_update_k<2>, dim3(RS_GRID), dim3(RS_B), 0, s, dcells, dstate, nlist, ncount + j, parts,
-                         nparts, pstride);
-    hipLaunchKernelGGL(rs_state_k, dim3(grid1d(ncells, RS_B / 64)), dim3(RS_B), 0, s, dstate, ncells, max_sweeps,
-                       active + j, parts, nparts, pstride);
+  bool settled = false;
+  for (int j0 = 0, grp = 0; j0 < max_sweeps && !rc_ && !settled; j0 += RS_GROUP, ++grp) {
+    const int j1 = j0 + RS_GROUP < max_sweeps ? j0 + RS_GROUP : max_sweeps;
+    for (int j = j0; j < j1; ++j) {
+      hipLaunchKernelGGL(rs_need_k, dim3(ng), dim3(RS_B), 0, s, dcells, ncells, dstate, nch, nlist, ncount + j);
+      if (d3)
+        hipLaunchKernelGGL(rs_update_k<3>, dim3(RS_GRID), dim3(RS_B), 0, s, dcells, dstate, nlist, ncount + j, parts,
+                           nparts, pstride);
+      else
+        hipLaunchKernelGGL(rs_update_k<2>, dim3(RS_GRID), dim3(RS_B), 0, s, dcells, dstate, nlist, ncount + j, parts,
+                           nparts, pstride);
+      hipLaunchKernelGGL(rs_state_k, dim3(grid1d(ncells, RS_B / 64)), dim3(RS_B), 0, s, dstate, ncells, max_sweeps,
+                         active + j, parts, nparts, pstride);
+    }
     if (hipGetLastError() != hipSuccess) {
       rc_ = set_error(DCL_ERR_LIB, "reinit sweep launch failed");
       break;
     }
-    if ((j & 3) == 3 || j == max_sweeps - 1) {
-      int a = 0;
-      hipError_t e = hipMemcpyAsync(&a, active + j, sizeof(int), hipMemcpyDeviceToHost, s);
-      if (e == hipSuccess) e = hipStreamSynchronize(s);
-      if (e != hipSuccess) rc_ = hip_fail(e, "reinit sweeps", __FILE__, __LINE__);
-      if (a == 0) break;
+    hipError_t e = hipMemcpyAsync(pin + (grp & 1), active + (j1 - 1), sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipEventRecord(ev[grp & 1], s);
+    if (e == hipSuccess && grp > 0) {
+      e = hipEventSynchronize(ev[(grp - 1) & 1]);
+      if (e == hipSuccess && pin[(grp - 1) & 1] == 0) settled = true;
     }
+    if (e != hipSuccess) rc_ = hip_fail(e, "reinit sweeps", __FILE__, __LINE__);
   }
+  for (int q = 0; q < 2; ++q)
+    if (ev[q]) (void)hipEventDestroy(ev[q]);
   if (!rc_) {
     hipError_t e = hipMemcpyAsync(hstate, dstate, sizeof(RsState) * ncells, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
