@@ -600,6 +600,37 @@ def get_velocity_run(n=256, steps=20):
                 roofline_frac=gbs * 1e9 / HBM_PEAK, bytes_per_node=48)
 
 
+def band_mode_run(n=256, steps=30, warmup=3):
+    """LevelSetAdvectAndReinit in band mode with the level set resident in HBM
+    (dcell_amd.advect.DeviceLevelSet): the n^3 sphere in the deformation
+    field, dt = h (cos(pi t / 3) folded in), Euler + trilinear SL over the
+    6-cell band, the CFL count and the irregular-node list every step, and
+    the parallel reinitialisation whenever the CFL count passes 3 (the
+    reference's threshold).  Wall time per step over the timed steps."""
+    import torch
+    import dcell_amd as D
+    from dcell_amd import configs
+    from dcell_amd.advect import DeviceLevelSet
+
+    win = D.GridWindow.unit_box(n)
+    vel = torch.from_numpy(configs.deformation_velocity(n)).to("cuda:0")
+    phi = torch.from_numpy(configs.sphere_sdf(n)).to("cuda:0")
+    ls = DeviceLevelSet(phi, win, reinit="parallel")
+    ls.update_irregular_nodes()
+    for k in range(warmup):
+        ls.advect_and_reinit(vel, configs.deformation_dt(n, k))
+    torch.cuda.synchronize()
+    r0, band_nodes = ls.reinits, 0
+    t0 = time.perf_counter()
+    for k in range(warmup, warmup + steps):
+        band_nodes += ls.band.shape[0]
+        ls.advect_and_reinit(vel, configs.deformation_dt(n, k))
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / steps
+    return dict(n=n, steps=steps, ms_per_step=t * 1e3, reinits=ls.reinits - r0,
+                band_nodes_per_step=band_nodes / steps, band_updates_per_s=band_nodes / steps / t)
+
+
 def formats_run(n=256, steps=20):
     """The output formats from device-resident data: the .Real64 bytes of an
     n^3 FP64 grid (16 B per value: 8 read, 8 written) with HIP events on the
@@ -830,6 +861,7 @@ FORMS = {
     "reinitialize_fp64": lambda: reinit_run(256, 5),
     "reinitialize_parallel_fp64": lambda: reinit_run(256, 5, exact=False),
     "formats_256_fp64": lambda: formats_run(256, 20),
+    "band_mode_256_fp64": lambda: band_mode_run(256, 30),
     "dropin_band_256_fp64": lambda: dropin_run(256, 10),
 }
 
