@@ -57,6 +57,7 @@ struct RsCell {
   double* buf[2];        // buf[0]: the caller's phi, buf[1]: scratch
   unsigned char* mask;   // 0: a - side node, 1: a + side node, 2: a boundary node
   unsigned char* nd[2];  // marks: sweep k + 1 recomputes the nodes marked in nd[k & 1]
+  unsigned char* cd[2];  // per chunk of RS_CH nodes: it holds a mark in nd[k & 1]
   KWin w;
   int d3;
   int small;  // below 2^28 nodes: buffer loads
@@ -101,15 +102,26 @@ __device__ __forceinline__ int64_t rs_nodes(const RsCell& c) { return (int64_t)c
 // a node changed: the nodes whose stencil holds it (itself and two nodes
 // each way along each axis) are recomputed in the next sweep
 template <int DIM>
-__device__ __forceinline__ void rs_mark(const RsCell& c, unsigned char* nd, int64_t i, int lx, int ly, int lz) {
+__device__ __forceinline__ void rs_mark(const RsCell& c, unsigned char* nd, unsigned char* cd, int64_t i, int lx,
+                                        int ly, int lz) {
   const int64_t sy = c.w.nx, sz = (int64_t)c.w.nx * c.w.ny;
   nd[i] = 1;
+  cd[i / RS_CH] = 1;
 #pragma unroll
   for (int o = -2; o <= 2; ++o) {
     if (o == 0) continue;
-    if (lx + o >= 0 && lx + o < c.w.nx) nd[i + o] = 1;
-    if (ly + o >= 0 && ly + o < c.w.ny) nd[i + o * sy] = 1;
-    if (DIM == 3 && lz + o >= 0 && lz + o < c.w.nz) nd[i + o * sz] = 1;
+    if (lx + o >= 0 && lx + o < c.w.nx) {
+      nd[i + o] = 1;
+      cd[(i + o) / RS_CH] = 1;
+    }
+    if (ly + o >= 0 && ly + o < c.w.ny) {
+      nd[i + o * sy] = 1;
+      cd[(i + o * sy) / RS_CH] = 1;
+    }
+    if (DIM == 3 && lz + o >= 0 && lz + o < c.w.nz) {
+      nd[i + o * sz] = 1;
+      cd[(i + o * sz) / RS_CH] = 1;
+    }
   }
 }
 
@@ -128,6 +140,10 @@ __global__ __launch_bounds__(RS_B) void rs_reset_k(const RsCell* __restrict__ ce
   c.buf[1][i] = v;
   c.nd[0][i] = 0;
   c.nd[1][i] = 0;
+  if (i % RS_CH == 0) {
+    c.cd[0][i / RS_CH] = 0;
+    c.cd[1][i / RS_CH] = 0;
+  }
 }
 
 // FMM_InitializeEikonal (FastMarching.c:60-74): the boundary values, which
@@ -149,7 +165,7 @@ __global__ __launch_bounds__(RS_B) void rs_bc_k(const RsCell* __restrict__ cells
   c.buf[0][j] = v;
   c.buf[1][j] = v;
   c.mask[j] = 2;
-  rs_mark<DIM>(c, c.nd[0], j, n.pos.x - c.w.px, n.pos.y - c.w.py, c.d3 ? n.pos.z - c.w.pz : 0);
+  rs_mark<DIM>(c, c.nd[0], c.cd[0], j, n.pos.x - c.w.px, n.pos.y - c.w.py, c.d3 ? n.pos.z - c.w.pz : 0);
 }
 
 // a byte or a double of a level set's node array (buffer loads below 2^28
@@ -300,43 +316,66 @@ __global__ __launch_bounds__(RS_B) void rs_need_k(const RsCell* __restrict__ cel
   __shared__ int2 stage[RS_STAGE];
   __shared__ unsigned wsum[RS_B / 64];
   __shared__ int at;
+  __shared__ int live[RS_B];  // this round's chunks holding marks, as batch chunk indices
   const int tid = threadIdx.x;
   int held = 0;  // entries in stage (uniform)
-  for (long long g = blockIdx.x;; g += gridDim.x) {
-    const bool live = g < nch;
-    if (held + RS_CH > RS_STAGE || (!live && held > 0)) {
-      if (tid == 0) at = atomicAdd(ncount, held);
-      __syncthreads();
-      for (int q = tid; q < held; q += RS_B) nlist[at + q] = stage[q];
-      __syncthreads();
-      held = 0;
+  // the block's chunks are blockIdx.x + q * gridDim.x; their flags are read
+  // RS_B at a time, one per thread, and the marked ones visited in turn
+  const long long per = (nch - blockIdx.x + gridDim.x - 1) / gridDim.x;
+  for (long long q0 = 0; q0 < per; q0 += RS_B) {
+    const long long g = blockIdx.x + (q0 + tid) * gridDim.x;
+    bool has = false;
+    if (q0 + tid < per) {
+      const int ci = rs_find<RS_KEY_CHUNK>(cells, ncells, g);
+      const RsCell& c = cells[ci];
+      if (!st[ci].done) {
+        unsigned char* cdk = (st[ci].k & 1) ? c.cd[1] : c.cd[0];
+        has = cdk[g - c.cb0] != 0;
+        if (has) cdk[g - c.cb0] = 0;
+      }
     }
-    if (!live) break;
-    const int ci = rs_find<RS_KEY_CHUNK>(cells, ncells, g);
-    const RsCell& c = cells[ci];
-    if (st[ci].done) continue;
-    unsigned char* nd = (st[ci].k & 1) ? c.nd[1] : c.nd[0];
-    const int64_t n = rs_nodes(c);
-    const int64_t i0 = (g - c.cb0) * RS_CH + 4 * tid;
-    unsigned w = 0;
-    if (i0 < n) {
-      // the mark arrays are padded to 256 bytes, so a whole word is in them
-      if (c.small)
-        w = __builtin_amdgcn_raw_buffer_load_b32(make_rsrc(nd, (unsigned)((n + 3) & ~3ll)), (int)i0, 0, 0);
-      else
-        w = *reinterpret_cast<const unsigned*>(nd + i0);
-      if (i0 + 4 > n) w &= n - i0 == 1 ? 0xffu : n - i0 == 2 ? 0xffffu : 0xffffffu;
-      if (w) *reinterpret_cast<unsigned*>(nd + i0) = 0u;
-    }
-    if (!__syncthreads_or(w != 0u)) continue;  // most chunks hold no mark
-    const int cnt = ((w & 0xffu) != 0) + ((w & 0xff00u) != 0) + ((w & 0xff0000u) != 0) + ((w >> 24) != 0);
-    int tot;
-    int pos = held + rs_scan(cnt, wsum, &tot);
-#pragma unroll
-    for (int b = 0; b < 4; ++b)
-      if ((w >> (8 * b)) & 0xffu) stage[pos++] = make_int2(ci, (int)(i0 + b));
-    held += tot;
+    int nlive;
+    const int slot = rs_scan(has ? 1 : 0, wsum, &nlive);
+    if (has) live[slot] = (int)g;
     __syncthreads();
+    for (int e = 0; e < nlive; ++e) {
+      if (held + RS_CH > RS_STAGE) {
+        if (tid == 0) at = atomicAdd(ncount, held);
+        __syncthreads();
+        for (int q = tid; q < held; q += RS_B) nlist[at + q] = stage[q];
+        __syncthreads();
+        held = 0;
+      }
+      const long long gc = live[e];
+      const int ci = rs_find<RS_KEY_CHUNK>(cells, ncells, gc);
+      const RsCell& c = cells[ci];
+      unsigned char* nd = (st[ci].k & 1) ? c.nd[1] : c.nd[0];
+      const int64_t n = rs_nodes(c);
+      const int64_t i0 = (gc - c.cb0) * RS_CH + 4 * tid;
+      unsigned w = 0;
+      if (i0 < n) {
+        // the mark arrays are padded to 256 bytes, so a whole word is in them
+        if (c.small)
+          w = __builtin_amdgcn_raw_buffer_load_b32(make_rsrc(nd, (unsigned)((n + 3) & ~3ll)), (int)i0, 0, 0);
+        else
+          w = *reinterpret_cast<const unsigned*>(nd + i0);
+        if (i0 + 4 > n) w &= n - i0 == 1 ? 0xffu : n - i0 == 2 ? 0xffffu : 0xffffffu;
+        if (w) *reinterpret_cast<unsigned*>(nd + i0) = 0u;
+      }
+      const int cnt = ((w & 0xffu) != 0) + ((w & 0xff00u) != 0) + ((w & 0xff0000u) != 0) + ((w >> 24) != 0);
+      int tot;
+      int pos = held + rs_scan(cnt, wsum, &tot);
+#pragma unroll
+      for (int b2 = 0; b2 < 4; ++b2)
+        if ((w >> (8 * b2)) & 0xffu) stage[pos++] = make_int2(ci, (int)(i0 + b2));
+      held += tot;
+      __syncthreads();
+    }
+  }
+  if (held > 0) {
+    if (tid == 0) at = atomicAdd(ncount, held);
+    __syncthreads();
+    for (int q = tid; q < held; q += RS_B) nlist[at + q] = stage[q];
   }
 }
 
@@ -361,6 +400,7 @@ __global__ __launch_bounds__(RS_B) void rs_update_k(const RsCell* __restrict__ c
       const double* __restrict__ in = (k & 1) ? c.buf[1] : c.buf[0];
       double* __restrict__ out = (k & 1) ? c.buf[0] : c.buf[1];
       unsigned char* __restrict__ ncur = (k & 1) ? c.nd[0] : c.nd[1];
+      unsigned char* __restrict__ ccur = (k & 1) ? c.cd[0] : c.cd[1];
       const int64_t i = w.y;
       const int64_t sy = c.w.nx, sz = (int64_t)c.w.nx * c.w.ny;
       const int lz = DIM == 3 ? (int)(i / sz) : 0;
@@ -395,7 +435,7 @@ __global__ __launch_bounds__(RS_B) void rs_update_k(const RsCell* __restrict__ c
         if (!settle || sg * u < sg * v) v = u;
         moved = fabs(v - old) > c.tol;
         out[i] = v;
-        if (__double_as_longlong(v) != __double_as_longlong(old)) rs_mark<DIM>(c, ncur, i, lx, ly, lz);
+        if (__double_as_longlong(v) != __double_as_longlong(old)) rs_mark<DIM>(c, ncur, ccur, i, lx, ly, lz);
       }
     }
     // one atomic per wave where the wave's nodes are of one level set
@@ -629,7 +669,8 @@ extern "C" int dcl_reinitialize_parallel(dcl_reinit_cell* cells, int ncells, int
     nq += (n + RS_B - 1) / RS_B;
     nch += (n + RS_CH - 1) / RS_CH;
     nr += (u.count + RS_B - 1) / RS_B;
-    cell_bytes += rs_al(sizeof(double) * (size_t)n) + 3 * rs_al((size_t)n);
+    cell_bytes += rs_al(sizeof(double) * (size_t)n) + 3 * rs_al((size_t)n) +
+                  2 * rs_al((size_t)((n + RS_CH - 1) / RS_CH));
   }
   if (nodes >= INT32_MAX || nr >= INT32_MAX)
     return set_error(DCL_ERR_ARG_SIZ, "a batch of more than 2^31 - 1 nodes or record blocks");
@@ -689,6 +730,8 @@ extern "C" int dcl_reinitialize_parallel(dcl_reinit_cell* cells, int ncells, int
     c.mask = reinterpret_cast<unsigned char*>(take((size_t)n));
     c.nd[0] = reinterpret_cast<unsigned char*>(take((size_t)n));
     c.nd[1] = reinterpret_cast<unsigned char*>(take((size_t)n));
+    c.cd[0] = reinterpret_cast<unsigned char*>(take((size_t)((n + RS_CH - 1) / RS_CH)));
+    c.cd[1] = reinterpret_cast<unsigned char*>(take((size_t)((n + RS_CH - 1) / RS_CH)));
     c.w = kwin(u.win);
     c.d3 = d3;
     c.small = n < (1 << 28);
