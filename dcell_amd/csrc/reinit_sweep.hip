@@ -25,12 +25,14 @@
 // each sweep recomputes only the marked nodes; the output buffer already
 // holds the others' values (both buffers start equal, and a node not
 // recomputed did not change in the sweep before).  The boundary values are
-// the changes before the first sweep.  A sweep is two launches: one lists
-// the marked nodes (reading and clearing the marks four at a time, in
-// chunks of 1024 nodes), one updates the listed nodes, one thread each, so
-// that whole waves run the chain of updates.  Lists are gathered per block
-// in LDS and published with one atomic each: an atomic per wave on one
-// counter would queue.
+// the changes before the first sweep.  A sweep is three launches: one lists
+// the marked nodes (each chunk of 1024 nodes carries a flag set with its
+// marks, so a block reads its chunks' flags at once and scans only the
+// marked chunks, four marks per thread, clearing them), one updates the
+// listed nodes, one thread each, so that whole waves run the chain of
+// updates, and one wave per level set sums the moved nodes and decides.
+// Lists are gathered per block in LDS and published with one atomic each:
+// an atomic per wave on one counter would queue.
 //
 // A batch of level sets runs in one launch per step, over all nodes (in
 // raster order) and all records, each block finding its level set by a
