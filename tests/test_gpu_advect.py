@@ -1,9 +1,9 @@
 """DeviceLevelSet.advect_and_reinit (LevelSetAdvectAndReinit,
 LevelSetMethod/Advect.c:31-50) with every stage in HBM, checked stage by
 stage against the oracle over eight steps of the 40^3 sphere in the
-deformation field (dt = 1.2 h, so the CFL count passes 3 every few steps and
-reinitialisations happen mid-run), for both reinitialisations and both SL
-schemes:
+deformation field and of a 96^2 circle in a rigid rotation (dt = 1.2 h, so
+the CFL count passes 3 every few steps and reinitialisations happen
+mid-run), for both reinitialisations and both SL schemes:
 
 - the SL step over the band from the step's starting phi (FP64, 1e-12 of
   max |phi|; nodes off the band untouched: GridCopy);
@@ -35,14 +35,19 @@ def _host_records(ls):
 
 @pytest.mark.parametrize("reinit", ["exact", "parallel"])
 @pytest.mark.parametrize("scheme", list(SCHEMES))
-def test_advect_and_reinit_stages(dev, reinit, scheme):
-    n = 40
-    win = D.GridWindow.unit_box(n)
+@pytest.mark.parametrize("dim", [3, 2])
+def test_advect_and_reinit_stages(dev, reinit, scheme, dim):
+    n = 40 if dim == 3 else 96
+    win = D.GridWindow.unit_box(n, dim)
     w = owin(win)
     h = win.d[0]
-    vel_np = configs.deformation_velocity(n)
+    if dim == 3:
+        vel_np = configs.deformation_velocity(n)
+        phi = torch.from_numpy(configs.sphere_sdf(n)).to("cuda:0")
+    else:
+        vel_np = 0.6 * configs.rotation_velocity(n)
+        phi = torch.from_numpy(configs.circle_sdf(n, (0.55, 0.45), 0.2)).to("cuda:0")
     vel = torch.from_numpy(vel_np).to("cuda:0")
-    phi = torch.from_numpy(configs.sphere_sdf(n)).to("cuda:0")
     ls = DeviceLevelSet(phi, win, interp=scheme[0], backtrace=scheme[1], reinit=reinit)
     ls.update_irregular_nodes()
     seen = {}
@@ -64,12 +69,13 @@ def test_advect_and_reinit_stages(dev, reinit, scheme):
         did = ls.advect_and_reinit(vel, dt)
         # SL over the band
         phi_o = phi_b.copy()
-        phi_o[band_b[:, 2], band_b[:, 1], band_b[:, 0]] = O.sl_advect(phi_b, vel_np, w, dt, interp, bt, band=band_b)
+        idx = (band_b[:, 2], band_b[:, 1], band_b[:, 0]) if dim == 3 else (band_b[:, 1], band_b[:, 0])
+        phi_o[idx] = O.sl_advect(phi_b, vel_np, w, dt, interp, bt, band=band_b)
         post = seen["phi"] if did else to_np(ls.phi)
         scale = np.abs(phi_o).max()
         assert np.abs(post - phi_o).max() <= 1e-12 * scale
         # CFL increment from the previous list
-        mv = O.max_velocity(vel_np, w, recs_b["pos"].copy()) if len(recs_b) else 0.0
+        mv = O.max_velocity(vel_np, w, recs_b["pos"].copy(), dim=dim) if len(recs_b) else 0.0
         assert abs(ls.max_vel - mv) <= 1e-12 * max(mv, 1.0)
         cfl += dt * mv / h
         adv += 1
