@@ -260,9 +260,10 @@ int dcl_reinitialize(dcl_reinit_cell *cells, int ncells, int64_t heap_capacity, 
  * marching would give it if its neighbours were accepted in the order of
  * their values (its pushes replayed in that order), from the previous
  * sweep's values, until no node moves by more than band_width * 1e-9.
- * Where the fast marching's acceptance order is its value order, phi comes
- * out bit for bit as dcl_reinitialize's; elsewhere it differs by far less
- * than either's distance error.  band = [projection nodes in record order |
+ * phi comes out bit for bit as dcl_reinitialize's at most nodes (98% of the
+ * band of a 256^3 sphere: wherever the fast marching accepts in value order
+ * and no tie is decided by its heap's layout); elsewhere the two differ by a
+ * fraction of either's distance error (the same band set but at band_width).  band = [projection nodes in record order |
  * - side | + side], each side in window order (z, y, x).  max_sweeps: the
  * sweeps allowed per level set (0: 256; not settling is an error); sweeps
  * (host, ncells entries, may be NULL) gets the sweeps each level set took.
