@@ -98,3 +98,24 @@ def test_irregular_rows_bytes(dev, tmp_path, dim, with_extra):
 def test_irregular_rows_empty(dev):
     e = torch.zeros(0, dtype=torch.uint8, device="cuda:0")
     assert F.irregular_node_rows(e, 0, e, 3).numel() == 0
+
+
+def test_irregular_rows_2d_many_tiles(dev):
+    """2D rows of 300 000 synthetic records (about a third projections): the
+    kept records' positions come from a scan over more than one scan tile."""
+    n = 300_000
+    rng = np.random.default_rng(11)
+    recs = np.zeros(n, dtype=O.IRREG_DTYPE)
+    recs["pos"] = rng.integers(-1000, 1000, (n, 3))
+    recs["axis"] = rng.integers(-1, 2, n)
+    recs["X"] = rng.normal(size=(n, 3))
+    surf = np.zeros(n, dtype=O.SURF_DTYPE)
+    surf["n"] = rng.normal(size=(n, 3))
+    surf["k"] = rng.normal(size=n)
+    extra = rng.normal(size=(n, 4))
+    d_recs = torch.from_numpy(recs.view(np.uint8).copy()).to("cuda:0")
+    d_surf = torch.from_numpy(surf.view(np.uint8).copy()).to("cuda:0")
+    d_extra = torch.from_numpy(extra).to("cuda:0")
+    got = F.irregular_node_rows(d_recs, n, d_surf, 2, d_extra).cpu().numpy().tobytes()
+    assert len(got) == (recs["axis"] != -1).sum() * 11 * 8
+    assert got == O.irregular_rows(recs, surf, extra, 2)
