@@ -8,7 +8,9 @@ window's faces, a 2D circle, a batch of level sets of different sizes that
 settle after different numbers of sweeps, a 2D batch holding a level set
 without records, an empty record list, a band capacity below the band's
 length (the count still whole, the prefix written), too few sweeps allowed
-and a batch mixing 2D and 3D (errors)."""
+and a batch mixing 2D and 3D (errors), and a 294M-node window (the
+large-window load path) checked against the closed-form distance to a
+plane."""
 import numpy as np
 import pytest
 import torch
@@ -156,3 +158,30 @@ def test_parallel_band_capacity_and_mixed_dims(dev):
     empty = torch.zeros(0, dtype=torch.uint8, device="cuda:0")
     with pytest.raises(RuntimeError, match="mixes 2D and 3D"):
         M.reinitialize_parallel([(to_dev(phi), win, drecs, len(recs)), (phi2, win2, empty, 0)])
+
+
+def test_parallel_large_window_flat_path(dev):
+    """A window of 2^28 nodes or more (700 x 700 x 600) takes the large-window
+    path (64-bit pointer loads instead of 32-bit buffer offsets).  phi = 3 x
+    the distance to the plane z = 300.3, so the answer is closed-form: the
+    exact distance on every node within band_width (1e-12, each z-plane
+    uniform), +-PHI_INF elsewhere, and the band is exactly those nodes."""
+    nx, ny, nz = 700, 700, 600
+    assert nx * ny * nz >= 1 << 28
+    win = D.GridWindow((0, 0, 0), (nx, ny, nz), (1.0, 1.0, 1.0))
+    sdf = torch.arange(nz, dtype=torch.float64, device="cuda:0") - 300.3
+    phi = (3.0 * sdf).view(nz, 1, 1).expand(nz, ny, nx).contiguous()
+    band, nb = M.band_compact(phi, win, 8.0)
+    recs = torch.empty(10 * nb * M.IRREGULAR_DTYPE.itemsize, dtype=torch.uint8, device="cuda:0")
+    count = M.irregular_nodes_into(phi, win, band.contiguous(), recs)
+    del band
+    print(f"reinitialize_parallel window {win.n} records {count}", flush=True)
+    (band2,), (sweeps,) = M.reinitialize_parallel([(phi, win, recs, count)])
+    inside = sdf.abs() <= 6.0
+    planes = phi.view(nz, -1)
+    assert torch.equal(planes.amax(1), planes.amin(1))
+    vals = planes[:, 0]
+    assert (vals[inside] - sdf[inside]).abs().max().item() < 1e-12
+    assert torch.all(vals[~inside].abs() == 7.5)
+    assert band2.shape[0] == int(inside.sum()) * nx * ny
+    assert sweeps <= 10
