@@ -670,13 +670,20 @@ def formats_run(n=256, steps=20):
     F.irregular_node_rows(recs, count, surf, 3)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    e0.record()
     for _ in range(steps):
         F.irregular_node_rows(recs, count, surf, 3)
+    e1.record()
+    torch.cuda.synchronize()
     t_rows = (time.perf_counter() - t0) / steps
+    t_rows_dev = e0.elapsed_time(e1) * 1e-3 / steps
+    row_bytes = M.IRREGULAR_DTYPE.itemsize + M.SURFACE_DTYPE.itemsize + 23 * 8  # read 88 + 104, write 184
     return dict(real64=dict(n=n, ms_per_call=t * 1e3, values_per_s=n ** 3 / t, achieved_gbs=gbs,
                             roofline_frac=gbs * 1e9 / HBM_PEAK, bytes_per_value=16),
                 grid_write_ms=t_write * 1e3,
-                irregular_rows=dict(records=count, ms_per_call=t_rows * 1e3, records_per_s=count / t_rows))
+                irregular_rows=dict(records=count, ms_per_call=t_rows * 1e3, records_per_s=count / t_rows,
+                                    device_ms_per_call=t_rows_dev * 1e3, bytes_per_record=row_bytes,
+                                    achieved_gbs=count * row_bytes / t_rows_dev / 1e9))
 
 
 def _reinit_inputs(n, scale, center, radius, torch, D, M, dev="cuda:0"):

@@ -14,13 +14,15 @@
 //                          only (axis != -1).
 //
 // Both are byte rearrangements, bound by HBM: one thread per value (per row
-// field), 8-byte loads and stores.  The 2D rows skip records, so their
-// positions come from a scan of per-block counts.
+// field), 8-byte loads and stores; the rows' records and surface quantities
+// pass through LDS so that their loads stay contiguous.  The 2D rows skip
+// records, so their positions come from a scan of per-block counts.
 #include "dcl_host.hpp"
 
 namespace dcl {
 
 constexpr int FB = 256;
+constexpr int RB = 64;  // node-list records per block of the row kernels (one wave)
 constexpr unsigned REAL64_CLASSID = 1211214u;  // PETSc's VEC_FILE_CLASSID
 
 __device__ __forceinline__ unsigned long long be64(double v) {
@@ -40,66 +42,48 @@ __global__ __launch_bounds__(FB) void real64_pack_k(const T* __restrict__ src, i
   if (i < count) dst[1 + i] = be64((double)src[i]);
 }
 
-// the row of record r (IrregularNodes.c:145-172 / 178-219); extra: the IIM
-// fields the row carries that this library does not compute (3D: f1, f1_n,
-// f1_t, f1_nn, f1_tt, f1_nt, numNei; 2D: f1, fa1, f1_n, f1_nn), zero when NULL
+// where row field f of a record comes from (IrregularNodes.c:145-172 /
+// 178-219): the record's 8-byte word w (code w), its int32 word w (code 16 +
+// w: pos), the surface's 8-byte word w (code 32 + w), or the IIM field w the
+// row carries that this library does not compute (code 64 + w; 3D: f1, f1_n,
+// f1_t, f1_nn, f1_tt, f1_nt, numNei; 2D: f1, fa1, f1_n, f1_nn), zero when
+// extra is NULL.  Record words: pos at int32 0-2, X at doubles 5-7; surface
+// words: n 0-2, s 3-5, r 6-8, k 9.
 template <int DIM>
-__device__ __forceinline__ double row_field(const dcl_irregular_node& n, const dcl_surface& s,
-                                            const double* __restrict__ extra, int64_t r, int f) {
-  constexpr int NX = DIM == 3 ? 7 : 4;
-  auto ex = [&](int k) { return extra ? extra[r * NX + k] : 0.0; };
+__device__ __forceinline__ int row_source(int f) {
   if (DIM == 3) {
-    switch (f) {
-      case 0: return n.X.x;
-      case 1: return n.X.y;
-      case 2: return n.X.z;
-      case 3: return n.pos.x;
-      case 4: return n.pos.y;
-      case 5: return n.pos.z;
-      case 6: return s.n.x;
-      case 7: return s.n.y;
-      case 8: return s.n.z;
-      case 9: return s.k;
-      case 10: return ex(0);  // f1
-      case 11: return ex(1);  // f1_n
-      case 12: return ex(2);  // f1_t
-      case 13: return ex(3);  // f1_nn
-      case 14: return ex(4);  // f1_tt
-      case 15: return ex(5);  // f1_nt
-      case 16: return s.s.x;
-      case 17: return s.s.y;
-      case 18: return s.s.z;
-      case 19: return s.r.x;
-      case 20: return s.r.y;
-      case 21: return s.r.z;
-      default: return ex(6);  // numNei
-    }
+    if (f < 3) return 5 + f;                    // X
+    if (f < 6) return 16 + (f - 3);             // pos
+    if (f < 9) return 32 + (f - 6);             // n
+    if (f == 9) return 32 + 9;                  // k
+    if (f < 16) return 64 + (f - 10);           // f1 .. f1_nt
+    if (f < 22) return 32 + 3 + (f - 16);       // s, r
+    return 64 + 6;                              // numNei
   }
-  switch (f) {
-    case 0: return n.X.x;
-    case 1: return n.X.y;
-    case 2: return n.pos.x;
-    case 3: return n.pos.y;
-    case 4: return s.n.x;
-    case 5: return s.n.y;
-    case 6: return ex(0);  // f1
-    case 7: return ex(1);  // fa1
-    case 8: return s.k;
-    case 9: return ex(2);  // f1_n
-    default: return ex(3);  // f1_nn
-  }
+  if (f < 2) return 5 + f;                      // X
+  if (f < 4) return 16 + (f - 2);               // pos
+  if (f < 6) return 32 + (f - 4);               // n
+  if (f < 8) return 64 + (f - 6);               // f1, fa1
+  if (f == 8) return 32 + 9;                    // k
+  return 64 + 2 + (f - 9);                      // f1_n, f1_nn
 }
 
-// 2D: crossing records per block of FB records
-__global__ __launch_bounds__(FB) void rows_count_k(const dcl_irregular_node* __restrict__ nodes, int64_t count,
+constexpr int REC_W = sizeof(dcl_irregular_node) / 8;  // 11
+constexpr int SURF_W = sizeof(dcl_surface) / 8;        // 13
+static_assert(sizeof(dcl_irregular_node) == 88 && sizeof(dcl_surface) == 104, "record layouts");
+
+// 2D: crossing records per block of RB records (one wave)
+__global__ __launch_bounds__(RB) void rows_count_k(const dcl_irregular_node* __restrict__ nodes, int64_t count,
                                                    unsigned* __restrict__ counts) {
-  const int64_t r = (int64_t)blockIdx.x * FB + threadIdx.x;
-  const int c = __syncthreads_count(r < count && nodes[r].axis != -1);
-  if (threadIdx.x == 0) counts[blockIdx.x] = (unsigned)c;
+  const int64_t r = (int64_t)blockIdx.x * RB + threadIdx.x;
+  const unsigned long long bal = __ballot(r < count && nodes[r].axis != -1);
+  if (threadIdx.x == 0) counts[blockIdx.x] = (unsigned)__popcll(bal);
 }
 
-// one block per FB records: each kept record's row at its position, one
-// thread per field over the block's rows
+// one block of FB threads per RB records: the block's records and surface
+// quantities are staged in LDS with contiguous 8-byte loads, then each kept
+// record's row is written at its position, one thread per field over the
+// block's rows (the stores contiguous)
 template <int DIM>
 __global__ __launch_bounds__(FB) void rows_write_k(const dcl_irregular_node* __restrict__ nodes,
                                                    const dcl_surface* __restrict__ surf,
@@ -108,28 +92,47 @@ __global__ __launch_bounds__(FB) void rows_write_k(const dcl_irregular_node* __r
                                                    const unsigned long long* __restrict__ tile_sums,
                                                    unsigned long long* __restrict__ dst, int64_t cap) {
   constexpr int NF = DIM == 3 ? 23 : 11;
-  __shared__ int slot[FB];  // the block's kept records, in order
-  __shared__ unsigned wsum[FB / 64];
-  const int64_t r0 = (int64_t)blockIdx.x * FB;
-  const int64_t r = r0 + threadIdx.x;
-  const bool keep = r < count && (DIM == 3 || nodes[r].axis != -1);
-  const unsigned long long bal = __ballot(keep);
-  const unsigned lane_pre =
-      __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
-  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = (unsigned)__popcll(bal);
+  constexpr int NX = DIM == 3 ? 7 : 4;
+  __shared__ double lrec[RB * REC_W];
+  __shared__ double lsurf[RB * SURF_W];
+  __shared__ int slot[RB];  // the block's kept records, in order
+  __shared__ int kept_s;
+  const int64_t r0 = (int64_t)blockIdx.x * RB;
+  const int nrec = (int)(count - r0 < RB ? count - r0 : RB);
+  const double* grec = reinterpret_cast<const double*>(nodes + r0);
+  const double* gsurf = reinterpret_cast<const double*>(surf + r0);
+  for (int i = threadIdx.x; i < nrec * REC_W; i += FB) lrec[i] = grec[i];
+  for (int i = threadIdx.x; i < nrec * SURF_W; i += FB) lsurf[i] = gsurf[i];
   __syncthreads();
-  unsigned pre = 0;
-  for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) pre += wsum[w];
-  const int kept = (int)(wsum[0] + wsum[1] + wsum[2] + wsum[3]);
-  if (keep) slot[pre + lane_pre] = threadIdx.x;
+  if (threadIdx.x < RB) {  // wave 0: the kept records in order
+    const bool keep =
+        (int)threadIdx.x < nrec &&
+        (DIM == 3 || reinterpret_cast<const dcl_irregular_node*>(lrec)[threadIdx.x].axis != -1);
+    const unsigned long long bal = __ballot(keep);
+    const unsigned pre =
+        __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
+    if (keep) slot[pre] = threadIdx.x;
+    if (threadIdx.x == 0) kept_s = __popcll(bal);
+  }
   __syncthreads();
+  const int kept = kept_s;
   const int64_t base = DIM == 3 ? r0 : (int64_t)(offsets[blockIdx.x] + tile_sums[blockIdx.x / SCAN_TILE]);
   for (int q = threadIdx.x; q < kept * NF; q += FB) {
     const int k = q / NF, f = q - k * NF;
     const int64_t row = base + k;
     if (row >= cap) continue;
-    const int64_t rr = r0 + slot[k];
-    dst[row * NF + f] = be64(row_field<DIM>(nodes[rr], surf[rr], extra, rr, f));
+    const int j = slot[k];
+    const int src = row_source<DIM>(f);
+    double v;
+    if (src < 16)
+      v = lrec[j * REC_W + src];
+    else if (src < 32)
+      v = (double)reinterpret_cast<const int*>(lrec + j * REC_W)[src - 16];
+    else if (src < 64)
+      v = lsurf[j * SURF_W + (src - 32)];
+    else
+      v = extra ? extra[(r0 + j) * NX + (src - 64)] : 0.0;
+    dst[row * NF + f] = be64(v);
   }
 }
 
@@ -167,7 +170,7 @@ extern "C" int dcl_irregular_node_rows(const dcl_irregular_node* nodes, const dc
     *rows_out = 0;
     return 0;
   }
-  const int64_t nb = (count + FB - 1) / FB;
+  const int64_t nb = (count + RB - 1) / RB;
   unsigned long long* out = static_cast<unsigned long long*>(dst);
   if (dim == 3) {
     if (capacity > 0)
@@ -184,7 +187,7 @@ extern "C" int dcl_irregular_node_rows(const dcl_irregular_node* nodes, const dc
   unsigned long long* total = static_cast<unsigned long long*>(sc);
   unsigned long long* sums = total + 2;
   unsigned* counts = reinterpret_cast<unsigned*>(sums + nt);
-  hipLaunchKernelGGL(rows_count_k, dim3((unsigned)nb), dim3(FB), 0, s, nodes, count, counts);
+  hipLaunchKernelGGL(rows_count_k, dim3((unsigned)nb), dim3(RB), 0, s, nodes, count, counts);
   DCL_LAUNCHED("irregular_rows_count");
   const ScanArrays sa = {{counts, nullptr}, {sums, nullptr}, {total, nullptr}};
   DCL_TRY(scan_counts(sa, 1, nb, s));
