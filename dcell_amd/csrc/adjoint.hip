@@ -10,10 +10,11 @@
 //             + node weights x (-(c/2) (dV/dXm)^T ...) (RK2 first stage)
 // where a clamped coordinate contributes no derivative.  The transposed
 // gather of one block lands in a small box of phi0 around the block's nodes,
-// so it is accumulated in LDS first (one atomic per LDS word) and flushed to
-// HBM with one atomic per touched word; stencil entries that fall outside the
-// block's box go to HBM directly.  Outer two layers: phi = phi0 there, so
-// dL/dphi0 += lam_next.
+// so it is accumulated in LDS first and flushed to HBM with one atomic per
+// touched word; stencil entries that fall outside the block's box go to HBM
+// directly.  x-neighbour lanes whose stencils overlap sum their shared words
+// with DPP row shifts before the LDS atomic.  Outer two layers: phi = phi0
+// there, so dL/dphi0 += lam_next.
 #include "dcl_host.hpp"
 
 namespace dcl {
@@ -104,6 +105,38 @@ __device__ __forceinline__ void mac_scatter(T* __restrict__ gvel, const KVel& v,
   }
 }
 
+// DPP row shifts (VALU, within rows of 16 lanes): the value of the lane d
+// below / above in the same row, 0 where that lane is outside the row.
+// Every lane of the wave executes them.
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, true);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_t(float v) {
+  return __int_as_float(dpp_i32<CTRL>(__float_as_int(v)));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_t(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_i32<CTRL>((int)(unsigned)(unsigned long long)b);
+  const int hi = dpp_i32<CTRL>((int)(unsigned)((unsigned long long)b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+template <int CTRL>
+__device__ __forceinline__ int dpp_t(int v) {
+  return dpp_i32<CTRL>(v);
+}
+// row_shr:d (0x110 + d): lane l reads lane l - d; row_shl:d (0x100 + d): l + d
+template <typename V>
+__device__ __forceinline__ V lane_from_lower(V v, int d) {
+  return d == 1 ? dpp_t<0x111>(v) : d == 2 ? dpp_t<0x112>(v) : dpp_t<0x113>(v);
+}
+template <typename V>
+__device__ __forceinline__ V lane_from_upper(V v, int d) {
+  return d == 1 ? dpp_t<0x101>(v) : d == 2 ? dpp_t<0x102>(v) : dpp_t<0x103>(v);
+}
+
 // LDS box of lam for one 32x4x2 block of nodes plus the stencil reach R
 constexpr int ABX = 32, ABY = 4, ABZ = 2;
 constexpr int AR = 5;  // reach: |departure| <= AR - 2 cells lands in the box (cubic stencil -1..+2)
@@ -129,113 +162,153 @@ __global__ __launch_bounds__(256) void sl_adjoint_k(const T* __restrict__ phi0, 
     const T g = lam_next[id];
     if (g != T(0)) atomic_add(&lam[id], g);
   }
-  if (inner) {
-    const int id = k * Z + j * Y + i;
-    const T g = lam_next[id];
-    if (g != T(0)) {
-      const int X = P.w.px + i, Yg = P.w.py + j, Zg = P.w.pz + k;
-      const double Xd[3] = {(double)X, (double)Yg, (double)Zg};
-      T V1[3];
-      node_vel<T, 3>(vel, P.v, X, Yg, Zg, V1);
-      // the midpoint value now (the forward's arithmetic); its derivative
-      // state only after the stencil loop, to keep it out of the loop's
-      // registers (more waves in flight hide the dependent gathers)
-      double Xm[3], S[3];
-      if (BT == DCL_RK2) {
+  // The stencil loop runs with every lane of the wave (lanes without work
+  // carry g = 0 and no box key), so that x-neighbour lanes can combine their
+  // contributions to the same LDS word before the atomic.
+  T g = T(0);
+  if (inner) g = lam_next[k * Z + j * Y + i];
+  const bool act = inner && g != T(0);
+  const int NP = INTERP == DCL_CUBIC ? 4 : 2;
+  const int OFF = INTERP == DCL_CUBIC ? -1 : 0;
+  double Xm[3] = {0.0, 0.0, 0.0};
+  int b[3] = {0, 0, 0};
+  bool cs[3] = {false, false, false};
+  T w[3][4], dw[3][4];
+  int bx0 = 0, by0 = 0, bz0 = 0;
+  bool in_box = false;
+  if (act) {
+    const int X = P.w.px + i, Yg = P.w.py + j, Zg = P.w.pz + k;
+    const double Xd[3] = {(double)X, (double)Yg, (double)Zg};
+    T V1[3];
+    node_vel<T, 3>(vel, P.v, X, Yg, Zg, V1);
+    // the midpoint value now (the forward's arithmetic); its derivative
+    // state only after the stencil loop, to keep it out of the loop's
+    // registers (more waves in flight hide the dependent gathers)
+    double S[3];
+    if (BT == DCL_RK2) {
 #pragma unroll
-        for (int a = 0; a < 3; ++a) Xm[a] = Xd[a] - (double)V1[a] * P.ch[a];
-        T V2[3];
-        mac_vel<T, 3>(vel, P.v, Xm, V2);
+      for (int a = 0; a < 3; ++a) Xm[a] = Xd[a] - (double)V1[a] * P.ch[a];
+      T V2[3];
+      mac_vel<T, 3>(vel, P.v, Xm, V2);
 #pragma unroll
-        for (int a = 0; a < 3; ++a) S[a] = Xd[a] - (double)V2[a] * P.c[a];
-      } else {
+      for (int a = 0; a < 3; ++a) S[a] = Xd[a] - (double)V2[a] * P.c[a];
+    } else {
 #pragma unroll
-        for (int a = 0; a < 3; ++a) S[a] = Xd[a] - (double)V1[a] * P.c[a];
-      }
-      // split + weights
-      const int NP = INTERP == DCL_CUBIC ? 4 : 2;
-      const int OFF = INTERP == DCL_CUBIC ? -1 : 0;
-      const int pp[3] = {P.w.px, P.w.py, P.w.pz}, nn[3] = {nx, ny, nz};
-      int b[3];
-      bool cs[3];
-      T w[3][4], dw[3][4];
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        double f;
-        if (INTERP == DCL_CUBIC) {
-          const double lo = pp[a] + 1, hi = pp[a] + nn[a] - 2;
-          cs[a] = !(S[a] >= lo && S[a] <= hi);
-          b[a] = cub_split(S[a], pp[a], nn[a], f);
-          cr_weights<T>((T)f, w[a]);
-          cr_dweights<T>((T)f, dw[a]);
-        } else {
-          const double lo = pp[a], hi = pp[a] + nn[a] - 1;
-          cs[a] = !(S[a] >= lo && S[a] <= hi);
-          b[a] = lin_split(S[a], pp[a], nn[a], f);
-          w[a][0] = (T)(1.0 - f);
-          w[a][1] = (T)f;
-          dw[a][0] = T(-1);
-          dw[a][1] = T(1);
-        }
-      }
-      // transposed gather into the LDS box (or HBM outside it) + gradient gather
-      const int bx0 = b[0] + OFF - (ox - AR), by0 = b[1] + OFF - (oy - AR), bz0 = b[2] + OFF - (oz - AR);
-      const bool in_box = bx0 >= 0 && bx0 + NP <= LBX && by0 >= 0 && by0 + NP <= LBY && bz0 >= 0 &&
-                          bz0 + NP <= LBZ;
-      T G[3] = {T(0), T(0), T(0)};
-      const T* src = phi0 + (b[2] + OFF) * Z + (b[1] + OFF) * Y + (b[0] + OFF);
-      // one stencil plane at a time: bounds the loads in flight (registers)
-#pragma unroll 1
-      for (int kk = 0; kk < NP; ++kk)
-#pragma unroll
-        for (int jj = 0; jj < NP; ++jj) {
-          const T wyz = w[1][jj] * w[2][kk];
-          const T* r = src + kk * Z + jj * Y;
-          T rx = T(0), rd = T(0);
-#pragma unroll
-          for (int ii = 0; ii < NP; ++ii) {
-            const T f = r[ii];
-            rx += w[0][ii] * f;
-            rd += dw[0][ii] * f;
-            const T add = g * w[0][ii] * wyz;
-            if (in_box)
-              atomicAdd(&box[((bz0 + kk) * LBY + (by0 + jj)) * LBX + (bx0 + ii)], add);
-            else
-              atomic_add(&lam[(b[2] + OFF + kk) * Z + (b[1] + OFF + jj) * Y + (b[0] + OFF + ii)], add);
-          }
-          G[0] += rd * wyz;
-          G[1] += rx * dw[1][jj] * w[2][kk];
-          G[2] += rx * w[1][jj] * dw[2][kk];
-        }
-      T H[3];
-#pragma unroll
-      for (int a = 0; a < 3; ++a) H[a] = cs[a] ? T(0) : (T)(-P.c[a]) * g * G[a];
-      T H1[3];
-      if (BT == DCL_RK2) {
-        MacState<T> m2;
-        bool cm[3] = {false, false, false};
-        mac_state<T>(vel, P.v, Xm, m2, cm);
-        mac_scatter<T>(gvel, P.v, m2, H);
-#pragma unroll
-        for (int bb = 0; bb < 3; ++bb) {
-          const T K = H[0] * m2.dV[0][bb] + H[1] * m2.dV[1][bb] + H[2] * m2.dV[2][bb];
-          H1[bb] = (T)(-P.ch[bb]) * K;
-        }
-      } else {
-#pragma unroll
-        for (int a = 0; a < 3; ++a) H1[a] = H[a];
-      }
-      // node velocity: V_a = (f(X) + f(X + e_a)) / 2
-      const int sx = P.v.dof, sy = P.w.nx * P.v.dof, sz = P.w.ny * sy;
-      T* gb = gvel + (k * P.w.ny + j) * sy + i * sx + P.v.udof;
-      const T h0 = T(0.5) * H1[0], h1 = T(0.5) * H1[1], h2 = T(0.5) * H1[2];
-      atomic_add(gb, h0);
-      atomic_add(gb + sx, h0);
-      atomic_add(gb + 1, h1);
-      atomic_add(gb + sy + 1, h1);
-      atomic_add(gb + 2, h2);
-      atomic_add(gb + sz + 2, h2);
+      for (int a = 0; a < 3; ++a) S[a] = Xd[a] - (double)V1[a] * P.c[a];
     }
+    // split + weights
+    const int pp[3] = {P.w.px, P.w.py, P.w.pz}, nn[3] = {nx, ny, nz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      double f;
+      if (INTERP == DCL_CUBIC) {
+        const double lo = pp[a] + 1, hi = pp[a] + nn[a] - 2;
+        cs[a] = !(S[a] >= lo && S[a] <= hi);
+        b[a] = cub_split(S[a], pp[a], nn[a], f);
+        cr_weights<T>((T)f, w[a]);
+        cr_dweights<T>((T)f, dw[a]);
+      } else {
+        const double lo = pp[a], hi = pp[a] + nn[a] - 1;
+        cs[a] = !(S[a] >= lo && S[a] <= hi);
+        b[a] = lin_split(S[a], pp[a], nn[a], f);
+        w[a][0] = (T)(1.0 - f);
+        w[a][1] = (T)f;
+        dw[a][0] = T(-1);
+        dw[a][1] = T(1);
+      }
+    }
+    bx0 = b[0] + OFF - (ox - AR);
+    by0 = b[1] + OFF - (oy - AR);
+    bz0 = b[2] + OFF - (oz - AR);
+    in_box = bx0 >= 0 && bx0 + NP <= LBX && by0 >= 0 && by0 + NP <= LBY && bz0 >= 0 && bz0 + NP <= LBZ;
+  } else {
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) w[a][q] = dw[a][q] = T(0);
+  }
+  // the box word of the lane's stencil corner, plus one (0: no box word).
+  // Lanes d = 1 .. NP-1 apart in a 16-lane row whose keys differ by d write
+  // the same word with their entries ii and 0, in every stencil row: the
+  // lower one leaves it to the upper one, which adds both.
+  const int key = in_box ? (bz0 * LBY + by0) * LBX + bx0 + 1 : 0;
+  bool lo_m[4] = {false, false, false, false}, up_m[4] = {false, false, false, false};
+#pragma unroll
+  for (int d = 1; d < NP; ++d) {
+    const int kl = lane_from_lower(key, d), ku = lane_from_upper(key, d);
+    lo_m[d] = key != 0 && kl != 0 && kl + d == key;
+    up_m[d] = key != 0 && ku != 0 && key + d == ku;
+  }
+  // transposed gather into the LDS box (or HBM outside it) + gradient gather
+  T G[3] = {T(0), T(0), T(0)};
+  const T* src = phi0 + (b[2] + OFF) * Z + (b[1] + OFF) * Y + (b[0] + OFF);
+  // one stencil plane at a time: bounds the loads in flight (registers)
+#pragma unroll 1
+  for (int kk = 0; kk < NP; ++kk)
+#pragma unroll
+    for (int jj = 0; jj < NP; ++jj) {
+      const T wyz = w[1][jj] * w[2][kk];
+      T add[4];
+#pragma unroll
+      for (int ii = 0; ii < NP; ++ii) add[ii] = g * w[0][ii] * wyz;
+      T word0 = add[0];
+#pragma unroll
+      for (int d = 1; d < NP; ++d) {
+        const T o = lane_from_lower(add[d], d);
+        if (lo_m[d]) word0 += o;
+      }
+      if (act) {
+        const T* r = src + kk * Z + jj * Y;
+        T rx = T(0), rd = T(0);
+#pragma unroll
+        for (int ii = 0; ii < NP; ++ii) {
+          const T f = r[ii];
+          rx += w[0][ii] * f;
+          rd += dw[0][ii] * f;
+          if (in_box) {
+            T* bw = &box[((bz0 + kk) * LBY + (by0 + jj)) * LBX + (bx0 + ii)];
+            if (ii == 0)
+              atomicAdd(bw, word0);
+            else if (!up_m[ii])
+              atomicAdd(bw, add[ii]);
+          } else {
+            atomic_add(&lam[(b[2] + OFF + kk) * Z + (b[1] + OFF + jj) * Y + (b[0] + OFF + ii)], add[ii]);
+          }
+        }
+        G[0] += rd * wyz;
+        G[1] += rx * dw[1][jj] * w[2][kk];
+        G[2] += rx * w[1][jj] * dw[2][kk];
+      }
+    }
+  if (act) {
+    T H[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) H[a] = cs[a] ? T(0) : (T)(-P.c[a]) * g * G[a];
+    T H1[3];
+    if (BT == DCL_RK2) {
+      MacState<T> m2;
+      bool cm[3] = {false, false, false};
+      mac_state<T>(vel, P.v, Xm, m2, cm);
+      mac_scatter<T>(gvel, P.v, m2, H);
+#pragma unroll
+      for (int bb = 0; bb < 3; ++bb) {
+        const T K = H[0] * m2.dV[0][bb] + H[1] * m2.dV[1][bb] + H[2] * m2.dV[2][bb];
+        H1[bb] = (T)(-P.ch[bb]) * K;
+      }
+    } else {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) H1[a] = H[a];
+    }
+    // node velocity: V_a = (f(X) + f(X + e_a)) / 2
+    const int sx = P.v.dof, sy = P.w.nx * P.v.dof, sz = P.w.ny * sy;
+    T* gb = gvel + (k * P.w.ny + j) * sy + i * sx + P.v.udof;
+    const T h0 = T(0.5) * H1[0], h1 = T(0.5) * H1[1], h2 = T(0.5) * H1[2];
+    atomic_add(gb, h0);
+    atomic_add(gb + sx, h0);
+    atomic_add(gb + 1, h1);
+    atomic_add(gb + sy + 1, h1);
+    atomic_add(gb + 2, h2);
+    atomic_add(gb + sz + 2, h2);
   }
   __syncthreads();
   // flush the box: one HBM atomic per nonzero word inside the window
