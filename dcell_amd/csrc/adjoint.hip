@@ -13,8 +13,9 @@
 // so it is accumulated in LDS first and flushed to HBM with one atomic per
 // touched word; stencil entries that fall outside the block's box go to HBM
 // directly.  x-neighbour lanes whose stencils overlap sum their shared words
-// with DPP row shifts before the LDS atomic.  Outer two layers: phi = phi0
-// there, so dL/dphi0 += lam_next.
+// with DPP row shifts before the LDS atomic, and likewise the shared corners
+// of their MAC velocity scatter before the HBM atomic.  Outer two layers:
+// phi = phi0 there, so dL/dphi0 += lam_next.
 #include "dcl_host.hpp"
 
 namespace dcl {
@@ -84,27 +85,6 @@ __device__ __forceinline__ void mac_state(const T* __restrict__ vel, const KVel&
   }
 }
 
-template <typename T>
-__device__ __forceinline__ void mac_scatter(T* __restrict__ gvel, const KVel& v, const MacState<T>& m,
-                                            const T H[3]) {
-  const int sx = v.dof, sy = v.w.nx * v.dof, sz = v.w.ny * sy;
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    if (H[c] == T(0)) continue;
-    T* b = gvel + m.base[c];
-    const T x1 = m.f[c][0], x0 = T(1) - x1, y1 = m.f[c][1], y0 = T(1) - y1, z1 = m.f[c][2], z0 = T(1) - z1;
-    const T h = H[c];
-    atomic_add(b, h * x0 * y0 * z0);
-    atomic_add(b + sx, h * x1 * y0 * z0);
-    atomic_add(b + sy, h * x0 * y1 * z0);
-    atomic_add(b + sy + sx, h * x1 * y1 * z0);
-    atomic_add(b + sz, h * x0 * y0 * z1);
-    atomic_add(b + sz + sx, h * x1 * y0 * z1);
-    atomic_add(b + sz + sy, h * x0 * y1 * z1);
-    atomic_add(b + sz + sy + sx, h * x1 * y1 * z1);
-  }
-}
-
 // DPP row shifts (VALU, within rows of 16 lanes): the value of the lane d
 // below / above in the same row, 0 where that lane is outside the row.
 // Every lane of the wave executes them.
@@ -135,6 +115,48 @@ __device__ __forceinline__ V lane_from_lower(V v, int d) {
 template <typename V>
 __device__ __forceinline__ V lane_from_upper(V v, int d) {
   return d == 1 ? dpp_t<0x101>(v) : d == 2 ? dpp_t<0x102>(v) : dpp_t<0x103>(v);
+}
+
+// mac_scatter with x-neighbour lanes combined: where the lane d = 1 below in
+// the 16-lane row has its corner cube of component c one x cell lower, its
+// x1 corners are this lane's x0 corners, which this lane adds; that lane
+// skips them.  Every lane of the wave calls it (act false: nothing to add).
+template <typename T>
+__device__ __forceinline__ void mac_scatter_lanes(T* __restrict__ gvel, const KVel& v, const MacState<T>& m,
+                                                  const T H[3], bool act) {
+  const int sx = v.dof, sy = v.w.nx * v.dof, sz = v.w.ny * sy;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const bool on = act && H[c] != T(0);
+    const int key = on ? m.base[c] + 1 : 0;
+    const int kl = lane_from_lower(key, 1), ku = lane_from_upper(key, 1);
+    const bool lo = on && kl != 0 && kl + sx == key, up = on && ku != 0 && key + sx == ku;
+    T a[4] = {T(0), T(0), T(0), T(0)}, b[4] = {T(0), T(0), T(0), T(0)};  // x0 / x1 corners: y0z0 y1z0 y0z1 y1z1
+    if (on) {
+      const T x1 = m.f[c][0], x0 = T(1) - x1, y1 = m.f[c][1], y0 = T(1) - y1, z1 = m.f[c][2], z0 = T(1) - z1;
+      const T h = H[c];
+      a[0] = h * x0 * y0 * z0;
+      b[0] = h * x1 * y0 * z0;
+      a[1] = h * x0 * y1 * z0;
+      b[1] = h * x1 * y1 * z0;
+      a[2] = h * x0 * y0 * z1;
+      b[2] = h * x1 * y0 * z1;
+      a[3] = h * x0 * y1 * z1;
+      b[3] = h * x1 * y1 * z1;
+    }
+    T bl[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) bl[q] = lane_from_lower(b[q], 1);
+    if (on) {
+      T* p = gvel + m.base[c];
+      const int off[4] = {0, sy, sz, sz + sy};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        atomic_add(p + off[q], lo ? a[q] + bl[q] : a[q]);
+        if (!up) atomic_add(p + off[q] + sx, b[q]);
+      }
+    }
+  }
 }
 
 // LDS box of lam for one 32x4x2 block of nodes plus the stencil reach R
@@ -280,16 +302,14 @@ __global__ __launch_bounds__(256) void sl_adjoint_k(const T* __restrict__ phi0, 
         G[2] += rx * w[1][jj] * dw[2][kk];
       }
     }
+  T H[3] = {T(0), T(0), T(0)}, H1[3] = {T(0), T(0), T(0)};
+  MacState<T> m2;
   if (act) {
-    T H[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) H[a] = cs[a] ? T(0) : (T)(-P.c[a]) * g * G[a];
-    T H1[3];
     if (BT == DCL_RK2) {
-      MacState<T> m2;
       bool cm[3] = {false, false, false};
       mac_state<T>(vel, P.v, Xm, m2, cm);
-      mac_scatter<T>(gvel, P.v, m2, H);
 #pragma unroll
       for (int bb = 0; bb < 3; ++bb) {
         const T K = H[0] * m2.dV[0][bb] + H[1] * m2.dV[1][bb] + H[2] * m2.dV[2][bb];
@@ -299,12 +319,21 @@ __global__ __launch_bounds__(256) void sl_adjoint_k(const T* __restrict__ phi0, 
 #pragma unroll
       for (int a = 0; a < 3; ++a) H1[a] = H[a];
     }
-    // node velocity: V_a = (f(X) + f(X + e_a)) / 2
-    const int sx = P.v.dof, sy = P.w.nx * P.v.dof, sz = P.w.ny * sy;
-    T* gb = gvel + (k * P.w.ny + j) * sy + i * sx + P.v.udof;
-    const T h0 = T(0.5) * H1[0], h1 = T(0.5) * H1[1], h2 = T(0.5) * H1[2];
-    atomic_add(gb, h0);
-    atomic_add(gb + sx, h0);
+  }
+  if (BT == DCL_RK2) mac_scatter_lanes<T>(gvel, P.v, m2, H, act);
+  // node velocity: V_a = (f(X) + f(X + e_a)) / 2.  The x face at X + e_x is
+  // the next x lane's face at its node: that lane adds both.
+  const int sx = P.v.dof, sy = P.w.nx * P.v.dof, sz = P.w.ny * sy;
+  const int gofs = (k * P.w.ny + j) * sy + i * sx + P.v.udof;
+  const T h0 = T(0.5) * H1[0], h1 = T(0.5) * H1[1], h2 = T(0.5) * H1[2];
+  const int nkey = act ? gofs + 1 : 0;
+  const int nkl = lane_from_lower(nkey, 1), nku = lane_from_upper(nkey, 1);
+  const bool nlo = act && nkl != 0 && nkl + sx == nkey, nup = act && nku != 0 && nkey + sx == nku;
+  const T h0l = lane_from_lower(h0, 1);
+  if (act) {
+    T* gb = gvel + gofs;
+    atomic_add(gb, nlo ? h0 + h0l : h0);
+    if (!nup) atomic_add(gb + sx, h0);
     atomic_add(gb + 1, h1);
     atomic_add(gb + sy + 1, h1);
     atomic_add(gb + 2, h2);
