@@ -506,35 +506,50 @@ __device__ __forceinline__ long long rs_prefix(const unsigned* counts, const uns
   return (long long)counts[b] + (long long)sums[b / SCAN_TILE];
 }
 
+// rs_final_k runs one raster block (RS_B nodes) per wave, nodes e * 64 +
+// lane for e = 0 .. RS_B / 64 - 1, its loads issued before its stores, and
+// counts each side with per-row ballots.
+constexpr int RS_WPB = RS_B / 64;  // raster blocks per workgroup
+constexpr int RS_ROWS = RS_B / 64; // 64-node rows per raster block
+
 // the result into the caller's buffer where the last sweep wrote the other
 // one, then the band counts: each side per raster block (the block past the
 // last level set counts nothing, so the scans' last entry is the total)
 __global__ __launch_bounds__(RS_B) void rs_final_k(const RsCell* __restrict__ cells, int ncells, long long nq,
-                                                   const RsState* __restrict__ st, unsigned* __restrict__ c0,
-                                                   unsigned* __restrict__ c1) {
-  const long long g = blockIdx.x;
-  bool p0 = false, p1 = false;
-  if (g < nq) {
+                                                   long long nq1, const RsState* __restrict__ st,
+                                                   unsigned* __restrict__ c0, unsigned* __restrict__ c1) {
+  const long long g = (long long)blockIdx.x * RS_WPB + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  unsigned n0 = 0, n1 = 0;
+  if (g < nq) {  // uniform over the wave
     const int ci = rs_find<RS_KEY_RASTER>(cells, ncells, g);
     const RsCell& c = cells[ci];
-    const int64_t n = (int64_t)c.w.nx * c.w.ny * c.w.nz;
-    const int64_t i = (g - c.qb0) * RS_B + threadIdx.x;
-    if (i < n) {
-      double v = c.buf[0][i];
-      if (st[ci].k & 1) {
-        v = c.buf[1][i];
-        c.buf[0][i] = v;
-      }
-      const int m = c.mask[i];
-      p0 = m == 0 && fabs(v) < c.inf;
-      p1 = m == 1 && fabs(v) < c.inf;
+    const int64_t n = rs_nodes(c);
+    const bool odd = st[ci].k & 1;
+    const int64_t i0 = (g - c.qb0) * RS_B + lane;
+    const double* src = c.buf[odd ? 1 : 0];
+    // every row's loads first: the copy's stores could alias later rows'
+    // loads and would otherwise serialise them
+    double v[RS_ROWS];
+    int m[RS_ROWS];
+#pragma unroll
+    for (int e = 0; e < RS_ROWS; ++e) {
+      const int64_t i = i0 + e * 64;
+      v[e] = i < n ? src[i] : 0.0;
+      m[e] = i < n ? (int)c.mask[i] : 2;
+    }
+#pragma unroll
+    for (int e = 0; e < RS_ROWS; ++e) {
+      const int64_t i = i0 + e * 64;
+      if (odd && i < n) c.buf[0][i] = v[e];
+      const bool fin = fabs(v[e]) < c.inf;
+      n0 += (unsigned)__popcll(__ballot(m[e] == 0 && fin));
+      n1 += (unsigned)__popcll(__ballot(m[e] == 1 && fin));
     }
   }
-  const int n0 = __syncthreads_count(p0);
-  const int n1 = __syncthreads_count(p1);
-  if (threadIdx.x == 0) {
-    c0[g] = (unsigned)n0;
-    c1[g] = (unsigned)n1;
+  if (lane == 0 && g < nq1) {
+    c0[g] = n0;
+    c1[g] = n1;
   }
 }
 
@@ -833,7 +848,8 @@ extern "C" int dcl_reinitialize_parallel(dcl_reinit_cell* cells, int ncells, int
   }
   // the result, then the band
   if (!rc_) {
-    hipLaunchKernelGGL(rs_final_k, dim3((unsigned)nq1), dim3(RS_B), 0, s, dcells, ncells, nq, dstate, c0, c1);
+    hipLaunchKernelGGL(rs_final_k, dim3((unsigned)((nq1 + RS_WPB - 1) / RS_WPB)), dim3(RS_B), 0, s, dcells,
+                       ncells, nq, nq1, dstate, c0, c1);
     hipLaunchKernelGGL(rs_bc_count_k, dim3((unsigned)nr1), dim3(RS_B), 0, s, dcells, ncells, nr, rc);
     if (hipGetLastError() != hipSuccess) rc_ = set_error(DCL_ERR_LIB, "reinit band launch failed");
   }
