@@ -636,7 +636,8 @@ def formats_run(n=256, steps=20):
     n^3 FP64 grid (16 B per value: 8 read, 8 written) with HIP events on the
     current stream against the HBM peak, the whole grid_write (pack, one copy
     to the host, the file write), and the node-list rows of the n^3 sphere's
-    6-cell band (23 doubles per record)."""
+    6-cell band (23 doubles per record; the rows call synchronises to return
+    its row count, so its event time holds that wait as well as the kernel)."""
     import tempfile
     import torch
     import dcell_amd as D
