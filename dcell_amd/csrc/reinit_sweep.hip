@@ -506,18 +506,21 @@ __device__ __forceinline__ long long rs_prefix(const unsigned* counts, const uns
   return (long long)counts[b] + (long long)sums[b / SCAN_TILE];
 }
 
-// rs_final_k runs one raster block (RS_B nodes) per wave, nodes e * 64 +
-// lane for e = 0 .. RS_B / 64 - 1, its loads issued before its stores, and
-// counts each side with per-row ballots.
+// rs_final_k and rs_side_write_k run one raster block (RS_B nodes) per wave,
+// nodes e * 64 + lane for e = 0 .. RS_B / 64 - 1 (raster order: e, then
+// lane); rs_final_k issues its loads before its stores and keeps each row's
+// ballot of each side, from which rs_side_write_k ranks the band nodes.
 constexpr int RS_WPB = RS_B / 64;  // raster blocks per workgroup
 constexpr int RS_ROWS = RS_B / 64; // 64-node rows per raster block
 
 // the result into the caller's buffer where the last sweep wrote the other
 // one, then the band counts: each side per raster block (the block past the
-// last level set counts nothing, so the scans' last entry is the total)
+// last level set counts nothing, so the scans' last entry is the total), and
+// each row's ballot of each side for rs_side_write_k
 __global__ __launch_bounds__(RS_B) void rs_final_k(const RsCell* __restrict__ cells, int ncells, long long nq,
                                                    long long nq1, const RsState* __restrict__ st,
-                                                   unsigned* __restrict__ c0, unsigned* __restrict__ c1) {
+                                                   unsigned* __restrict__ c0, unsigned* __restrict__ c1,
+                                                   unsigned long long* __restrict__ sides) {
   const long long g = (long long)blockIdx.x * RS_WPB + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   unsigned n0 = 0, n1 = 0;
@@ -538,14 +541,19 @@ __global__ __launch_bounds__(RS_B) void rs_final_k(const RsCell* __restrict__ ce
       v[e] = i < n ? src[i] : 0.0;
       m[e] = i < n ? (int)c.mask[i] : 2;
     }
+    unsigned long long mine = 0;  // lane 2e / 2e + 1: row e's - / + side ballot
 #pragma unroll
     for (int e = 0; e < RS_ROWS; ++e) {
       const int64_t i = i0 + e * 64;
       if (odd && i < n) c.buf[0][i] = v[e];
       const bool fin = fabs(v[e]) < c.inf;
-      n0 += (unsigned)__popcll(__ballot(m[e] == 0 && fin));
-      n1 += (unsigned)__popcll(__ballot(m[e] == 1 && fin));
+      const unsigned long long b0 = __ballot(m[e] == 0 && fin), b1 = __ballot(m[e] == 1 && fin);
+      n0 += (unsigned)__popcll(b0);
+      n1 += (unsigned)__popcll(b1);
+      if (lane == 2 * e) mine = b0;
+      if (lane == 2 * e + 1) mine = b1;
     }
+    if (lane < 2 * RS_ROWS) sides[g * (2 * RS_ROWS) + lane] = mine;
   }
   if (lane == 0 && g < nq1) {
     c0[g] = n0;
@@ -591,45 +599,64 @@ __global__ __launch_bounds__(RS_B) void rs_bc_write_k(const RsCell* __restrict__
   if (pos < c.band_cap) c.band[pos] = q;
 }
 
+// the band nodes of raster block g, ranked by rs_final_k's ballots
+__device__ __forceinline__ void rs_side_block(const RsCell* __restrict__ cells, int ncells, long long g, int lane,
+                                              const unsigned* __restrict__ rc,
+                                              const unsigned long long* __restrict__ rs,
+                                              const unsigned* __restrict__ c0,
+                                              const unsigned long long* __restrict__ s0,
+                                              const unsigned* __restrict__ c1,
+                                              const unsigned long long* __restrict__ s1,
+                                              const unsigned long long* __restrict__ sides) {
+  const RsCell& c = cells[rs_find<RS_KEY_RASTER>(cells, ncells, g)];
+  const long long p0 = rs_prefix(c0, s0, g), p1 = rs_prefix(c1, s1, g);
+  const long long nbc = rs_prefix(rc, rs, c.rb1) - rs_prefix(rc, rs, c.rb0);
+  const long long n0 = rs_prefix(c0, s0, c.qb1) - rs_prefix(c0, s0, c.qb0);
+  const long long base0 = nbc + p0 - rs_prefix(c0, s0, c.qb0);
+  const long long base1 = nbc + n0 + p1 - rs_prefix(c1, s1, c.qb0);
+  const int64_t plane = (int64_t)c.w.nx * c.w.ny;
+  const int64_t i0 = (g - c.qb0) * RS_B + lane;
+  const unsigned long long lt = (1ull << lane) - 1ull;  // the lanes below this one
+  const unsigned long long* sb = sides + g * (2 * RS_ROWS);
+  long long r0 = 0, r1 = 0;  // band nodes of each side in the rows before
+#pragma unroll
+  for (int e = 0; e < RS_ROWS; ++e) {
+    const unsigned long long b0 = sb[2 * e], b1 = sb[2 * e + 1];
+    const bool k0 = (b0 >> lane) & 1ull, k1 = (b1 >> lane) & 1ull;
+    if (k0 || k1) {
+      const int64_t pos = k0 ? base0 + r0 + __popcll(b0 & lt) : base1 + r1 + __popcll(b1 & lt);
+      if (pos < c.band_cap) {
+        const int64_t i = i0 + e * 64;
+        const int z = c.d3 ? (int)(i / plane) : 0, rr = (int)(i - (int64_t)z * plane);
+        const int y = rr / c.w.nx, x = rr - y * c.w.nx;
+        dcl_icoor q;
+        q.x = c.w.px + x;
+        q.y = c.w.py + y;
+        q.z = c.w.pz + z;
+        c.band[pos] = q;
+      }
+    }
+    r0 += __popcll(b0);
+    r1 += __popcll(b1);
+  }
+}
+
+// the two sides of the band, from the counts' scans and rs_final_k's
+// ballots: one raster block per wave; a block without band nodes reads its
+// two counts and stops
 __global__ __launch_bounds__(RS_B) void rs_side_write_k(const RsCell* __restrict__ cells, int ncells, long long nq,
                                                         const unsigned* __restrict__ rc,
                                                         const unsigned long long* __restrict__ rs,
                                                         const unsigned* __restrict__ c0,
                                                         const unsigned long long* __restrict__ s0,
                                                         const unsigned* __restrict__ c1,
-                                                        const unsigned long long* __restrict__ s1) {
-  __shared__ unsigned sh[RS_B / 64];
-  const long long g = blockIdx.x;
-  if (g >= nq) return;
-  const RsCell& c = cells[rs_find<RS_KEY_RASTER>(cells, ncells, g)];
-  const int64_t n = (int64_t)c.w.nx * c.w.ny * c.w.nz;
-  const int64_t i = (g - c.qb0) * RS_B + threadIdx.x;
-  bool in = false;
-  int m = 2;
-  if (i < n) {
-    in = fabs(c.buf[0][i]) < c.inf;
-    m = c.mask[i];
-  }
-  const unsigned r0 = rs_rank(in && m == 0, sh);
-  const unsigned r1 = rs_rank(in && m == 1, sh);
-  if (!in || m == 2) return;
-  const long long nbc = rs_prefix(rc, rs, c.rb1) - rs_prefix(rc, rs, c.rb0);
-  int64_t pos;
-  if (m == 0) {
-    pos = nbc + rs_prefix(c0, s0, g) - rs_prefix(c0, s0, c.qb0) + r0;
-  } else {
-    const long long n0 = rs_prefix(c0, s0, c.qb1) - rs_prefix(c0, s0, c.qb0);
-    pos = nbc + n0 + rs_prefix(c1, s1, g) - rs_prefix(c1, s1, c.qb0) + r1;
-  }
-  if (pos >= c.band_cap) return;
-  const int64_t plane = (int64_t)c.w.nx * c.w.ny;
-  const int z = c.d3 ? (int)(i / plane) : 0, rr = (int)(i - (int64_t)z * plane);
-  const int y = rr / c.w.nx, x = rr - y * c.w.nx;
-  dcl_icoor q;
-  q.x = c.w.px + x;
-  q.y = c.w.py + y;
-  q.z = c.w.pz + z;
-  c.band[pos] = q;
+                                                        const unsigned long long* __restrict__ s1,
+                                                        const unsigned long long* __restrict__ sides) {
+  const long long g = (long long)blockIdx.x * RS_WPB + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (g >= nq) return;  // uniform over the wave; no block barrier below
+  if (rs_prefix(c0, s0, g + 1) == rs_prefix(c0, s0, g) && rs_prefix(c1, s1, g + 1) == rs_prefix(c1, s1, g)) return;
+  rs_side_block(cells, ncells, g, lane, rc, rs, c0, s0, c1, s1, sides);
 }
 
 // per level set: boundary nodes, - side, + side
@@ -703,7 +730,8 @@ extern "C" int dcl_reinitialize_parallel(dcl_reinit_cell* cells, int ncells, int
                        2 * rs_al(sizeof(unsigned long long) * (size_t)scan_tiles(nq1)) +
                        rs_al(sizeof(unsigned) * (size_t)nr1) +
                        rs_al(sizeof(unsigned long long) * (size_t)scan_tiles(nr1)) +
-                       rs_al(sizeof(unsigned long long) * 3) + cell_bytes;
+                       rs_al(sizeof(unsigned long long) * 3) +
+                       rs_al(sizeof(unsigned long long) * 2 * RS_ROWS * (size_t)nq) + cell_bytes;
   void* sc = nullptr;
   DCL_TRY(scratch(bytes, &sc));
   char* base = static_cast<char*>(sc);
@@ -727,6 +755,8 @@ extern "C" int dcl_reinitialize_parallel(dcl_reinit_cell* cells, int ncells, int
   unsigned* rc = reinterpret_cast<unsigned*>(take(sizeof(unsigned) * (size_t)nr1));
   unsigned long long* rsum = reinterpret_cast<unsigned long long*>(take(sizeof(unsigned long long) * scan_tiles(nr1)));
   unsigned long long* grand = reinterpret_cast<unsigned long long*>(take(sizeof(unsigned long long) * 3));
+  unsigned long long* sides =
+      reinterpret_cast<unsigned long long*>(take(sizeof(unsigned long long) * 2 * RS_ROWS * (size_t)nq));
 
   RsCell* host = static_cast<RsCell*>(malloc(sizeof(RsCell) * ncells));
   RsState* hstate = static_cast<RsState*>(malloc(sizeof(RsState) * ncells));
@@ -849,7 +879,7 @@ extern "C" int dcl_reinitialize_parallel(dcl_reinit_cell* cells, int ncells, int
   // the result, then the band
   if (!rc_) {
     hipLaunchKernelGGL(rs_final_k, dim3((unsigned)((nq1 + RS_WPB - 1) / RS_WPB)), dim3(RS_B), 0, s, dcells,
-                       ncells, nq, nq1, dstate, c0, c1);
+                       ncells, nq, nq1, dstate, c0, c1, sides);
     hipLaunchKernelGGL(rs_bc_count_k, dim3((unsigned)nr1), dim3(RS_B), 0, s, dcells, ncells, nr, rc);
     if (hipGetLastError() != hipSuccess) rc_ = set_error(DCL_ERR_LIB, "reinit band launch failed");
   }
@@ -863,8 +893,8 @@ extern "C" int dcl_reinitialize_parallel(dcl_reinit_cell* cells, int ncells, int
   }
   if (!rc_) {
     if (nr > 0) hipLaunchKernelGGL(rs_bc_write_k, dim3((unsigned)nr), dim3(RS_B), 0, s, dcells, ncells, nr, rc, rsum);
-    hipLaunchKernelGGL(rs_side_write_k, dim3((unsigned)nq), dim3(RS_B), 0, s, dcells, ncells, nq, rc, rsum, c0, s0,
-                       c1, s1);
+    hipLaunchKernelGGL(rs_side_write_k, dim3((unsigned)((nq + RS_WPB - 1) / RS_WPB)), dim3(RS_B), 0, s, dcells,
+                       ncells, nq, rc, rsum, c0, s0, c1, s1, sides);
     hipLaunchKernelGGL(rs_totals_k, dim3(grid1d(ncells, RS_B)), dim3(RS_B), 0, s, dcells, ncells, rc, rsum, c0, s0,
                        c1, s1, totals);
     if (hipGetLastError() != hipSuccess) rc_ = set_error(DCL_ERR_LIB, "reinit band launch failed");

@@ -268,8 +268,9 @@ int dcl_reinitialize(dcl_reinit_cell *cells, int ncells, int64_t heap_capacity, 
  * sweeps allowed per level set (0: 256; not settling is an error); sweeps
  * (host, ncells entries, may be NULL) gets the sweeps each level set took.
  * Synchronous; band_count is set on return.  Uses device scratch of about
- * 19 bytes per window node (a second buffer, masks, marks, the node list),
- * kept by the calling thread for later calls. */
+ * 19.3 bytes per window node (a second buffer, masks, marks, the node
+ * list, the band's row ballots), kept by the calling thread for later
+ * calls. */
 int dcl_reinitialize_parallel(dcl_reinit_cell *cells, int ncells, int max_sweeps, int *sweeps,
                               void *stream);
 
