@@ -119,3 +119,28 @@ def test_irregular_rows_2d_many_tiles(dev):
     got = F.irregular_node_rows(d_recs, n, d_surf, 2, d_extra).cpu().numpy().tobytes()
     assert len(got) == (recs["axis"] != -1).sum() * 11 * 8
     assert got == O.irregular_rows(recs, surf, extra, 2)
+
+
+@pytest.mark.parametrize("dim", [3, 2])
+def test_irregular_rows_capacity(dev, dim):
+    """Through the C ABI with a capacity below the row count, ending inside a
+    block of records: rows_out is the whole count, the first capacity rows are
+    the oracle's, and the bytes past capacity are untouched."""
+    import ctypes as C
+    from dcell_amd import _lib as L
+    from dcell_amd.levelset import _p, stream_handle
+    recs, surf, count, h_recs, h_surf = _records_and_surface(dim)
+    nf = F.ROW_FIELDS[dim]
+    ref = np.frombuffer(O.irregular_rows(h_recs, h_surf, None, dim), dtype=np.uint8)
+    total = ref.size // (nf * 8)
+    cap = total // 2 + 37
+    assert cap % 64 not in (0, 63) and cap < total
+    out = torch.full(((cap + 5) * nf * 8,), 0xA5, dtype=torch.uint8, device="cuda:0")
+    rows = C.c_int64()
+    L.call("dcl_irregular_node_rows", _p(recs), _p(surf), None, int(count), dim, _p(out), cap, C.byref(rows),
+           stream_handle())
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    assert rows.value == total
+    assert got[:cap * nf * 8].tobytes() == ref[:cap * nf * 8].tobytes()
+    assert np.all(got[cap * nf * 8:] == 0xA5)
