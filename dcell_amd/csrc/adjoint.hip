@@ -251,8 +251,9 @@ __global__ __launch_bounds__(256) void sl_adjoint_k(const T* __restrict__ phi0, 
   }
   // the box word of the lane's stencil corner, plus one (0: no box word).
   // Lanes d = 1 .. NP-1 apart in a 16-lane row whose keys differ by d write
-  // the same word with their entries ii and 0, in every stencil row: the
-  // lower one leaves it to the upper one, which adds both.
+  // the same word in every stencil row, the lower lane with its entry d and
+  // the upper lane with its entry 0: the lower one leaves it to the upper
+  // one, which adds both.
   const int key = in_box ? (bz0 * LBY + by0) * LBX + bx0 + 1 : 0;
   bool lo_m[4] = {false, false, false, false}, up_m[4] = {false, false, false, false};
 #pragma unroll
