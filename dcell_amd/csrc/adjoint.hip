@@ -159,8 +159,9 @@ __device__ __forceinline__ void mac_scatter_lanes(T* __restrict__ gvel, const KV
   }
 }
 
-// LDS box of lam for one 32x4x2 block of nodes plus the stencil reach R
-constexpr int ABX = 32, ABY = 4, ABZ = 2;
+// LDS box of lam for one 16x4x4 block of nodes plus the stencil reach R (a
+// block's 16-lane DPP rows are its x rows)
+constexpr int ABX = 16, ABY = 4, ABZ = 4;
 constexpr int AR = 5;  // reach: |departure| <= AR - 2 cells lands in the box (cubic stencil -1..+2)
 constexpr int LBX = ABX + 2 * AR, LBY = ABY + 2 * AR, LBZ = ABZ + 2 * AR;
 
