@@ -162,6 +162,7 @@ __device__ __forceinline__ void mac_scatter_lanes(T* __restrict__ gvel, const KV
 // LDS box of lam for one 16x4x4 block of nodes plus the stencil reach R (a
 // block's 16-lane DPP rows are its x rows)
 constexpr int ABX = 16, ABY = 4, ABZ = 4;
+static_assert(ABX * ABY == 64, "a wave is one z plane of the block: lanes ABX apart are y neighbours");
 constexpr int AR = 5;  // reach: |departure| <= AR - 2 cells lands in the box (cubic stencil -1..+2)
 constexpr int LBX = ABX + 2 * AR, LBY = ABY + 2 * AR, LBZ = ABZ + 2 * AR;
 
@@ -324,7 +325,8 @@ __global__ __launch_bounds__(256) void sl_adjoint_k(const T* __restrict__ phi0, 
   }
   if (BT == DCL_RK2) mac_scatter_lanes<T>(gvel, P.v, m2, H, act);
   // node velocity: V_a = (f(X) + f(X + e_a)) / 2.  The x face at X + e_x is
-  // the next x lane's face at its node: that lane adds both.
+  // the next x lane's face at its node, and the y face at X + e_y that of the
+  // lane ABX above (the node at y + 1 in the same wave): that lane adds both.
   const int sx = P.v.dof, sy = P.w.nx * P.v.dof, sz = P.w.ny * sy;
   const int gofs = (k * P.w.ny + j) * sy + i * sx + P.v.udof;
   const T h0 = T(0.5) * H1[0], h1 = T(0.5) * H1[1], h2 = T(0.5) * H1[2];
@@ -332,12 +334,17 @@ __global__ __launch_bounds__(256) void sl_adjoint_k(const T* __restrict__ phi0, 
   const int nkl = lane_from_lower(nkey, 1), nku = lane_from_upper(nkey, 1);
   const bool nlo = act && nkl != 0 && nkl + sx == nkey, nup = act && nku != 0 && nkey + sx == nku;
   const T h0l = lane_from_lower(h0, 1);
+  const int lane = tid & 63;  // the linear thread index: x + ABX (y + ABY z)
+  const int ykl = __shfl_up(nkey, ABX), yku = __shfl_down(nkey, ABX);
+  const bool ylo = act && lane >= ABX && ykl != 0 && ykl + sy == nkey;
+  const bool yup = act && lane < 64 - ABX && yku != 0 && nkey + sy == yku;
+  const T h1l = __shfl_up(h1, ABX);
   if (act) {
     T* gb = gvel + gofs;
     atomic_add(gb, nlo ? h0 + h0l : h0);
     if (!nup) atomic_add(gb + sx, h0);
-    atomic_add(gb + 1, h1);
-    atomic_add(gb + sy + 1, h1);
+    atomic_add(gb + 1, ylo ? h1 + h1l : h1);
+    if (!yup) atomic_add(gb + sy + 1, h1);
     atomic_add(gb + 2, h2);
     atomic_add(gb + sz + 2, h2);
   }
