@@ -209,6 +209,34 @@ def read_traffic(n):
     return kv["kernel"], float(kv["fetch_bytes_per_launch"]) + float(kv["write_bytes_per_launch"])
 
 
+def host_cpu():
+    """lscpu's model name of the host, with its family / model / stepping and
+    socket / core counts (the model name alone can be a bare vendor string on
+    virtual machines)."""
+    import subprocess
+    keys = ("Model name", "CPU family", "Model", "Stepping", "Socket(s)", "Core(s) per socket",
+            "Thread(s) per core", "CPU(s)")
+    info = {}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in txt.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys and k.strip() not in info:
+                info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    if "Model name" not in info:
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith("model name"):
+                        info["Model name"] = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+    return info
+
+
 def cpu_baseline(args):
     """The oracle (a C restatement of the reference loops) on the host: the same
     workload (RK2 + tricubic SL, then the 19-point curvature, FP64, dense over
@@ -277,15 +305,7 @@ def cpu_baseline(args):
         t0 = time.perf_counter()
         list(ex.map(one, jobs))
         t_bn = time.perf_counter() - t0
-    model = "unknown"
-    try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
+    model = host_cpu()
     return dict(value=out[1], unit="grid-point updates/s", cores=1, kind="port",
                 sample=f"median of {args.cpu_steps} timed full steps (after 1 warm-up step) of the {n}^3 "
                        "workload (RK2+tricubic SL + 19-point curvature, FP64, dense) by the oracle, 1 thread",
@@ -352,13 +372,54 @@ def batch_run(ncells, seed, steps, warmup, interp="cubic", bt="rk2", autotune=Tr
     return out
 
 
-def pls_run(n, steps, per_cell=32, seed=3):
+def pls_oracle(phi_np, vel_np, P_np, n, dts):
+    """The FP64 oracle's run of the precision form on the host with every host
+    thread: per step the particle RK2, the SL step, the escaped-particle
+    correction and the radius update, in LevelSetAdvectPLS's order
+    (LevelSetMethod/ParticleLS.c:299-329).  Returns phi after the last step."""
+    import oracle as O
+    h = 1.0 / (n - 1)
+    w = O.Win((0, 0, 0), (n, n, n), (h, h, h))
+    O.set_threads(len(os.sched_getaffinity(0)))
+    phi, P = phi_np, P_np
+    for dt in dts:
+        P = O.pls_advect(P, vel_np, w, w.d, dt)
+        phi = O.sl_advect(phi, vel_np, w, dt, O.CUBIC, O.RK2)
+        phi = O.pls_correct(P, phi, w, None, 0, 3)
+        P = O.pls_adjust_radii(P, phi, w)
+    O.set_threads(1)
+    return phi
+
+
+def pls_errors(phi, ref, n):
+    """Errors of phi against the reference run ref (both float64 numpy): max
+    and RMS |dphi| over the grid, max |dphi| where |ref| < 6, and the relative
+    errors of the interface area and the delta-weighted mean curvature (the
+    oracle's metric functions on both, eps = 1.5 cells)."""
+    import oracle as O
+    h = 1.0 / (n - 1)
+    w = O.Win((0, 0, 0), (n, n, n), (h, h, h))
+    d = phi - ref
+    near = np.abs(ref) < 6
+    O.set_threads(len(os.sched_getaffinity(0)))
+    a, _, km, _ = O.interface_metrics(phi, w, O.curvature(phi, w))
+    ar, _, kmr, _ = O.interface_metrics(ref, w, O.curvature(ref, w))
+    O.set_threads(1)
+    return dict(max_abs_phi=float(np.abs(d).max()), rms_phi=float(np.sqrt(np.mean(d * d))),
+                max_abs_phi_band=float(np.abs(d[near]).max()),
+                area=float(a), area_rel=float(abs(a - ar) / ar),
+                mean_curvature=float(km), mean_curvature_rel=float(abs(km - kmr) / abs(kmr)))
+
+
+def pls_run(n, steps, per_cell=32, seed=3, oracle=True):
     """The precision form: n^3 deformation workload with phi and velocity in
     FP32 (departure coordinates FP64), a particle level set of `per_cell`
-    particles per cell in |phi| < 3 (seeded), per step: particle RK2, SL +
-    curvature, escaped-particle correction, radius update.  The same run in
-    FP64 gives the error figures (max / RMS phi difference, interface area,
-    mean curvature)."""
+    particles per cell in |phi| < 3 (seeded), per step in LevelSetAdvectPLS's
+    order (LevelSetMethod/ParticleLS.c:299-329): particle RK2, SL, escaped-
+    particle correction, then the curvature of the corrected phi and the
+    radius update.  The same run in FP64 on the GPU, and the FP64 oracle's run
+    on the host, give the error figures (max / RMS phi difference, interface
+    area, mean curvature)."""
     import torch
     import dcell_amd as D
     from dcell_amd import configs
@@ -372,14 +433,15 @@ def pls_run(n, steps, per_cell=32, seed=3):
     P_np = configs.seed_particles(phi_np, per_cell, 3.0, seed)
     dts = [configs.deformation_dt(n, s) for s in range(steps)]
     out = {}
+    finals = {}
     for dtype in (torch.float32, torch.float64):
         phi = torch.from_numpy(phi_np).to(dev, dtype)
         vel = torch.from_numpy(vel_np).to(dev, dtype)
         P = torch.from_numpy(P_np).to(dev, dtype)
         other, kappa = torch.empty_like(phi), torch.empty_like(phi)
         ws = PL.CorrectionWorkspace(win.nodes, 8 * len(P), dtype)
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
-        t_pa = t_sl = t_pc = t_rad = 0.0
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
+        t_pa = t_sl = t_pc = t_cv = t_rad = 0.0
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for s in range(steps):
@@ -387,32 +449,34 @@ def pls_run(n, steps, per_cell=32, seed=3):
             PL.advect_particles(P, vel, win, win.d, dts[s])
             ev[1].record()
             D.advect_sl(phi, vel, win, dts[s], "cubic", "rk2", out=other)
-            D.curvature(other, win, out=kappa)
             ev[2].record()
             PL.correct(P, other, win, ws)
             ev[3].record()
-            PL.adjust_radii(P, other, win)
+            D.curvature(other, win, out=kappa)
             ev[4].record()
+            PL.adjust_radii(P, other, win)
+            ev[5].record()
             phi, other = other, phi
             torch.cuda.synchronize()
             t_pa += ev[0].elapsed_time(ev[1])
             t_sl += ev[1].elapsed_time(ev[2])
             t_pc += ev[2].elapsed_time(ev[3])
-            t_rad += ev[3].elapsed_time(ev[4])
+            t_cv += ev[3].elapsed_time(ev[4])
+            t_rad += ev[4].elapsed_time(ev[5])
         wall = time.perf_counter() - t0
-        kappa = D.curvature(phi, win, out=kappa)
         area, kint, kmean, cnt = M.interface_metrics(phi, win, kappa)
         key = "fp32" if dtype == torch.float32 else "fp64"
         interior = (n - 4) ** 3
-        out[key] = dict(ms_per_step=wall / steps * 1e3, sl_curv_ms=t_sl / steps, pls_advect_ms=t_pa / steps,
+        out[key] = dict(ms_per_step=wall / steps * 1e3, sl_curv_ms=(t_sl + t_cv) / steps, sl_ms=t_sl / steps,
+                        curv_ms=t_cv / steps, pls_advect_ms=t_pa / steps,
                         pls_correct_ms=t_pc / steps, pls_radii_ms=t_rad / steps,
-                        sl_curv_updates_per_s=interior / (t_sl / steps * 1e-3),
+                        sl_curv_updates_per_s=interior / ((t_sl + t_cv) / steps * 1e-3),
                         particle_updates_per_s=len(P) / ((t_pa + t_pc) / steps * 1e-3),
                         area=area, mean_curvature=kmean)
-        out[key + "_phi"] = phi
+        finals[key] = phi
         ws.close()
         del vel, P, other, kappa
-    a, b = out.pop("fp32_phi"), out.pop("fp64_phi")
+    a, b = finals["fp32"], finals["fp64"]
     mx, ss, mb, cnt = M.error_norms(a, b)
     near = (b.abs() < 6)
     out["errors_vs_fp64"] = dict(
@@ -422,8 +486,17 @@ def pls_run(n, steps, per_cell=32, seed=3):
         mean_curvature_rel=abs(out["fp32"]["mean_curvature"] - out["fp64"]["mean_curvature"])
         / abs(out["fp64"]["mean_curvature"]))
     out.update(n=n, steps=steps, particles=int(len(P_np)), per_cell=per_cell)
-    del a, b
+    host = {k: v.double().cpu().numpy() for k, v in finals.items()}
+    del a, b, finals, near
     torch.cuda.empty_cache()
+    if oracle:
+        t0 = time.perf_counter()
+        ref = pls_oracle(phi_np, vel_np, P_np, n, dts)
+        t_orc = time.perf_counter() - t0
+        out["errors_vs_oracle"] = {k: pls_errors(v, ref, n) for k, v in host.items()}
+        out["errors_vs_oracle"].update(
+            oracle_s=t_orc, oracle_threads=len(os.sched_getaffinity(0)),
+            sample=f"the FP64 oracle's full {steps}-step run of the same inputs on the host")
     return out
 
 

@@ -21,9 +21,20 @@ def sl_adjoint(phi0, vel, lam_next, win: GridWindow, dt: float, interp="cubic", 
                lam=None, gvel=None):
     """lam += dL/dphi0 and gvel += dL/dvel for one step with lam_next = dL/dphi."""
     _dev(phi0, "phi0")
+    precision_of(phi0)
+    _dev(vel, "vel", phi0.dtype)
     _dev(lam_next, "lam_next", phi0.dtype)
+    for t, name in ((phi0, "phi0"), (lam_next, "lam_next")):
+        if t.numel() != win.nodes:
+            raise ValueError(f"{name} has {t.numel()} elements, window needs {win.nodes}")
     lam = torch.zeros_like(phi0) if lam is None else lam
     gvel = torch.zeros_like(vel) if gvel is None else gvel
+    _dev(lam, "lam", phi0.dtype)
+    _dev(gvel, "gvel", phi0.dtype)
+    if lam.numel() != win.nodes:
+        raise ValueError(f"lam has {lam.numel()} elements, window needs {win.nodes}")
+    if gvel.shape != vel.shape:
+        raise ValueError(f"gvel is shaped {tuple(gvel.shape)}, vel {tuple(vel.shape)}")
     V = velocity(vel, win)
     L.call("dcl_sl_adjoint", _p(phi0), C.byref(win.c()), C.byref(V), _p(lam_next), _p(lam), _p(gvel),
            float(dt), precision_of(phi0), INTERP[interp], BACKTRACE[backtrace], stream_handle())

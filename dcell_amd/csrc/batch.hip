@@ -296,6 +296,12 @@ extern "C" int dcl_batch_create(int32_t ncells, const dcl_window* wins, int prec
   for (int c = 0; c < ncells; ++c) {
     DCL_TRY(check_window(&wins[c], "cell", 5, 3));
     if (wins[c].n.z < 2) return set_error(DCL_ERR_SUP, "cell %d is 2D; batches are 3D", c);
+    // the kernels address a cell's velocity through a buffer descriptor whose
+    // byte size is 32-bit
+    const int64_t vbytes = 3 * win_nodes(wins[c]) * (precision == DCL_F64 ? 8 : 4);
+    if (vbytes >= ((int64_t)1 << 32))
+      return set_error(DCL_ERR_ARG_SIZ, "cell %d: its velocity takes %lld bytes; the limit is 2^32 - 1", c,
+                       (long long)vbytes);
   }
   dcl_batch b = new (std::nothrow) dcl_batch_s();
   if (!b) return set_error(DCL_ERR_MEM, "host allocation failed");

@@ -1,13 +1,16 @@
 /*
  * reinit.c -- LevelSetReinitialize (LevelSetMethod/FastMarching.c:10-41) over
  * the C ABI: LevelSetResize first, as the reference does, then phi and the
- * irregular nodes go to the GPU, dcl_reinitialize runs the second-order fast
- * marching in the reference's heap order, and phi (the whole window: +-PHI_INF
- * outside the new band) and the new band come back into the level set.
- * With DCELL_GPU_REINIT=parallel in the environment the parallel sweeps
- * (dcl_reinitialize_parallel) run instead: the same band, phi equal to the
- * fast marching's at most nodes and within its discretisation error
- * elsewhere, and the band's two sides in window order.
+ * irregular nodes go to the GPU, dcl_reinitialize_parallel recomputes phi in
+ * parallel sweeps that replay the fast marching's pushes in value order (the
+ * same band, phi equal to the fast marching's at most nodes and within its
+ * discretisation error elsewhere, the band's two sides in window order), and
+ * phi (the whole window: +-PHI_INF outside the new band) and the new band come
+ * back into the level set.  With DCELL_GPU_REINIT=exact in the environment
+ * dcl_reinitialize runs the second-order fast marching itself in the
+ * reference's heap order instead (bit for bit the reference, band order
+ * included): a sequential chain of dependent steps, slower on the GPU than
+ * the reference's on one host core, kept as the parity check.
  */
 #include <stdlib.h>
 #include <string.h>
@@ -55,10 +58,10 @@ PetscErrorCode LevelSetReinitialize(LevelSet ls) {
   cell.band = (dcl_icoor *)dband;
   cell.band_capacity = nodes_w;
   const char *mode = getenv("DCELL_GPU_REINIT");
-  if (mode && strcmp(mode, "parallel") == 0) {
-    DL_CHECK(where, dcl_reinitialize_parallel(&cell, 1, 0, NULL, NULL));
-  } else {
+  if (mode && strcmp(mode, "exact") == 0) {
     DL_CHECK(where, dcl_reinitialize(&cell, 1, 0, NULL));
+  } else {
+    DL_CHECK(where, dcl_reinitialize_parallel(&cell, 1, 0, NULL, NULL));
   }
   DL_CHECK(where, dcl_memcpy_dtoh(ls->phi->v1, dphi, sizeof(double) * (size_t)nodes_w, NULL));
   ierr = ArraySetSize(ls->band, (int)cell.band_count); CHKERRQ(ierr);

@@ -14,9 +14,25 @@ from .levelset import GridWindow, _band, _dev, _p, precision_of, stream_handle, 
 REF_AZ = L.DCL_PLS_REF_AZ
 
 
+def _particles(particles, dtype=None):
+    """Particles must be a contiguous device tensor shaped (n, 4) of the given dtype."""
+    _dev(particles, "particles", dtype)
+    if particles.dim() != 2 or particles.shape[1] != 4:
+        raise ValueError(f"particles must be shaped (n, 4), got {tuple(particles.shape)}")
+    return particles
+
+
+def _phi(phi, win: GridWindow, dtype):
+    _dev(phi, "phi", dtype)
+    if phi.numel() != win.nodes:
+        raise ValueError(f"phi has {phi.numel()} elements, window needs {win.nodes}")
+    return phi
+
+
 def advect_particles(particles, vel, vel_win: GridWindow, dh, dt: float, dim: int = 3, udof=0):
     """In-place RK2 midpoint advection (ParticleLS_AdvectParticles3D/2D)."""
-    _dev(particles, "particles")
+    _particles(particles)
+    precision_of(particles)
     if particles.dtype != vel.dtype:
         raise TypeError("particles and vel must share a dtype")
     V = velocity(vel, vel_win, udof)
@@ -55,8 +71,8 @@ def correct(particles, phi, win: GridWindow, ws: CorrectionWorkspace, band=None,
     merged band values when band_values=True."""
     _dev(phi, "phi")
     prec = precision_of(phi)
-    if particles.dtype != phi.dtype:
-        raise TypeError("particles and phi must share a dtype")
+    _phi(phi, win, phi.dtype)
+    _particles(particles, phi.dtype)
     b, n = _band(band)
     if b is not None and n == 0:
         b = torch.zeros((1, 3), dtype=torch.int32, device=phi.device)  # non-NULL, length 0
@@ -70,7 +86,8 @@ def correct(particles, phi, win: GridWindow, ws: CorrectionWorkspace, band=None,
 
 def adjust_radii(particles, phi, win: GridWindow, rmin=0.1, rmax=0.5):
     """ParticleLS_AdjustRadii: r = s clamp(s phi(X), rmin, rmax), in place."""
-    _dev(particles, "particles")
+    _particles(particles)
+    _phi(phi, win, particles.dtype)
     L.call("dcl_pls_adjust_radii", _p(particles), particles.shape[0], _p(phi), C.byref(win.c()),
            float(rmin), float(rmax), win.dim, precision_of(phi), stream_handle())
     return particles

@@ -467,47 +467,85 @@ static inline int in_win(const orc_win *w, int x, int y, int z, int dim) {
 }
 
 /* ParticleLS_ErrorCorrection3D / 2D (ParticleLS3D.c:40-116, ParticleLS2D.c:40-115).
- * Serial over particles, like the reference (the max/min scatter is order-free). */
+ * The reference copies phi into phi+ and phi- (ParticleLS3D.c:60-61), takes
+ * max / min of each escaped particle's corner values into them (:86-93) and
+ * merges phi = |phi+| <= |phi-| ? phi+ : phi- (:104-113).  At a node no
+ * escaped particle touches phi+ = phi- = phi, so the merge leaves it as it is;
+ * this restatement therefore keeps phi+ and phi- only at touched nodes: the
+ * escape test of every particle runs in parallel (it reads only phi), the
+ * corner values of the escaped ones are listed serially in particle order,
+ * sorted by node, folded with the same max / min (order-free), and merged.
+ * The result equals the dense form's at every node. */
+typedef struct { int64_t id; double v; int pos; } orc_pls_event;
+
+static int ev_cmp(const void *a, const void *b) {
+  const int64_t x = ((const orc_pls_event *)a)->id, y = ((const orc_pls_event *)b)->id;
+  return x < y ? -1 : x > y;
+}
+
 int orc_pls_correct(const double *P, int64_t count, double *phi,
                     const orc_icoor *band, int64_t band_len, const orc_win *w,
                     int flags, int dim) {
-  int64_t size = (int64_t)w->nx * w->ny * (dim == 3 ? w->nz : 1);
-  double *pos = malloc(sizeof(double) * size), *neg = malloc(sizeof(double) * size);
-  if (!pos || !neg) { free(pos); free(neg); return 55; }
-  memcpy(pos, phi, sizeof(double) * size);
-  memcpy(neg, phi, sizeof(double) * size);
+  unsigned char *esc = malloc(count > 0 ? (size_t)count : 1);
+  if (!esc) return 55;
+#pragma omp parallel for schedule(static) num_threads(g_threads)
   for (int64_t m = 0; m < count; ++m) {
     const double *p = P + 4 * m;
     double dist = dim == 3 ? orc_gridinterp3d(phi, w, p) : orc_gridinterp2d(phi, w, p);
     double sign = psign(p[3]);
-    if (sign * dist < 0 && fabs(dist) > fabs(p[3])) {
-      int Ax = (int)floor(p[0]);
-      int Ay = (int)floor(p[1]);
-      int Az = dim == 3 ? (int)floor((flags & ORC_PLS_REF_AZ) ? p[1] : p[2]) : w->pz;
-      for (int k = 0; k < (dim == 3 ? 2 : 1); ++k)
-        for (int j = 0; j < 2; ++j)
-          for (int i = 0; i < 2; ++i) {
-            int ax = Ax + i, ay = Ay + j, az = Az + k;
-            if (!in_win(w, ax, ay, az, dim)) continue;
-            double r2 = (p[0] - ax) * (p[0] - ax) + (p[1] - ay) * (p[1] - ay);
-            if (dim == 3) r2 += (p[2] - az) * (p[2] - az);
-            double phi_p = sign * (sign * p[3] - sqrt(r2));
-            int64_t id = sidx(w, ax, ay, az);
-            if (sign > 0) pos[id] = pmax(phi_p, pos[id]);
-            else neg[id] = pmin(phi_p, neg[id]);
-          }
+    esc[m] = sign * dist < 0 && fabs(dist) > fabs(p[3]);
+  }
+  int64_t nesc = 0;
+  for (int64_t m = 0; m < count; ++m) nesc += esc[m];
+  orc_pls_event *ev = malloc(sizeof(orc_pls_event) * (size_t)(8 * nesc + 1));
+  if (!ev) { free(esc); return 55; }
+  int64_t ne = 0;
+  for (int64_t m = 0; m < count; ++m) {
+    if (!esc[m]) continue;
+    const double *p = P + 4 * m;
+    double sign = psign(p[3]);
+    int Ax = (int)floor(p[0]);
+    int Ay = (int)floor(p[1]);
+    int Az = dim == 3 ? (int)floor((flags & ORC_PLS_REF_AZ) ? p[1] : p[2]) : w->pz;
+    for (int k = 0; k < (dim == 3 ? 2 : 1); ++k)
+      for (int j = 0; j < 2; ++j)
+        for (int i = 0; i < 2; ++i) {
+          int ax = Ax + i, ay = Ay + j, az = Az + k;
+          if (!in_win(w, ax, ay, az, dim)) continue;
+          double r2 = (p[0] - ax) * (p[0] - ax) + (p[1] - ay) * (p[1] - ay);
+          if (dim == 3) r2 += (p[2] - az) * (p[2] - az);
+          ev[ne].id = sidx(w, ax, ay, az);
+          ev[ne].v = sign * (sign * p[3] - sqrt(r2));
+          ev[ne].pos = sign > 0;
+          ++ne;
+        }
+  }
+  free(esc);
+  qsort(ev, (size_t)ne, sizeof(orc_pls_event), ev_cmp);
+  /* fold each touched node's events: ev[u] keeps the node and its merged value */
+  int64_t nu = 0;
+  for (int64_t a = 0; a < ne;) {
+    int64_t id = ev[a].id;
+    double pos = phi[id], neg = phi[id];
+    for (; a < ne && ev[a].id == id; ++a) {
+      if (ev[a].pos) pos = pmax(ev[a].v, pos);
+      else neg = pmin(ev[a].v, neg);
     }
+    ev[nu].id = id;
+    ev[nu].v = fabs(pos) <= fabs(neg) ? pos : neg;
+    ++nu;
   }
   if (band) {
     for (int64_t b = 0; b < band_len; ++b) {
-      int64_t id = sidx(w, band[b].x, band[b].y, band[b].z);
-      phi[id] = fabs(pos[id]) <= fabs(neg[id]) ? pos[id] : neg[id];
+      orc_pls_event key;
+      key.id = sidx(w, band[b].x, band[b].y, band[b].z);
+      const orc_pls_event *f = bsearch(&key, ev, (size_t)nu, sizeof(orc_pls_event), ev_cmp);
+      if (f) phi[key.id] = f->v;
     }
   } else {
-    for (int64_t id = 0; id < size; ++id) phi[id] = fabs(pos[id]) <= fabs(neg[id]) ? pos[id] : neg[id];
+    for (int64_t u = 0; u < nu; ++u) phi[ev[u].id] = ev[u].v;
   }
-  free(pos);
-  free(neg);
+  free(ev);
   return 0;
 }
 

@@ -114,10 +114,11 @@ def test_advect_and_reinit_3d_offset_velocity_window(dev, scheme):
         if cfl > 0.25:  # threshold crossed: counters reset and phi reinitialised (Advect.c:42-48)
             cfl = 0.0
             reinits += 1
-            rphi, rband = O.reinitialize(ref, w, recs, ls.contents.bandWidth, ls.contents.PHI_INF)
-            # the advected phi agrees with the oracle's to 1e-12, so the fast
-            # marching may order a near-tie differently: compare loosely here,
-            # then carry on from the drop-in's own result
+            # the drop-in's default reinitialisation: the parallel sweeps
+            rphi, rband, _ = O.reinitialize_sweep(ref, w, recs, ls.contents.bandWidth, ls.contents.PHI_INF)
+            # the advected phi agrees with the oracle's to 1e-12, so the sweeps
+            # may order a near-tie differently: compare loosely here, then
+            # carry on from the drop-in's own result
             got_band = DI.get_array(ls.contents.band, np.int32, 3)
             assert {tuple(b) for b in got_band} == {tuple(b) for b in rband}
             assert np.max(np.abs(phi - rphi)) <= 1e-9
@@ -366,18 +367,19 @@ def test_iim_update_surface_quantities(dev, dim):
 
 
 
-@pytest.mark.parametrize("mode", ["exact", "parallel"])
+@pytest.mark.parametrize("mode", ["exact", "parallel", "default"])
 @pytest.mark.parametrize("dim", [3, 2])
 def test_levelset_reinitialize(dev, dim, mode, monkeypatch):
     """LevelSetReinitialize (FastMarching.c:10-41) after
-    LevelSetUpdateIrregularNodeList: phi over the whole window and the new
-    band equal the oracle's bit for bit, band order included; with
-    DCELL_GPU_REINIT=parallel, equal to the oracle's restatement of the
-    parallel sweeps."""
-    if mode == "parallel":
-        monkeypatch.setenv("DCELL_GPU_REINIT", "parallel")
-    else:
+    LevelSetUpdateIrregularNodeList.  By default (no DCELL_GPU_REINIT) and
+    with DCELL_GPU_REINIT=parallel: the parallel sweeps, phi over the whole
+    window and the new band equal to the oracle's restatement of the sweeps
+    bit for bit; with DCELL_GPU_REINIT=exact: equal to the oracle's fast
+    marching bit for bit, band order included."""
+    if mode == "default":
         monkeypatch.delenv("DCELL_GPU_REINIT", raising=False)
+    else:
+        monkeypatch.setenv("DCELL_GPU_REINIT", mode)
     L = DI.lib()
     if dim == 3:
         p, n = (-3, 4, 1), (36, 34, 30)
@@ -397,7 +399,7 @@ def test_levelset_reinitialize(dev, dim, mode, monkeypatch):
     DI.set_array(ls.contents.band, band, 12)
     DI.check(L.LevelSetUpdateIrregularNodeList(ls), "LevelSetUpdateIrregularNodeList")
     recs = O.irregular_nodes(phi, w, band)
-    if mode == "parallel":
+    if mode != "exact":
         ref_phi, ref_band, _ = O.reinitialize_sweep(phi, w, recs, ls.contents.bandWidth, ls.contents.PHI_INF)
     else:
         ref_phi, ref_band = O.reinitialize(phi, w, recs, ls.contents.bandWidth, ls.contents.PHI_INF)
