@@ -322,31 +322,37 @@ def cpu_baseline(args):
                                          "and the 64-cell batch, cells one after another and one per thread"))
 
 
-def batch_run(ncells, seed, steps, warmup, interp="cubic", bt="rk2", autotune=True):
-    """The batching form: ncells cells (edges log-uniform in [32, 128]), FP32,
-    each a perturbed sphere in random Fourier-mode velocity with its own dt;
-    one step = one hipGraph launch (SL + curvature over the packed tile list)."""
+def batch_run(ncells, seed, steps, warmup, interp="cubic", bt="rk2", autotune=True, dim=3, lo=32, hi=128,
+              dtype=None):
+    """The batching form: ncells cells (edges log-uniform in [lo, hi], seeded),
+    FP32 by default, each a perturbed sphere (2D: circle) in random Fourier-mode
+    velocity with its own dt; one step = one hipGraph launch (SL + curvature
+    over the packed tile list).  Reports the autotuned configuration, with
+    the ring z-chunk chosen for each cell-size class."""
     import torch
     import dcell_amd as D
     from dcell_amd import configs
-    from dcell_amd.batch import Batch
+    from dcell_amd.batch import ZCHUNKS, Batch
 
-    edges = configs.batch_edges(ncells, seed)
+    dtype = torch.float32 if dtype is None else dtype
+    edges = configs.batch_edges(ncells, seed, lo, hi)
     g = configs.philox(seed)
-    wins = [D.GridWindow.unit_box(int(n)) for n in edges]
-    b = Batch(wins, torch.float32, interp, bt)
-    phi0, phi1, kap, vel = b.new_field(), b.new_field(), b.new_field(), b.new_field(3)
+    wins = [D.GridWindow.unit_box(int(n), dim) for n in edges]
+    b = Batch(wins, dtype, interp, bt)
+    phi0, phi1, kap, vel = b.new_field(), b.new_field(), b.new_field(), b.new_field(dim)
     dts = []
+    fields = configs.cell_fields if dim == 3 else configs.cell_fields_2d
     for c, n in enumerate(edges):
-        phi, v, dt = configs.cell_fields(int(n), g, "cuda:0", torch.float32)
+        phi, v, dt = fields(int(n), g, "cuda:0", dtype)
         b.cell(phi0, c).copy_(phi)
-        b.cell(vel, c, 3).copy_(v)
+        b.cell(vel, c, dim).copy_(v)
         dts.append(dt)
     b.set_dt(dts)
     tune = None
     if autotune:
-        best, table = b.autotune(phi0, phi1, kap, vel, reps=3)
-        tune = dict(best_ms=best, table_ms=table.round(4).tolist(), chosen=b.config())
+        best, table, ztable = b.autotune(phi0, phi1, kap, vel, reps=3)
+        tune = dict(best_ms=best, table_ms=table.round(4).tolist(), zchunks=list(ZCHUNKS),
+                    class_zchunk_ms=ztable.round(4).tolist(), chosen=b.config(), classes=b.classes())
     graphs = [b.graph(phi0, phi1, kap, vel), b.graph(phi1, phi0, kap, vel)]
     for s in range(warmup):
         Batch.launch(graphs[s % 2])
@@ -360,12 +366,14 @@ def batch_run(ncells, seed, steps, warmup, interp="cubic", bt="rk2", autotune=Tr
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     gpu_s = e0.elapsed_time(e1) * 1e-3
-    interior = int(sum((int(n) - 4) ** 3 for n in edges))
+    interior = int(sum((int(n) - 4) ** dim for n in edges))
     ups = interior * steps / wall
-    out = dict(cells=ncells, nodes=int(b.nodes), interior_updates_per_step=interior, steps=steps,
-               interp=interp, backtrace=bt, updates_per_s=ups, ms_per_step=wall / steps * 1e3,
-               gpu_ms_per_step=gpu_s / steps * 1e3, step_roofline_frac=ups * 24 / HBM_PEAK,
-               autotune=tune)
+    isz = torch.tensor([], dtype=dtype).element_size()
+    bpu = (3 + dim) * isz     # phi0 + velocity read, phi + kappa written
+    out = dict(cells=ncells, dim=dim, edges=[lo, hi], nodes=int(b.nodes), interior_updates_per_step=interior,
+               steps=steps, interp=interp, backtrace=bt, dtype=str(dtype).replace("torch.", ""),
+               updates_per_s=ups, ms_per_step=wall / steps * 1e3, gpu_ms_per_step=gpu_s / steps * 1e3,
+               bytes_per_update=bpu, step_roofline_frac=ups * bpu / HBM_PEAK, autotune=tune)
     b.close()
     del phi0, phi1, kap, vel
     torch.cuda.empty_cache()
@@ -930,6 +938,9 @@ def _f32():
 FORMS = {
     "batched_fp32_config_scheme": lambda: batch_run(512, 4, 100, 3, "cubic", "rk2"),
     "batched_fp32_reference_scheme": lambda: batch_run(512, 4, 100, 3, "linear", "euler"),
+    "batched_2d_fp32_config_scheme": lambda: batch_run(512, 4, 100, 3, "cubic", "rk2", dim=2, lo=64, hi=512),
+    "batched_2d_fp64_config_scheme": lambda: batch_run(512, 4, 100, 3, "cubic", "rk2", dim=2, lo=64, hi=512,
+                                                       dtype=_f64()),
     "precision_pls_384": lambda: pls_run(384, 150),
     "adjoint_128": lambda: adjoint_run(128, 50, 10),
     "fp64_reference_scheme_256": lambda: dense_run(256, _f64(), "linear", "euler", 50, 2),

@@ -22,14 +22,17 @@ DCL_PLS_REF_AZ = 1
 EXPORTS = (
     "dcl_abi_version", "dcl_last_error", "dcl_device_count", "dcl_malloc", "dcl_free",
     "dcl_memcpy_htod", "dcl_memcpy_dtoh", "dcl_memcpy_dtod", "dcl_stream_synchronize",
+    "dcl_host_malloc", "dcl_host_free",
     "dcl_sl_advect", "dcl_curvature", "dcl_levelset_step",
     "dcl_pls_advect", "dcl_pls_ws_create", "dcl_pls_ws_destroy", "dcl_pls_correct",
     "dcl_pls_adjust_radii", "dcl_max_velocity", "dcl_band_compact", "dcl_irregular_nodes",
     "dcl_surface_quantities", "dcl_get_velocity", "dcl_reinitialize", "dcl_reinitialize_parallel",
     "dcl_real64_pack", "dcl_irregular_node_rows",
     "dcl_interface_metrics", "dcl_error_norms", "dcl_sl_adjoint",
+    "dcl_sl_adjoint_record_bytes", "dcl_sl_adjoint_prepare", "dcl_sl_adjoint_step", "dcl_sl_adjoint_finish",
     "dcl_batch_create", "dcl_batch_destroy", "dcl_batch_layout", "dcl_batch_set_dt",
-    "dcl_batch_set_config", "dcl_batch_get_config", "dcl_batch_step", "dcl_batch_autotune",
+    "dcl_batch_set_config", "dcl_batch_get_config", "dcl_batch_set_class_zchunk", "dcl_batch_get_classes",
+    "dcl_batch_step", "dcl_batch_band_step", "dcl_batch_autotune",
     "dcl_batch_graph_create", "dcl_batch_graph_launch", "dcl_batch_graph_destroy",
 )
 
@@ -81,6 +84,8 @@ def _declare(L):
         "dcl_memcpy_dtoh": ([vp, vp, C.c_size_t, vp], i32),
         "dcl_memcpy_dtod": ([vp, vp, C.c_size_t, vp], i32),
         "dcl_stream_synchronize": ([vp], i32),
+        "dcl_host_malloc": ([C.POINTER(vp), C.c_size_t], i32),
+        "dcl_host_free": ([vp], i32),
         "dcl_sl_advect": ([vp, vp, W, V, vp, i64, vp, dbl, i32, i32, i32, vp], i32),
         "dcl_curvature": ([vp, vp, W, vp, i64, vp, i32, i32, vp], i32),
         "dcl_levelset_step": ([vp, vp, vp, W, V, dbl, i32, i32, i32, vp], i32),
@@ -101,14 +106,21 @@ def _declare(L):
         "dcl_interface_metrics": ([vp, vp, W, dbl, i32, C.POINTER(dbl), vp], i32),
         "dcl_error_norms": ([vp, i32, vp, i32, i64, C.POINTER(dbl), vp], i32),
         "dcl_sl_adjoint": ([vp, W, V, vp, vp, vp, dbl, i32, i32, i32, vp], i32),
+        "dcl_sl_adjoint_record_bytes": ([W, C.POINTER(i64)], i32),
+        "dcl_sl_adjoint_prepare": ([W, V, dbl, i32, i32, i32, vp, vp], i32),
+        "dcl_sl_adjoint_step": ([vp, W, vp, vp, vp, vp, dbl, i32, i32, vp], i32),
+        "dcl_sl_adjoint_finish": ([W, V, vp, vp, dbl, i32, i32, vp], i32),
         "dcl_batch_create": ([i32, W, i32, i32, i32, C.POINTER(vp)], i32),
         "dcl_batch_destroy": ([vp], i32),
         "dcl_batch_layout": ([vp, vp, vp], i32),
         "dcl_batch_set_dt": ([vp, vp], i32),
         "dcl_batch_set_config": ([vp, i32, i32], i32),
         "dcl_batch_get_config": ([vp, vp, vp, vp, vp], i32),
+        "dcl_batch_set_class_zchunk": ([vp, i32, i32], i32),
+        "dcl_batch_get_classes": ([vp, vp, vp, vp], i32),
         "dcl_batch_step": ([vp, vp, vp, vp, vp, vp], i32),
-        "dcl_batch_autotune": ([vp, vp, vp, vp, vp, i32, vp, vp, vp], i32),
+        "dcl_batch_band_step": ([vp, vp, vp, vp, vp, vp, i64, vp], i32),
+        "dcl_batch_autotune": ([vp, vp, vp, vp, vp, i32, vp, vp, vp, vp], i32),
         "dcl_batch_graph_create": ([vp, vp, vp, vp, vp, C.POINTER(vp)], i32),
         "dcl_batch_graph_launch": ([vp, vp], i32),
         "dcl_batch_graph_destroy": ([vp], i32),

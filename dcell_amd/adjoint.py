@@ -41,8 +41,57 @@ def sl_adjoint(phi0, vel, lam_next, win: GridWindow, dt: float, interp="cubic", 
     return lam, gvel
 
 
+def adjoint_record(vel, win: GridWindow, dt: float, interp="cubic", backtrace="rk2"):
+    """The departure record of a steady velocity (dcl_sl_adjoint_prepare): every
+    interior node's stencil base, fractions and clamped axes, 48 bytes per node."""
+    precision_of(vel)
+    nbytes = C.c_int64()
+    L.call("dcl_sl_adjoint_record_bytes", C.byref(win.c()), C.byref(nbytes))
+    rec = torch.empty(nbytes.value, dtype=torch.uint8, device=vel.device)
+    V = velocity(vel, win)
+    L.call("dcl_sl_adjoint_prepare", C.byref(win.c()), C.byref(V), float(dt), precision_of(vel), INTERP[interp],
+           BACKTRACE[backtrace], _p(rec), stream_handle())
+    return rec
+
+
+def adjoint_step(phi0, rec, lam_next, win: GridWindow, dt: float, lam, hsum, interp="cubic"):
+    """lam += dL/dphi0 of one step from the record; hsum (3 per node) += the
+    step's departure sensitivities (dcl_sl_adjoint_step)."""
+    _dev(phi0, "phi0")
+    prec = precision_of(phi0)
+    for t, name, count in ((lam_next, "lam_next", 1), (lam, "lam", 1), (hsum, "hsum", 3)):
+        _dev(t, name, phi0.dtype)
+        if t.numel() != win.nodes * count:
+            raise ValueError(f"{name} has {t.numel()} elements, the window needs {win.nodes * count}")
+    if phi0.numel() != win.nodes:
+        raise ValueError(f"phi0 has {phi0.numel()} elements, the window needs {win.nodes}")
+    _dev(rec, "record", torch.uint8)
+    if rec.numel() != 48 * win.nodes:
+        raise ValueError("record does not belong to this window")
+    L.call("dcl_sl_adjoint_step", _p(phi0), C.byref(win.c()), _p(rec), _p(lam_next), _p(lam), _p(hsum),
+           float(dt), prec, INTERP[interp], stream_handle())
+    return lam, hsum
+
+
+def adjoint_finish(vel, hsum, win: GridWindow, dt: float, gvel, backtrace="rk2"):
+    """gvel += dL/dvel formed from the accumulated sensitivities (dcl_sl_adjoint_finish)."""
+    prec = precision_of(vel)
+    _dev(hsum, "hsum", vel.dtype)
+    _dev(gvel, "gvel", vel.dtype)
+    if hsum.numel() != 3 * win.nodes or gvel.shape != vel.shape:
+        raise ValueError("hsum or gvel does not match the window / velocity")
+    V = velocity(vel, win)
+    L.call("dcl_sl_adjoint_finish", C.byref(win.c()), C.byref(V), _p(hsum), _p(gvel), float(dt), prec,
+           BACKTRACE[backtrace], stream_handle())
+    return gvel
+
+
 class AdvectN(torch.autograd.Function):
-    """phi_N = SL^N(phi_0; vel) with checkpointed adjoint backward."""
+    """phi_N = SL^N(phi_0; vel) with checkpointed adjoint backward.  The
+    velocity is the same at every step, so the backward computes the
+    departure record once, runs one record-driven adjoint step per SL step
+    (reverse order, states recomputed per checkpoint segment) and forms
+    dL/dvel once from the accumulated sensitivities."""
 
     @staticmethod
     def forward(ctx, phi0, vel, win, dt, steps, interp, backtrace, checkpoint_every):
@@ -63,7 +112,9 @@ class AdvectN(torch.autograd.Function):
         win, dt, steps, interp, backtrace, every = ctx.cfg
         vel = vel.detach().contiguous()
         lam = grad_out.detach().contiguous().clone()
-        gvel = torch.zeros_like(vel)
+        rec = adjoint_record(vel, win, dt, interp, backtrace)
+        hsum = torch.zeros(3 * win.nodes, dtype=vel.dtype, device=vel.device)
+        other = torch.empty_like(lam)
         nseg = len(ctx.ckpts)
         for seg in reversed(range(nseg)):
             s0 = seg * every
@@ -72,10 +123,10 @@ class AdvectN(torch.autograd.Function):
             for _ in range(s0, s1 - 1):
                 states.append(advect_sl(states[-1], vel, win, dt, interp, backtrace))
             for s in reversed(range(s0, s1)):
-                lam_prev = torch.zeros_like(lam)
-                sl_adjoint(states[s - s0], vel, lam, win, dt, interp, backtrace, lam=lam_prev,
-                           gvel=gvel)
-                lam = lam_prev
+                lam_prev = other.zero_()
+                adjoint_step(states[s - s0], rec, lam, win, dt, lam_prev, hsum, interp)
+                lam, other = lam_prev, lam
+        gvel = adjoint_finish(vel, hsum, win, dt, torch.zeros_like(vel), backtrace)
         ctx.ckpts = None
         return lam, gvel, None, None, None, None, None, None
 

@@ -238,3 +238,43 @@ def batch_edges(ncells: int = 512, seed: int = 4, lo: int = 32, hi: int = 128) -
     """Cell edges log-uniform in [lo, hi] (Philox, seed)."""
     g = philox(seed)
     return np.rint(np.exp(g.uniform(math.log(lo), math.log(hi), size=ncells))).astype(int)
+
+
+def cell_fields_2d(n: int, g: np.random.Generator, device, dtype, radius=0.25, modes=4, amp=0.05,
+                   vmodes=6, kmax=4, vamp=0.5):
+    """phi (grid units) and MAC velocity of one 2D cell on [0, 1]^2 with n nodes
+    per axis, the 2D counterpart of cell_fields: a circle of `radius`
+    perturbed by `modes` random angular modes cos/sin(m theta), m 1..4 (peak
+    amplitude amp * radius), in a sum of `vmodes` random divergence-free
+    Fourier modes A sin(2 pi k.x + theta) with A perpendicular to k
+    (|k|_inf 1..kmax, A ~ N(0, vamp))."""
+    import torch
+    h = 1.0 / (n - 1)
+    c = torch.arange(n, device=device, dtype=torch.float64) * h
+    Y, X = torch.meshgrid(c, c, indexing="ij")
+    dx, dy = X - 0.5, Y - 0.5
+    r = torch.sqrt(dx * dx + dy * dy).clamp_min(1e-12)
+    th = torch.atan2(dy, dx)
+    pert = torch.zeros_like(r)
+    for _ in range(modes):
+        m = int(g.integers(1, 5))
+        pert = pert + float(g.normal()) * (torch.cos(m * th) if g.random() < 0.5 else torch.sin(m * th))
+    pert = pert * (amp / max(float(pert.abs().max()), 1e-12))
+    phi = (r - radius * (1.0 + pert)) / h
+    vel = torch.zeros((n, n, 2), device=device, dtype=torch.float64)
+    for _ in range(vmodes):
+        while True:
+            k = g.integers(-kmax, kmax + 1, size=2)
+            if 1 <= np.abs(k).max() <= kmax:
+                break
+        a = g.normal(0.0, vamp, size=2)
+        a = a - k * (a @ k) / (k @ k)
+        theta = g.uniform(0, 2 * math.pi)
+        for comp in range(2):
+            sh = [0.0, 0.0]
+            sh[comp] = -0.5 * h
+            arg = 2 * math.pi * (k[0] * (X + sh[0]) + k[1] * (Y + sh[1])) + theta
+            vel[..., comp] += float(a[comp]) * torch.sin(arg)
+    vmax = float(vel.abs().max())
+    dt = 0.5 * h / vmax if vmax > 0 else h
+    return phi.to(dtype), vel.to(dtype), dt

@@ -16,6 +16,8 @@
 // with DPP row shifts before the LDS atomic, and likewise the shared corners
 // of their MAC velocity scatter before the HBM atomic.  Outer two layers:
 // phi = phi0 there, so dL/dphi0 += lam_next.
+#include <algorithm>
+
 #include "dcl_host.hpp"
 
 namespace dcl {
@@ -360,6 +362,270 @@ __global__ __launch_bounds__(256) void sl_adjoint_k(const T* __restrict__ phi0, 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Steady velocity (one velocity for every step, the differentiation form):
+// the departure geometry of a node is the same at every step, and the
+// velocity gradient of a step is linear in the node's sensitivity
+// H = -c g dI/dS.  So
+//   prepare (once):  each interior node's stencil base, fractions and
+//                    clamped axes -> a record (no phi);
+//   step (per step): the transposed gather of lam_next into lam from the
+//                    record, and hsum[node] += H (the node's own entries, no
+//                    atomics, no velocity reads);
+//   finish (once):   dL/dvel += the MAC / node-velocity scatter of hsum, the
+//                    same scatter as sl_adjoint_k's per step.
+// The record holds {bx, by, bz, flags} (int4, flags: bit a = axis a clamped,
+// bit 3 = interior node) and the three fractions in double (a double4), from
+// which the weights are formed exactly as the forward forms them.
+constexpr size_t REC_BYTES = sizeof(int4) + sizeof(double4);
+
+template <typename T, int INTERP, int BT>
+__global__ __launch_bounds__(256) void adj_prepare_k(const T* __restrict__ vel, AdjP P, int4* __restrict__ rb,
+                                                     double4* __restrict__ rf) {
+  const int64_t n = (int64_t)P.w.nx * P.w.ny * P.w.nz;
+  for (int64_t id = (int64_t)blockIdx.x * 256 + threadIdx.x; id < n; id += (int64_t)gridDim.x * 256) {
+    const int i = (int)(id % P.w.nx), j = (int)((id / P.w.nx) % P.w.ny), k = (int)(id / ((int64_t)P.w.nx * P.w.ny));
+    const bool inner = i >= 2 && i <= P.w.nx - 3 && j >= 2 && j <= P.w.ny - 3 && k >= 2 && k <= P.w.nz - 3;
+    int4 q = make_int4(0, 0, 0, 0);
+    double4 f4 = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (inner) {
+      const int X = P.w.px + i, Yg = P.w.py + j, Zg = P.w.pz + k;
+      const double Xd[3] = {(double)X, (double)Yg, (double)Zg};
+      T V1[3];
+      node_vel<T, 3>(vel, P.v, X, Yg, Zg, V1);
+      double S[3];
+      if (BT == DCL_RK2) {
+        double Xm[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) Xm[a] = Xd[a] - (double)V1[a] * P.ch[a];
+        T V2[3];
+        mac_vel<T, 3>(vel, P.v, Xm, V2);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) S[a] = Xd[a] - (double)V2[a] * P.c[a];
+      } else {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) S[a] = Xd[a] - (double)V1[a] * P.c[a];
+      }
+      const int pp[3] = {P.w.px, P.w.py, P.w.pz}, nn[3] = {P.w.nx, P.w.ny, P.w.nz};
+      int b[3], flags = 8;
+      double fr[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        double f;
+        if (INTERP == DCL_CUBIC) {
+          const double lo = pp[a] + 1, hi = pp[a] + nn[a] - 2;
+          if (!(S[a] >= lo && S[a] <= hi)) flags |= 1 << a;
+          b[a] = cub_split(S[a], pp[a], nn[a], f);
+        } else {
+          const double lo = pp[a], hi = pp[a] + nn[a] - 1;
+          if (!(S[a] >= lo && S[a] <= hi)) flags |= 1 << a;
+          b[a] = lin_split(S[a], pp[a], nn[a], f);
+        }
+        fr[a] = f;
+      }
+      q = make_int4(b[0], b[1], b[2], flags);
+      f4.x = fr[0];
+      f4.y = fr[1];
+      f4.z = fr[2];
+    }
+    rb[id] = q;
+    rf[id] = f4;
+  }
+}
+
+// One adjoint step from the record: blocks of 16 x 4 x 4 nodes, the same LDS
+// box, lane combining and flush as sl_adjoint_k.
+template <typename T, int INTERP>
+__global__ __launch_bounds__(256) void adj_step_k(const T* __restrict__ phi0, const int4* __restrict__ rb,
+                                                  const double4* __restrict__ rf,
+                                                  const T* __restrict__ lam_next, T* __restrict__ lam,
+                                                  T* __restrict__ hsum, AdjP P) {
+  __shared__ T box[LBZ * LBY * LBX];
+  const int tid = threadIdx.x + ABX * (threadIdx.y + ABY * threadIdx.z);
+  for (int i = tid; i < LBZ * LBY * LBX; i += 256) box[i] = T(0);
+  __syncthreads();
+
+  const int ox = blockIdx.x * ABX, oy = blockIdx.y * ABY, oz = blockIdx.z * ABZ;
+  const int i = ox + threadIdx.x, j = oy + threadIdx.y, k = oz + threadIdx.z;
+  const int nx = P.w.nx, ny = P.w.ny, nz = P.w.nz;
+  const int Y = nx, Z = nx * ny;
+  const bool valid = i < nx && j < ny && k < nz;
+  const bool inner = valid && i >= 2 && i <= nx - 3 && j >= 2 && j <= ny - 3 && k >= 2 && k <= nz - 3;
+  const int id = k * Z + j * Y + i;
+  if (valid && !inner) {
+    const T g = lam_next[id];
+    if (g != T(0)) atomic_add(&lam[id], g);
+  }
+  T g = T(0);
+  int4 q = make_int4(0, 0, 0, 0);
+  double4 f4 = make_double4(0.0, 0.0, 0.0, 0.0);
+  if (inner) {
+    g = lam_next[id];
+    q = rb[id];
+    f4 = rf[id];
+  }
+  const bool act = inner && g != T(0);
+  const int NP = INTERP == DCL_CUBIC ? 4 : 2;
+  const int OFF = INTERP == DCL_CUBIC ? -1 : 0;
+  const int b[3] = {q.x, q.y, q.z};
+  T w[3][4], dw[3][4];
+  int bx0 = 0, by0 = 0, bz0 = 0;
+  bool in_box = false;
+  if (act) {
+    const double fr[3] = {f4.x, f4.y, f4.z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (INTERP == DCL_CUBIC) {
+        cr_weights<T>((T)fr[a], w[a]);
+        cr_dweights<T>((T)fr[a], dw[a]);
+      } else {
+        w[a][0] = (T)(1.0 - fr[a]);
+        w[a][1] = (T)fr[a];
+        dw[a][0] = T(-1);
+        dw[a][1] = T(1);
+      }
+    }
+    bx0 = b[0] + OFF - (ox - AR);
+    by0 = b[1] + OFF - (oy - AR);
+    bz0 = b[2] + OFF - (oz - AR);
+    in_box = bx0 >= 0 && bx0 + NP <= LBX && by0 >= 0 && by0 + NP <= LBY && bz0 >= 0 && bz0 + NP <= LBZ;
+  } else {
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq) w[a][qq] = dw[a][qq] = T(0);
+  }
+  const int key = in_box ? (bz0 * LBY + by0) * LBX + bx0 + 1 : 0;
+  bool lo_m[4] = {false, false, false, false}, up_m[4] = {false, false, false, false};
+#pragma unroll
+  for (int d = 1; d < NP; ++d) {
+    const int kl = lane_from_lower(key, d), ku = lane_from_upper(key, d);
+    lo_m[d] = key != 0 && kl != 0 && kl + d == key;
+    up_m[d] = key != 0 && ku != 0 && key + d == ku;
+  }
+  T G[3] = {T(0), T(0), T(0)};
+  const T* src = phi0 + (b[2] + OFF) * Z + (b[1] + OFF) * Y + (b[0] + OFF);
+#pragma unroll 1
+  for (int kk = 0; kk < NP; ++kk)
+#pragma unroll
+    for (int jj = 0; jj < NP; ++jj) {
+      const T wyz = w[1][jj] * w[2][kk];
+      T add[4];
+#pragma unroll
+      for (int ii = 0; ii < NP; ++ii) add[ii] = g * w[0][ii] * wyz;
+      T word0 = add[0];
+#pragma unroll
+      for (int d = 1; d < NP; ++d) {
+        const T o = lane_from_lower(add[d], d);
+        if (lo_m[d]) word0 += o;
+      }
+      if (act) {
+        const T* r = src + kk * Z + jj * Y;
+        T rx = T(0), rd = T(0);
+#pragma unroll
+        for (int ii = 0; ii < NP; ++ii) {
+          const T f = r[ii];
+          rx += w[0][ii] * f;
+          rd += dw[0][ii] * f;
+          if (in_box) {
+            T* bw = &box[((bz0 + kk) * LBY + (by0 + jj)) * LBX + (bx0 + ii)];
+            if (ii == 0)
+              atomicAdd(bw, word0);
+            else if (!up_m[ii])
+              atomicAdd(bw, add[ii]);
+          } else {
+            atomic_add(&lam[(b[2] + OFF + kk) * Z + (b[1] + OFF + jj) * Y + (b[0] + OFF + ii)], add[ii]);
+          }
+        }
+        G[0] += rd * wyz;
+        G[1] += rx * dw[1][jj] * w[2][kk];
+        G[2] += rx * w[1][jj] * dw[2][kk];
+      }
+    }
+  if (act) {
+    T* hs = hsum + 3 * (int64_t)id;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      if (!(q.w & (1 << a))) hs[a] += (T)(-P.c[a]) * g * G[a];
+  }
+  __syncthreads();
+  for (int t = tid; t < LBZ * LBY * LBX; t += 256) {
+    const T v = box[t];
+    if (v == T(0)) continue;
+    const int lx = t % LBX, ly = (t / LBX) % LBY, lz = t / (LBX * LBY);
+    const int gx = ox - AR + lx, gy = oy - AR + ly, gz = oz - AR + lz;
+    if (gx < 0 || gx >= nx || gy < 0 || gy >= ny || gz < 0 || gz >= nz) continue;
+    atomic_add(&lam[gz * Z + gy * Y + gx], v);
+  }
+}
+
+// dL/dvel += the velocity scatter of the accumulated sensitivities H = hsum
+// (the tail of sl_adjoint_k: the MAC corners at the midpoint and, through the
+// midpoint's dependence on the node velocity, the node's faces), 16 x 4 x 4
+// blocks for the same lane combining.
+template <typename T, int BT>
+__global__ __launch_bounds__(256) void adj_finish_k(const T* __restrict__ vel, const T* __restrict__ hsum,
+                                                    T* __restrict__ gvel, AdjP P) {
+  const int tid = threadIdx.x + ABX * (threadIdx.y + ABY * threadIdx.z);
+  const int i = blockIdx.x * ABX + threadIdx.x, j = blockIdx.y * ABY + threadIdx.y, k = blockIdx.z * ABZ + threadIdx.z;
+  const int nx = P.w.nx, ny = P.w.ny, nz = P.w.nz;
+  const bool inner = i < nx && j < ny && k < nz && i >= 2 && i <= nx - 3 && j >= 2 && j <= ny - 3 && k >= 2 &&
+                     k <= nz - 3;
+  T H[3] = {T(0), T(0), T(0)};
+  if (inner) {
+    const T* hs = hsum + 3 * (int64_t)((k * ny + j) * nx + i);
+    H[0] = hs[0];
+    H[1] = hs[1];
+    H[2] = hs[2];
+  }
+  const bool act = inner && (H[0] != T(0) || H[1] != T(0) || H[2] != T(0));
+  T H1[3] = {T(0), T(0), T(0)};
+  MacState<T> m2;
+  if (act) {
+    if (BT == DCL_RK2) {
+      const int X = P.w.px + i, Yg = P.w.py + j, Zg = P.w.pz + k;
+      const double Xd[3] = {(double)X, (double)Yg, (double)Zg};
+      T V1[3];
+      node_vel<T, 3>(vel, P.v, X, Yg, Zg, V1);
+      double Xm[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) Xm[a] = Xd[a] - (double)V1[a] * P.ch[a];
+      bool cm[3] = {false, false, false};
+      mac_state<T>(vel, P.v, Xm, m2, cm);
+#pragma unroll
+      for (int bb = 0; bb < 3; ++bb) {
+        const T K = H[0] * m2.dV[0][bb] + H[1] * m2.dV[1][bb] + H[2] * m2.dV[2][bb];
+        H1[bb] = (T)(-P.ch[bb]) * K;
+      }
+    } else {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) H1[a] = H[a];
+    }
+  }
+  if (BT == DCL_RK2) mac_scatter_lanes<T>(gvel, P.v, m2, H, act);
+  const int sx = P.v.dof, sy = P.w.nx * P.v.dof, sz = P.w.ny * sy;
+  const int gofs = (k * P.w.ny + j) * sy + i * sx + P.v.udof;
+  const T h0 = T(0.5) * H1[0], h1 = T(0.5) * H1[1], h2 = T(0.5) * H1[2];
+  const int nkey = act ? gofs + 1 : 0;
+  const int nkl = lane_from_lower(nkey, 1), nku = lane_from_upper(nkey, 1);
+  const bool nlo = act && nkl != 0 && nkl + sx == nkey, nup = act && nku != 0 && nkey + sx == nku;
+  const T h0l = lane_from_lower(h0, 1);
+  const int lane = tid & 63;
+  const int ykl = __shfl_up(nkey, ABX), yku = __shfl_down(nkey, ABX);
+  const bool ylo = act && lane >= ABX && ykl != 0 && ykl + sy == nkey;
+  const bool yup = act && lane < 64 - ABX && yku != 0 && nkey + sy == yku;
+  const T h1l = __shfl_up(h1, ABX);
+  if (act) {
+    T* gb = gvel + gofs;
+    atomic_add(gb, nlo ? h0 + h0l : h0);
+    if (!nup) atomic_add(gb + sx, h0);
+    atomic_add(gb + 1, ylo ? h1 + h1l : h1);
+    if (!yup) atomic_add(gb + sy + 1, h1);
+    atomic_add(gb + 2, h2);
+    atomic_add(gb + sz + 2, h2);
+  }
+}
+
 template <typename T, int INTERP, int BT>
 static int launch_adj(const void* phi0, const void* vel, const void* lam_next, void* lam, void* gvel,
                       const AdjP& P, hipStream_t s) {
@@ -383,6 +649,132 @@ static int dispatch_adj(int interp, int bt, const void* phi0, const void* vel, c
 }  // namespace dcl
 
 using namespace dcl;
+
+static int adj_params(const dcl_window* win, const dcl_velocity* vel, double dt, AdjP& P) {
+  DCL_TRY(check_window(win, "phi", 5));
+  if (win->n.z < 2) return set_error(DCL_ERR_SUP, "the adjoint is 3D only");
+  DCL_TRY(check_velocity(vel, 3));
+  const dcl_window& vw = vel->win;
+  if (vw.p.x != win->p.x || vw.p.y != win->p.y || vw.p.z != win->p.z || vw.n.x != win->n.x ||
+      vw.n.y != win->n.y || vw.n.z != win->n.z)
+    return set_error(DCL_ERR_SUP, "the adjoint needs the velocity window to equal phi's");
+  P.w = kwin(*win);
+  P.v.w = kwin(vw);
+  P.v.dof = vel->dof;
+  P.v.udof = vel->udof;
+  const double d[3] = {win->d.x, win->d.y, win->d.z};
+  for (int a = 0; a < 3; ++a) {
+    P.c[a] = dt / d[a];
+    P.ch[a] = dt / (2.0 * d[a]);
+  }
+  return 0;
+}
+
+static int check_scheme(int interp, int backtrace) {
+  if (interp != DCL_LINEAR && interp != DCL_CUBIC) return set_error(DCL_ERR_ARG_WRONG, "bad interp");
+  if (backtrace != DCL_EULER && backtrace != DCL_RK2) return set_error(DCL_ERR_ARG_WRONG, "bad backtrace");
+  return 0;
+}
+
+extern "C" int dcl_sl_adjoint_record_bytes(const dcl_window* win, int64_t* bytes) {
+  if (!bytes) return set_error(DCL_ERR_ARG_NULL, "bytes is NULL");
+  DCL_TRY(check_window(win, "phi", 5));
+  *bytes = win_nodes(*win) * (int64_t)REC_BYTES;
+  return 0;
+}
+
+template <typename T>
+static int adj_prepare_t(int interp, int bt, const void* vel, const AdjP& P, void* record, hipStream_t s) {
+  const int64_t n = (int64_t)P.w.nx * P.w.ny * P.w.nz;
+  int4* rb = static_cast<int4*>(record);
+  double4* rf = reinterpret_cast<double4*>(rb + n);
+  const unsigned nb = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
+  const T* v = static_cast<const T*>(vel);
+#define DCL_PREP(I, B) hipLaunchKernelGGL((adj_prepare_k<T, I, B>), dim3(nb), dim3(256), 0, s, v, P, rb, rf)
+  if (interp == DCL_CUBIC && bt == DCL_RK2) DCL_PREP(DCL_CUBIC, DCL_RK2);
+  else if (interp == DCL_CUBIC) DCL_PREP(DCL_CUBIC, DCL_EULER);
+  else if (bt == DCL_RK2) DCL_PREP(DCL_LINEAR, DCL_RK2);
+  else DCL_PREP(DCL_LINEAR, DCL_EULER);
+#undef DCL_PREP
+  DCL_LAUNCHED("adj_prepare");
+  return 0;
+}
+
+extern "C" int dcl_sl_adjoint_prepare(const dcl_window* win, const dcl_velocity* vel, double dt, int precision,
+                                      int interp, int backtrace, void* record, void* stream) {
+  DCL_TRY(check_precision(precision));
+  DCL_TRY(check_scheme(interp, backtrace));
+  if (!record) return set_error(DCL_ERR_ARG_NULL, "record is NULL");
+  AdjP P;
+  DCL_TRY(adj_params(win, vel, dt, P));
+  hipStream_t s = as_stream(stream);
+  if (precision == DCL_F64) return adj_prepare_t<double>(interp, backtrace, vel->data, P, record, s);
+  return adj_prepare_t<float>(interp, backtrace, vel->data, P, record, s);
+}
+
+template <typename T>
+static int adj_step_t(int interp, const void* phi0, const void* record, const void* lam_next, void* lam,
+                      void* hsum, const AdjP& P, hipStream_t s) {
+  const int64_t n = (int64_t)P.w.nx * P.w.ny * P.w.nz;
+  const int4* rb = static_cast<const int4*>(record);
+  const double4* rf = reinterpret_cast<const double4*>(rb + n);
+  dim3 grid((P.w.nx + ABX - 1) / ABX, (P.w.ny + ABY - 1) / ABY, (P.w.nz + ABZ - 1) / ABZ);
+  const T* p0 = static_cast<const T*>(phi0);
+  const T* ln = static_cast<const T*>(lam_next);
+  if (interp == DCL_CUBIC)
+    hipLaunchKernelGGL((adj_step_k<T, DCL_CUBIC>), grid, dim3(ABX, ABY, ABZ), 0, s, p0, rb, rf, ln,
+                       static_cast<T*>(lam), static_cast<T*>(hsum), P);
+  else
+    hipLaunchKernelGGL((adj_step_k<T, DCL_LINEAR>), grid, dim3(ABX, ABY, ABZ), 0, s, p0, rb, rf, ln,
+                       static_cast<T*>(lam), static_cast<T*>(hsum), P);
+  DCL_LAUNCHED("adj_step");
+  return 0;
+}
+
+extern "C" int dcl_sl_adjoint_step(const void* phi0, const dcl_window* win, const void* record,
+                                   const void* lam_next, void* lam, void* hsum, double dt, int precision,
+                                   int interp, void* stream) {
+  DCL_TRY(check_precision(precision));
+  if (!phi0 || !record || !lam_next || !lam || !hsum) return set_error(DCL_ERR_ARG_NULL, "a buffer is NULL");
+  if (interp != DCL_LINEAR && interp != DCL_CUBIC) return set_error(DCL_ERR_ARG_WRONG, "bad interp");
+  DCL_TRY(check_window(win, "phi", 5));
+  if (win->n.z < 2) return set_error(DCL_ERR_SUP, "the adjoint is 3D only");
+  if (lam == lam_next || lam == phi0 || hsum == lam || hsum == lam_next)
+    return set_error(DCL_ERR_ARG_WRONG, "lam and hsum must not alias inputs or each other");
+  AdjP P;
+  P.w = kwin(*win);
+  const double d[3] = {win->d.x, win->d.y, win->d.z};
+  for (int a = 0; a < 3; ++a) {
+    P.c[a] = dt / d[a];
+    P.ch[a] = dt / (2.0 * d[a]);
+  }
+  hipStream_t s = as_stream(stream);
+  if (precision == DCL_F64) return adj_step_t<double>(interp, phi0, record, lam_next, lam, hsum, P, s);
+  return adj_step_t<float>(interp, phi0, record, lam_next, lam, hsum, P, s);
+}
+
+extern "C" int dcl_sl_adjoint_finish(const dcl_window* win, const dcl_velocity* vel, const void* hsum, void* gvel,
+                                     double dt, int precision, int backtrace, void* stream) {
+  DCL_TRY(check_precision(precision));
+  if (backtrace != DCL_EULER && backtrace != DCL_RK2) return set_error(DCL_ERR_ARG_WRONG, "bad backtrace");
+  if (!hsum || !gvel) return set_error(DCL_ERR_ARG_NULL, "a buffer is NULL");
+  AdjP P;
+  DCL_TRY(adj_params(win, vel, dt, P));
+  if (gvel == vel->data || gvel == hsum) return set_error(DCL_ERR_ARG_WRONG, "gvel must not alias inputs");
+  dim3 grid((P.w.nx + ABX - 1) / ABX, (P.w.ny + ABY - 1) / ABY, (P.w.nz + ABZ - 1) / ABZ);
+  hipStream_t s = as_stream(stream);
+#define DCL_FIN(T, B)                                                                                      \
+  hipLaunchKernelGGL((adj_finish_k<T, B>), grid, dim3(ABX, ABY, ABZ), 0, s, static_cast<const T*>(vel->data), \
+                     static_cast<const T*>(hsum), static_cast<T*>(gvel), P)
+  if (precision == DCL_F64) {
+    if (backtrace == DCL_RK2) DCL_FIN(double, DCL_RK2); else DCL_FIN(double, DCL_EULER);
+  } else {
+    if (backtrace == DCL_RK2) DCL_FIN(float, DCL_RK2); else DCL_FIN(float, DCL_EULER);
+  }
+#undef DCL_FIN
+  DCL_LAUNCHED("adj_finish");
+  return 0;
+}
 
 extern "C" int dcl_sl_adjoint(const void* phi0, const dcl_window* win, const dcl_velocity* vel,
                               const void* lam_next, void* lam, void* gvel, double dt, int precision,

@@ -3,15 +3,21 @@
 // The reference advects the DCells of a rank one after another
 // (DWorld/DCellsArray.c:57-60), each over its own small window.  Here the
 // windows are packed back to back in one buffer per field and cut into tiles;
-// a persistent kernel walks the packed tile list (each XCD a contiguous run
-// of it, for L2 locality) and runs the SL step of dcl_tile.hpp on every tile,
-// then a second persistent kernel does the curvature.  One step
-// can be recorded as a hipGraph of these two launches.  Tile shape and
-// resident blocks per CU come from a table of configurations; dcl_batch_autotune
-// times them all on the caller's buffers and keeps the fastest.
+// a persistent kernel claims the tiles of the packed tile list (work queue
+// below) and runs the SL step on every tile, then a second persistent kernel
+// does the curvature.  One step can be recorded as a hipGraph of these two
+// launches.  3D cells: the curvature tile shape, the SL kernel (tile gathers
+// or LDS-ring columns) and the resident blocks per CU come from a table of
+// configurations, and the ring columns' z-chunk is chosen per cell-size
+// class; dcl_batch_autotune times them on the caller's buffers and keeps the
+// fastest.  2D cells: 64 x 4 node tiles, one node per thread.  Band mode
+// (dcl_batch_band_step): only the listed nodes of every cell, as the
+// reference's band loop (LevelSetMethod/AdvectSL.c:23-89), one thread per
+// node over the packed band list.
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
 #include <new>
 #include <vector>
 
@@ -23,8 +29,10 @@ namespace dcl {
 
 struct BCell {
   long long phi_off;  // element offset of the cell in the packed phi / kappa buffers
-  long long vel_off;  // element offset in the packed velocity buffer (3 per node)
+  long long vel_off;  // element offset in the packed velocity buffer (dim per node)
   int px, py, pz, nx, ny, nz;
+  int zc;              // z-chunk of the cell's ring columns (3D)
+  int pad;
   double c[3], ch[3];  // dt / dh, dt / (2 dh)
   double dx, dy, dz;
 };
@@ -34,11 +42,10 @@ struct TileShape {
 };
 // tile shapes of the configuration table (threads = tx * ty).  Shapes
 // NBASE .. 2 NBASE - 1 use the same curvature tiles as shape s - NBASE with
-// the SL step over 32 x 8 x RING_ZC columns staged in an LDS ring.
+// the SL step over 32 x 8 x zc columns staged in an LDS ring.
 constexpr int NBASE = 6;
 constexpr int NSHAPES = 2 * NBASE;
 constexpr int RING_LIST = NBASE;  // index of the ring column list
-constexpr int RING_ZC = 32;
 __host__ __device__ constexpr TileShape shape_of(int s) {
   s = s % NBASE;
   return s == 0 ? TileShape{32, 8, 4}
@@ -50,101 +57,219 @@ __host__ __device__ constexpr TileShape shape_of(int s) {
 }
 constexpr int NOCC = 3;
 constexpr int occ_of(int o) { return o == 0 ? 2 : o == 1 ? 4 : 8; }  // resident blocks per CU
+// 2D tiles: 64 x 4 nodes
+constexpr int T2X = 64, T2Y = 4;
+// cell-size classes (largest axis up to 48, up to 80, beyond) and the ring
+// z-chunks tried for each
+constexpr int NCLASS = DCL_BATCH_CLASSES;
+constexpr int CLASS_EDGE[NCLASS] = {48, 80, 1 << 30};
+constexpr int NZC = DCL_BATCH_ZCHUNKS;
+constexpr int ZC_OPT[NZC] = {16, 32, 64, 128};
 
-__device__ __forceinline__ void tile_range(int ntiles, int& t0, int& t1, int& slot, int& nslot) {
-  const int xcd = blockIdx.x % 8;
-  nslot = gridDim.x / 8;
-  slot = blockIdx.x / 8;
-  const int chunk = (ntiles + 7) / 8;
-  t0 = xcd * chunk;
-  t1 = min(t0 + chunk, ntiles);
+// Work queue of one persistent launch: the tile list is cut into 8
+// contiguous chunks of about equal cost, one per XCD (blocks are dealt to the
+// XCDs round-robin, so block b runs on XCD b mod 8), and the blocks of an XCD
+// claim the tiles of its chunk one at a time with an atomic counter, so that
+// neighbouring columns (which share halo rows) run in the same L2; a block
+// whose chunk is exhausted takes tiles from the other chunks.  The last block
+// to finish resets the counters, so the next launch (or graph replay) starts
+// from zero.
+struct WorkQ {
+  unsigned claim[8];
+  unsigned done;
+  unsigned pad[7];
+};
+
+template <typename F>
+__device__ __forceinline__ void claim_tiles(WorkQ* __restrict__ q, const int* __restrict__ bounds, F body) {
+  __shared__ int s_tile;
+  const bool lead = threadIdx.x == 0 && threadIdx.y == 0;
+  const int xcd = blockIdx.x & 7;
+  for (int s = 0; s < 8; ++s) {
+    const int x = (xcd + s) & 7;
+    const int lo = bounds[x], hi = bounds[x + 1];
+    if (lo >= hi) continue;
+    for (;;) {
+      if (lead) s_tile = lo + (int)atomicAdd(&q->claim[x], 1u);
+      __syncthreads();
+      const int t = s_tile;
+      __syncthreads();
+      if (t >= hi) break;
+      body(t);
+    }
+  }
+  if (lead && atomicAdd(&q->done, 1u) == gridDim.x - 1) {
+    for (int x = 0; x < 8; ++x) atomicExch(&q->claim[x], 0u);
+    atomicExch(&q->done, 0u);
+  }
+}
+
+template <int DIM>
+__device__ __forceinline__ SLP cell_slp(const BCell& c) {
+  SLP P;
+  P.w.px = c.px; P.w.py = c.py; P.w.pz = c.pz;
+  P.w.nx = c.nx; P.w.ny = c.ny; P.w.nz = c.nz;
+  P.v.w = P.w;
+  P.v.dof = DIM;
+  P.v.udof = 0;
+  for (int a = 0; a < 3; ++a) {
+    P.c[a] = c.c[a];
+    P.ch[a] = c.ch[a];
+  }
+  return P;
+}
+
+__device__ __forceinline__ KWin cell_win(const BCell& c) {
+  KWin w;
+  w.px = c.px; w.py = c.py; w.pz = c.pz;
+  w.nx = c.nx; w.ny = c.ny; w.nz = c.nz;
+  return w;
 }
 
 template <typename T, int INTERP, int BT, int S>
 __global__ __launch_bounds__(shape_of(S).tx * shape_of(S).ty) void batch_sl_k(
     const T* __restrict__ phi0, T* __restrict__ phi, const T* __restrict__ vel, const BCell* __restrict__ cells,
-    const int4* __restrict__ tiles, int ntiles) {
+    const int4* __restrict__ tiles, const int* __restrict__ bounds, WorkQ* __restrict__ q) {
   constexpr TileShape sh = shape_of(S);
-  int t0, t1, slot, nslot;
-  tile_range(ntiles, t0, t1, slot, nslot);
-  for (int t = t0 + slot; t < t1; t += nslot) {
+  claim_tiles(q, bounds, [&](int t) {
     const int4 tl = tiles[t];
     const BCell c = cells[tl.x];
-    SLP P;
-    P.w.px = c.px; P.w.py = c.py; P.w.pz = c.pz;
-    P.w.nx = c.nx; P.w.ny = c.ny; P.w.nz = c.nz;
-    P.v.w = P.w;
-    P.v.dof = 3;
-    P.v.udof = 0;
-    for (int a = 0; a < 3; ++a) {
-      P.c[a] = c.c[a];
-      P.ch[a] = c.ch[a];
-    }
+    const SLP P = cell_slp<3>(c);
     const unsigned nodes = (unsigned)c.nx * c.ny * c.nz;
     const rsrc_t rp = make_rsrc(phi0 + c.phi_off, nodes * (unsigned)sizeof(T));
     const rsrc_t rv = make_rsrc(vel + c.vel_off, 3u * nodes * (unsigned)sizeof(T));
     sl_tile<T, INTERP, BT, sh.tx, sh.ty, sh.tz>(rp, rv, phi0 + c.phi_off, phi + c.phi_off, P, tl.y, tl.z, tl.w);
-  }
+  });
 }
 
-// ring columns of the batch: 32 x 8 nodes, velocity staged in FP32
+// ring columns of the batch: 32 x 8 nodes, the cell's z-chunk, velocity staged in FP32
 constexpr int RING_TX = 32, RING_TY = 8;
 
 template <typename T, int INTERP, int BT>
 __global__ __launch_bounds__(256) void batch_ring_k(const T* __restrict__ phi0, T* __restrict__ phi,
                                                       const T* __restrict__ vel, const BCell* __restrict__ cells,
-                                                      const int4* __restrict__ tiles, int ntiles) {
+                                                      const int4* __restrict__ tiles,
+                                                      const int* __restrict__ bounds, WorkQ* __restrict__ q) {
   typedef RingGeom<RING_TX, RING_TY> G;
   typedef VelGeom<RING_TX, RING_TY> V;
   constexpr bool VL = sizeof(T) == 4;
   __shared__ T ring[RG_R * G::PLANE];
   __shared__ T vr[VL ? VR_R * 3 * V::PLANE : 1];
-  int t0, t1, slot, nslot;
-  tile_range(ntiles, t0, t1, slot, nslot);
-  for (int t = t0 + slot; t < t1; t += nslot) {
+  claim_tiles(q, bounds, [&](int t) {
     const int4 tl = tiles[t];
     const BCell c = cells[tl.x];
-    SLP P;
-    P.w.px = c.px; P.w.py = c.py; P.w.pz = c.pz;
-    P.w.nx = c.nx; P.w.ny = c.ny; P.w.nz = c.nz;
-    P.v.w = P.w;
-    P.v.dof = 3;
-    P.v.udof = 0;
-    for (int a = 0; a < 3; ++a) {
-      P.c[a] = c.c[a];
-      P.ch[a] = c.ch[a];
-    }
+    const SLP P = cell_slp<3>(c);
     const unsigned nodes = (unsigned)c.nx * c.ny * c.nz;
     const rsrc_t rp = make_rsrc(phi0 + c.phi_off, nodes * (unsigned)sizeof(T));
     const rsrc_t rv = make_rsrc(vel + c.vel_off, 3u * nodes * (unsigned)sizeof(T));
-    ring_column<T, INTERP, BT, RING_ZC, G, VL>(phi0 + c.phi_off, phi + c.phi_off, rp, rv, P, tl.y, tl.z, tl.w,
-                                               ring, vr);
-  }
+    ring_column<T, INTERP, BT, G, VL>(phi0 + c.phi_off, phi + c.phi_off, rp, rv, P, tl.y, tl.z, tl.w, c.zc, ring,
+                                      vr);
+  });
 }
 
 template <typename T, int S>
 __global__ __launch_bounds__(shape_of(S).tx * shape_of(S).ty) void batch_curv_k(
     const T* __restrict__ phi, T* __restrict__ kappa, const BCell* __restrict__ cells,
-    const int4* __restrict__ tiles, int ntiles) {
+    const int4* __restrict__ tiles, const int* __restrict__ bounds, WorkQ* __restrict__ q) {
   constexpr TileShape sh = shape_of(S);
   __shared__ T pl[4 * (sh.tx + 2) * (sh.ty + 2)];
-  int t0, t1, slot, nslot;
-  tile_range(ntiles, t0, t1, slot, nslot);
-  for (int t = t0 + slot; t < t1; t += nslot) {
+  claim_tiles(q, bounds, [&](int t) {
     const int4 tl = tiles[t];
     const BCell c = cells[tl.x];
-    KWin w;
-    w.px = c.px; w.py = c.py; w.pz = c.pz;
-    w.nx = c.nx; w.ny = c.ny; w.nz = c.nz;
     const unsigned nodes = (unsigned)c.nx * c.ny * c.nz;
     curv_column<T, sh.tx, sh.ty, sh.tz>(make_rsrc(phi + c.phi_off, nodes * (unsigned)sizeof(T)), kappa + c.phi_off,
-                                        w, curv_recips<T>(c.dx, c.dy, c.dz), tl.y, tl.z, tl.w, pl);
+                                        cell_win(c), curv_recips<T>(c.dx, c.dy, c.dz), tl.y, tl.z, tl.w, pl);
+  });
+}
+
+// 2D cells: one node per thread over 64 x 4 tiles; nodes outside [2, n-3]
+// copy phi0 (as the dense 2D step)
+template <typename T, int INTERP, int BT>
+__global__ __launch_bounds__(T2X * T2Y) void batch2d_sl_k(const T* __restrict__ phi0, T* __restrict__ phi,
+                                                         const T* __restrict__ vel, const BCell* __restrict__ cells,
+                                                         const int4* __restrict__ tiles,
+                                                         const int* __restrict__ bounds, WorkQ* __restrict__ q) {
+  claim_tiles(q, bounds, [&](int t) {
+    const int4 tl = tiles[t];
+    const BCell c = cells[tl.x];
+    const int i = tl.y + threadIdx.x, j = tl.z + threadIdx.y;
+    if (i < c.nx && j < c.ny) {
+      const SLP P = cell_slp<2>(c);
+      const int id = j * c.nx + i;
+      const T* p0 = phi0 + c.phi_off;
+      T v;
+      if (i >= 2 && i <= c.nx - 3 && j >= 2 && j <= c.ny - 3)
+        v = sl_point<T, 2, INTERP, BT>(p0, vel + c.vel_off, P, c.px + i, c.py + j, c.pz);
+      else
+        v = p0[id];
+      phi[c.phi_off + id] = v;
+    }
+  });
+}
+
+// GridFunction2D_Curv (Common/Grid.c:231-254) at a local node; 0 where the
+// stencil leaves the window
+template <typename T>
+__device__ __forceinline__ T curv2d_node(const T* __restrict__ p, const BCell& c, int i, int j) {
+  if (i < 1 || i > c.nx - 2 || j < 1 || j > c.ny - 2) return T(0);
+  return curv2d_at<T>(p + j * c.nx + i, c.nx, (T)c.dx, (T)c.dy);
+}
+
+template <typename T>
+__global__ __launch_bounds__(T2X * T2Y) void batch2d_curv_k(const T* __restrict__ phi, T* __restrict__ kappa,
+                                                           const BCell* __restrict__ cells,
+                                                           const int4* __restrict__ tiles,
+                                                           const int* __restrict__ bounds, WorkQ* __restrict__ q) {
+  claim_tiles(q, bounds, [&](int t) {
+    const int4 tl = tiles[t];
+    const BCell c = cells[tl.x];
+    const int i = tl.y + threadIdx.x, j = tl.z + threadIdx.y;
+    if (i < c.nx && j < c.ny) kappa[c.phi_off + j * c.nx + i] = curv2d_node<T>(phi + c.phi_off, c, i, j);
+  });
+}
+
+// Band mode: entry b of the packed list is {x, y, z, cell} in the cell's
+// global coordinates.  The new value is written to phi at the node when the
+// node lies in the cell's window.
+template <typename T, int DIM, int INTERP, int BT>
+__global__ __launch_bounds__(256) void batch_band_sl_k(const T* __restrict__ phi0, T* __restrict__ phi,
+                                                       const T* __restrict__ vel, const BCell* __restrict__ cells,
+                                                       const int4* __restrict__ band, int64_t n) {
+  for (int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x; b < n; b += (int64_t)gridDim.x * 256) {
+    const int4 e = band[b];
+    const BCell c = cells[e.w];
+    const int x = e.x - c.px, y = e.y - c.py, z = DIM == 3 ? e.z - c.pz : 0;
+    if (x < 0 || x >= c.nx || y < 0 || y >= c.ny || z < 0 || z >= (DIM == 3 ? c.nz : 1)) continue;
+    const SLP P = cell_slp<DIM>(c);
+    const T v = sl_point<T, DIM, INTERP, BT>(phi0 + c.phi_off, vel + c.vel_off, P, e.x, e.y, DIM == 3 ? e.z : c.pz);
+    phi[c.phi_off + ((long long)z * c.ny + y) * c.nx + x] = v;
   }
 }
 
-typedef void (*SLKernel)(const void*, void*, const void*, const BCell*, const int4*, int);
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void batch_band_curv_k(const T* __restrict__ phi, T* __restrict__ kappa,
+                                                         const BCell* __restrict__ cells,
+                                                         const int4* __restrict__ band, int64_t n) {
+  for (int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x; b < n; b += (int64_t)gridDim.x * 256) {
+    const int4 e = band[b];
+    const BCell c = cells[e.w];
+    const int x = e.x - c.px, y = e.y - c.py, z = DIM == 3 ? e.z - c.pz : 0;
+    if (x < 0 || x >= c.nx || y < 0 || y >= c.ny || z < 0 || z >= (DIM == 3 ? c.nz : 1)) continue;
+    const T* p = phi + c.phi_off;
+    T k;
+    if (DIM == 3) {
+      const bool in = x >= 1 && x <= c.nx - 2 && y >= 1 && y <= c.ny - 2 && z >= 1 && z <= c.nz - 2;
+      k = in ? curv3d_at<T>(p + ((long long)z * c.ny + y) * c.nx + x, c.nx, c.nx * c.ny,
+                            curv_recips<T>(c.dx, c.dy, c.dz))
+             : T(0);
+    } else {
+      k = curv2d_node<T>(p, c, x, y);
+    }
+    kappa[c.phi_off + ((long long)z * c.ny + y) * c.nx + x] = k;
+  }
+}
 
-// the kernel pointers of one precision / scheme for every shape
+// the kernel pointers of one precision / scheme
 template <typename T, int INTERP, int BT>
 struct KTable {
   static const void* sl(int s) {
@@ -156,6 +281,12 @@ struct KTable {
       case 4: return (const void*)&batch_sl_k<T, INTERP, BT, 4>;
       default: return (const void*)&batch_sl_k<T, INTERP, BT, 5>;
     }
+  }
+  static const void* ring() { return (const void*)&batch_ring_k<T, INTERP, BT>; }
+  static const void* sl2d() { return (const void*)&batch2d_sl_k<T, INTERP, BT>; }
+  static const void* band(int dim) {
+    return dim == 3 ? (const void*)&batch_band_sl_k<T, 3, INTERP, BT>
+                    : (const void*)&batch_band_sl_k<T, 2, INTERP, BT>;
   }
 };
 
@@ -171,20 +302,19 @@ static const void* curv_kernel(int s) {
   }
 }
 
-template <typename T>
-static const void* ring_kernel(int interp, int bt) {
-  if (interp == DCL_CUBIC && bt == DCL_RK2) return (const void*)&batch_ring_k<T, DCL_CUBIC, DCL_RK2>;
-  if (interp == DCL_CUBIC) return (const void*)&batch_ring_k<T, DCL_CUBIC, DCL_EULER>;
-  if (bt == DCL_RK2) return (const void*)&batch_ring_k<T, DCL_LINEAR, DCL_RK2>;
-  return (const void*)&batch_ring_k<T, DCL_LINEAR, DCL_EULER>;
+// SL kernels by kind: 0 tile gathers of shape s, 1 ring columns, 2 2D
+// tiles, 3 band mode of dimension s
+template <typename K>
+static const void* sl_pick(int kind, int s) {
+  return kind == 0 ? K::sl(s) : kind == 1 ? K::ring() : kind == 2 ? K::sl2d() : K::band(s);
 }
 
 template <typename T>
-static const void* sl_kernel(int interp, int bt, int s) {
-  if (interp == DCL_CUBIC && bt == DCL_RK2) return KTable<T, DCL_CUBIC, DCL_RK2>::sl(s);
-  if (interp == DCL_CUBIC) return KTable<T, DCL_CUBIC, DCL_EULER>::sl(s);
-  if (bt == DCL_RK2) return KTable<T, DCL_LINEAR, DCL_RK2>::sl(s);
-  return KTable<T, DCL_LINEAR, DCL_EULER>::sl(s);
+static const void* sl_kernel(int interp, int bt, int kind, int s) {
+  if (interp == DCL_CUBIC && bt == DCL_RK2) return sl_pick<KTable<T, DCL_CUBIC, DCL_RK2>>(kind, s);
+  if (interp == DCL_CUBIC) return sl_pick<KTable<T, DCL_CUBIC, DCL_EULER>>(kind, s);
+  if (bt == DCL_RK2) return sl_pick<KTable<T, DCL_LINEAR, DCL_RK2>>(kind, s);
+  return sl_pick<KTable<T, DCL_LINEAR, DCL_EULER>>(kind, s);
 }
 
 }  // namespace dcl
@@ -192,39 +322,99 @@ static const void* sl_kernel(int interp, int bt, int s) {
 using namespace dcl;
 
 struct dcl_batch_s {
-  int ncells, precision, interp, backtrace;
+  int ncells, precision, interp, backtrace, dim;
   std::vector<dcl_window> wins;
   std::vector<BCell> host_cells;
   std::vector<long long> phi_off, vel_off;
+  std::vector<int> cls;                 // size class of every cell
   BCell* cells = nullptr;
-  int4* tiles[NBASE + 1] = {nullptr};  // per tile shape, then the ring column list
+  int4* tiles[NBASE + 1] = {nullptr};   // per tile shape, then the ring column list (2D: [0])
+  int* bounds[NBASE + 1] = {nullptr};   // 9 chunk bounds (one chunk per XCD) of each tile list
   int ntiles[NBASE + 1] = {0};
+  WorkQ* queues = nullptr;              // [0] the SL launch, [1] the curvature launch
   int shape = 1, occ = 1;
+  int zc[NCLASS] = {32, 32, 32};        // ring z-chunk per size class
   int cus = 256;
   bool dt_set = false;
 };
 
-// tile list s < NBASE: tiles of shape s; s == RING_LIST: 32 x 8 x RING_ZC columns
+static int cell_class(const dcl_window& w) {
+  const int e = std::max(w.n.x, std::max(w.n.y, w.n.z));
+  int c = 0;
+  while (c < NCLASS - 1 && e > CLASS_EDGE[c]) ++c;
+  return c;
+}
+
+static int upload_cells(dcl_batch b) {
+  for (int c = 0; c < b->ncells; ++c) b->host_cells[c].zc = b->zc[b->cls[c]];
+  DCL_HIP(hipMemcpy(b->cells, b->host_cells.data(), sizeof(BCell) * b->ncells, hipMemcpyHostToDevice));
+  return 0;
+}
+
+// tile list s < NBASE: tiles of shape s (2D batches: s = 0, 64 x 4 tiles);
+// s == RING_LIST: 32 x 8 columns of each cell's z-chunk.  Cells come largest
+// first (a block that claims late gets a small tile), each cell's tiles in
+// raster order; the list is cut into 8 contiguous chunks of about equal cost,
+// a tile costing its lanes times the planes it walks plus a halo's worth of
+// planes.
 static int build_tiles(dcl_batch b, int s) {
   if (b->tiles[s]) return 0;
-  const TileShape sh = s == RING_LIST ? TileShape{RING_TX, RING_TY, RING_ZC} : shape_of(s);
+  const bool two = b->dim == 2;
+  std::vector<int> order(b->ncells);
+  for (int c = 0; c < b->ncells; ++c) order[c] = c;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int c) { return win_nodes(b->wins[a]) > win_nodes(b->wins[c]); });
   std::vector<int4> t;
-  for (int c = 0; c < b->ncells; ++c) {
+  std::vector<double> cost;
+  for (int c : order) {
     const dcl_window& w = b->wins[c];
-    for (int z = 0; z < w.n.z; z += sh.tz)
+    const TileShape sh = two                ? TileShape{T2X, T2Y, 1}
+                         : s == RING_LIST ? TileShape{RING_TX, RING_TY, b->zc[b->cls[c]]}
+                                          : shape_of(s);
+    const int halo = two ? 0 : s == RING_LIST ? 2 * RG_H : 2;
+    const int nz = two ? 1 : w.n.z;
+    for (int z = 0; z < nz; z += sh.tz)
       for (int y = 0; y < w.n.y; y += sh.ty)
-        for (int x = 0; x < w.n.x; x += sh.tx) t.push_back(make_int4(c, x, y, z));
+        for (int x = 0; x < w.n.x; x += sh.tx) {
+          t.push_back(make_int4(c, x, y, z));
+          cost.push_back((double)sh.tx * sh.ty * (std::min(sh.tz, nz - z) + halo));
+        }
   }
   if (t.size() >= (size_t)1 << 31) return set_error(DCL_ERR_ARG_SIZ, "too many tiles");
+  double total = 0;
+  for (double c : cost) total += c;
+  int bnd[9];
+  bnd[0] = 0;
+  double acc = 0;
+  size_t i = 0;
+  for (int x = 1; x < 8; ++x) {
+    while (i < t.size() && acc + 0.5 * cost[i] < total * x / 8.0) acc += cost[i++];
+    bnd[x] = (int)i;
+  }
+  bnd[8] = (int)t.size();
   void* p = nullptr;
-  DCL_TRY(dcl_malloc(&p, sizeof(int4) * t.size()));
+  DCL_TRY(dcl_malloc(&p, sizeof(int4) * t.size() + sizeof(bnd)));
   hipError_t e = hipMemcpy(p, t.data(), sizeof(int4) * t.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(static_cast<char*>(p) + sizeof(int4) * t.size(), bnd, sizeof(bnd), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     dcl_free(p);
     return hip_fail(e, "tile list upload", __FILE__, __LINE__);
   }
   b->tiles[s] = static_cast<int4*>(p);
+  b->bounds[s] = reinterpret_cast<int*>(static_cast<char*>(p) + sizeof(int4) * t.size());
   b->ntiles[s] = (int)t.size();
+  return 0;
+}
+
+static int drop_tiles(dcl_batch b, int s) {
+  if (b->tiles[s]) {
+    DCL_HIP(hipDeviceSynchronize());
+    dcl_free(b->tiles[s]);
+  }
+  b->tiles[s] = nullptr;
+  b->bounds[s] = nullptr;
+  b->ntiles[s] = 0;
   return 0;
 }
 
@@ -240,32 +430,46 @@ struct Plan {
   dim3 grid, bsl, bcv;
   const int4* tsl;
   const int4* tcv;
-  int nsl, ncv;
+  const int* bsl_bounds;
+  const int* bcv_bounds;
+  WorkQ* qsl;
+  WorkQ* qcv;
 };
 
 static int make_plan(dcl_batch b, int s, int occ, Plan& p) {
+  const bool f64 = b->precision == DCL_F64;
+  p.grid = dim3(grid_blocks(b, occ));
+  p.qsl = b->queues;
+  p.qcv = b->queues + 1;
+  if (b->dim == 2) {
+    DCL_TRY(build_tiles(b, 0));
+    p.bsl = p.bcv = dim3(T2X, T2Y);
+    p.tsl = p.tcv = b->tiles[0];
+    p.bsl_bounds = p.bcv_bounds = b->bounds[0];
+    p.ksl = f64 ? sl_kernel<double>(b->interp, b->backtrace, 2, 0) : sl_kernel<float>(b->interp, b->backtrace, 2, 0);
+    p.kcv = f64 ? (const void*)&batch2d_curv_k<double> : (const void*)&batch2d_curv_k<float>;
+    return 0;
+  }
   const int cs = s % NBASE;
   const bool ring = s >= NBASE;
   DCL_TRY(build_tiles(b, cs));
   if (ring) DCL_TRY(build_tiles(b, RING_LIST));
   const TileShape sh = shape_of(cs);
-  const bool f64 = b->precision == DCL_F64;
-  p.grid = dim3(grid_blocks(b, occ));
   p.bcv = dim3(sh.tx, sh.ty);
   p.tcv = b->tiles[cs];
-  p.ncv = b->ntiles[cs];
+  p.bcv_bounds = b->bounds[cs];
   p.kcv = f64 ? curv_kernel<double>(cs) : curv_kernel<float>(cs);
   if (ring) {
-    p.ksl = f64 ? ring_kernel<double>(b->interp, b->backtrace) : ring_kernel<float>(b->interp, b->backtrace);
+    p.ksl = f64 ? sl_kernel<double>(b->interp, b->backtrace, 1, 0) : sl_kernel<float>(b->interp, b->backtrace, 1, 0);
     p.bsl = dim3(RING_TX, RING_TY);
     p.tsl = b->tiles[RING_LIST];
-    p.nsl = b->ntiles[RING_LIST];
+    p.bsl_bounds = b->bounds[RING_LIST];
   } else {
-    p.ksl = f64 ? sl_kernel<double>(b->interp, b->backtrace, cs)
-                : sl_kernel<float>(b->interp, b->backtrace, cs);
+    p.ksl = f64 ? sl_kernel<double>(b->interp, b->backtrace, 0, cs)
+                : sl_kernel<float>(b->interp, b->backtrace, 0, cs);
     p.bsl = p.bcv;
     p.tsl = p.tcv;
-    p.nsl = p.ncv;
+    p.bsl_bounds = p.bcv_bounds;
   }
   return 0;
 }
@@ -276,10 +480,11 @@ static int batch_launch(dcl_batch b, int s, int occ, const void* phi0, void* phi
   Plan p;
   DCL_TRY(make_plan(b, s, occ, p));
   const BCell* cells = b->cells;
-  void* a1[] = {(void*)&phi0, (void*)&phi, (void*)&vel, (void*)&cells, (void*)&p.tsl, (void*)&p.nsl};
+  void* a1[] = {(void*)&phi0, (void*)&phi, (void*)&vel, (void*)&cells, (void*)&p.tsl, (void*)&p.bsl_bounds,
+                (void*)&p.qsl};
   DCL_HIP(hipLaunchKernel(p.ksl, p.grid, p.bsl, a1, 0, st));
   const void* cphi = phi;
-  void* a2[] = {(void*)&cphi, (void*)&kappa, (void*)&cells, (void*)&p.tcv, (void*)&p.ncv};
+  void* a2[] = {(void*)&cphi, (void*)&kappa, (void*)&cells, (void*)&p.tcv, (void*)&p.bcv_bounds, (void*)&p.qcv};
   DCL_HIP(hipLaunchKernel(p.kcv, p.grid, p.bcv, a2, 0, st));
   return 0;
 }
@@ -293,12 +498,15 @@ extern "C" int dcl_batch_create(int32_t ncells, const dcl_window* wins, int prec
   if (!wins) return set_error(DCL_ERR_ARG_NULL, "wins is NULL");
   if (interp != DCL_LINEAR && interp != DCL_CUBIC) return set_error(DCL_ERR_ARG_WRONG, "bad interp");
   if (backtrace != DCL_EULER && backtrace != DCL_RK2) return set_error(DCL_ERR_ARG_WRONG, "bad backtrace");
+  const int dim = wins[0].n.z < 2 ? 2 : 3;
   for (int c = 0; c < ncells; ++c) {
-    DCL_TRY(check_window(&wins[c], "cell", 5, 3));
-    if (wins[c].n.z < 2) return set_error(DCL_ERR_SUP, "cell %d is 2D; batches are 3D", c);
-    // the kernels address a cell's velocity through a buffer descriptor whose
-    // byte size is 32-bit
-    const int64_t vbytes = 3 * win_nodes(wins[c]) * (precision == DCL_F64 ? 8 : 4);
+    DCL_TRY(check_window(&wins[c], "cell", 5, dim));
+    if ((wins[c].n.z < 2 ? 2 : 3) != dim)
+      return set_error(DCL_ERR_SUP, "cell %d is %dD, cell 0 %dD: a batch holds cells of one dimension", c,
+                       wins[c].n.z < 2 ? 2 : 3, dim);
+    // the 3D kernels address a cell's velocity through a buffer descriptor
+    // whose byte size is 32-bit
+    const int64_t vbytes = dim * win_nodes(wins[c]) * (precision == DCL_F64 ? 8 : 4);
     if (vbytes >= ((int64_t)1 << 32))
       return set_error(DCL_ERR_ARG_SIZ, "cell %d: its velocity takes %lld bytes; the limit is 2^32 - 1", c,
                        (long long)vbytes);
@@ -309,6 +517,7 @@ extern "C" int dcl_batch_create(int32_t ncells, const dcl_window* wins, int prec
   b->precision = precision;
   b->interp = interp;
   b->backtrace = backtrace;
+  b->dim = dim;
   b->wins.assign(wins, wins + ncells);
   long long po = 0, vo = 0;
   for (int c = 0; c < ncells; ++c) {
@@ -318,13 +527,14 @@ extern "C" int dcl_batch_create(int32_t ncells, const dcl_window* wins, int prec
     cell.phi_off = po;
     cell.vel_off = vo;
     cell.px = w.p.x; cell.py = w.p.y; cell.pz = w.p.z;
-    cell.nx = w.n.x; cell.ny = w.n.y; cell.nz = w.n.z;
+    cell.nx = w.n.x; cell.ny = w.n.y; cell.nz = dim == 2 ? 1 : w.n.z;
     cell.dx = w.d.x; cell.dy = w.d.y; cell.dz = w.d.z;
     b->phi_off.push_back(po);
     b->vel_off.push_back(vo);
     b->host_cells.push_back(cell);
+    b->cls.push_back(cell_class(w));
     po += win_nodes(w);
-    vo += 3 * win_nodes(w);
+    vo += dim * win_nodes(w);
   }
   b->phi_off.push_back(po);
   b->vel_off.push_back(vo);
@@ -335,12 +545,20 @@ extern "C" int dcl_batch_create(int32_t ncells, const dcl_window* wins, int prec
       b->cus = cus;
   }
   void* p = nullptr;
-  int rc = dcl_malloc(&p, sizeof(BCell) * ncells);
+  int rc = dcl_malloc(&p, sizeof(BCell) * ncells + 2 * sizeof(WorkQ));
   if (rc) {
     delete b;
     return rc;
   }
   b->cells = static_cast<BCell*>(p);
+  b->queues = reinterpret_cast<WorkQ*>(static_cast<char*>(p) + sizeof(BCell) * ncells);
+  // the queues start at zero; each launch's last block leaves them at zero
+  const hipError_t e = hipMemset(b->queues, 0, 2 * sizeof(WorkQ));
+  if (e != hipSuccess) {
+    dcl_free(p);
+    delete b;
+    return hip_fail(e, "work queue reset", __FILE__, __LINE__);
+  }
   *out = b;
   return 0;
 }
@@ -373,7 +591,7 @@ extern "C" int dcl_batch_set_dt(dcl_batch b, const double* dt) {
       cell.ch[a] = dt[c] / (2.0 * d[a]);
     }
   }
-  DCL_HIP(hipMemcpy(b->cells, b->host_cells.data(), sizeof(BCell) * b->ncells, hipMemcpyHostToDevice));
+  DCL_TRY(upload_cells(b));
   b->dt_set = true;
   return 0;
 }
@@ -392,13 +610,39 @@ extern "C" int dcl_batch_get_config(dcl_batch b, int* shape, int* occupancy, int
   if (!b) return set_error(DCL_ERR_ARG_NULL, "batch is NULL");
   if (shape) *shape = b->shape;
   if (occupancy) *occupancy = b->occ;
-  const TileShape sh = shape_of(b->shape);
+  const TileShape sh = b->dim == 2 ? TileShape{T2X, T2Y, 1} : shape_of(b->shape);
   if (tile) {
     tile[0] = sh.tx;
     tile[1] = sh.ty;
     tile[2] = sh.tz;
   }
   if (blocks) *blocks = grid_blocks(b, b->occ);
+  return 0;
+}
+
+static int set_zc(dcl_batch b, int cls, int zc) {
+  if (b->zc[cls] == zc) return 0;
+  b->zc[cls] = zc;
+  DCL_TRY(drop_tiles(b, RING_LIST));
+  if (b->dt_set) DCL_TRY(upload_cells(b));
+  return 0;
+}
+
+extern "C" int dcl_batch_set_class_zchunk(dcl_batch b, int cls, int zchunk) {
+  if (!b) return set_error(DCL_ERR_ARG_NULL, "batch is NULL");
+  if (cls < 0 || cls >= NCLASS) return set_error(DCL_ERR_ARG_WRONG, "class %d out of range", cls);
+  if (zchunk < 1 || zchunk > 4096) return set_error(DCL_ERR_ARG_WRONG, "z-chunk %d out of range", zchunk);
+  return set_zc(b, cls, zchunk);
+}
+
+extern "C" int dcl_batch_get_classes(dcl_batch b, int32_t max_edge[DCL_BATCH_CLASSES],
+                                     int32_t cells[DCL_BATCH_CLASSES], int32_t zchunk[DCL_BATCH_CLASSES]) {
+  if (!b) return set_error(DCL_ERR_ARG_NULL, "batch is NULL");
+  for (int k = 0; k < NCLASS; ++k) {
+    if (max_edge) max_edge[k] = CLASS_EDGE[k] < (1 << 30) ? CLASS_EDGE[k] : -1;
+    if (cells) cells[k] = (int32_t)std::count(b->cls.begin(), b->cls.end(), k);
+    if (zchunk) zchunk[k] = b->zc[k];
+  }
   return 0;
 }
 
@@ -417,45 +661,94 @@ extern "C" int dcl_batch_step(dcl_batch b, const void* phi0, void* phi, void* ka
   return batch_launch(b, b->shape, b->occ, phi0, phi, kappa, vel, as_stream(stream));
 }
 
-extern "C" int dcl_batch_autotune(dcl_batch b, const void* phi0, void* phi, void* kappa, const void* vel,
-                                  int reps, double* ms_table, double* best_ms, void* stream) {
+extern "C" int dcl_batch_band_step(dcl_batch b, const void* phi0, void* phi, void* kappa, const void* vel,
+                                   const int32_t* band, int64_t count, void* stream) {
   DCL_TRY(check_step(b, phi0, phi, kappa, vel));
-  if (reps < 1) return set_error(DCL_ERR_ARG_SIZ, "reps < 1");
+  if (count < 0) return set_error(DCL_ERR_ARG_SIZ, "count < 0");
+  if (count == 0) return 0;
+  if (!band) return set_error(DCL_ERR_ARG_NULL, "band is NULL");
+  const bool f64 = b->precision == DCL_F64;
   hipStream_t st = as_stream(stream);
-  hipEvent_t e0, e1;
-  DCL_HIP(hipEventCreate(&e0));
-  DCL_HIP(hipEventCreate(&e1));
+  const int64_t nb64 = (count + 255) / 256;
+  const unsigned nb = (unsigned)std::min<int64_t>(nb64, (int64_t)b->cus * 16);
+  const void* ksl = f64 ? sl_kernel<double>(b->interp, b->backtrace, 3, b->dim)
+                        : sl_kernel<float>(b->interp, b->backtrace, 3, b->dim);
+  const void* kcv = f64 ? (b->dim == 3 ? (const void*)&batch_band_curv_k<double, 3>
+                                       : (const void*)&batch_band_curv_k<double, 2>)
+                        : (b->dim == 3 ? (const void*)&batch_band_curv_k<float, 3>
+                                       : (const void*)&batch_band_curv_k<float, 2>);
+  const BCell* cells = b->cells;
+  const int4* bd = reinterpret_cast<const int4*>(band);
+  void* a1[] = {(void*)&phi0, (void*)&phi, (void*)&vel, (void*)&cells, (void*)&bd, (void*)&count};
+  DCL_HIP(hipLaunchKernel(ksl, dim3(nb), dim3(256), a1, 0, st));
+  const void* cphi = phi;
+  void* a2[] = {(void*)&cphi, (void*)&kappa, (void*)&cells, (void*)&bd, (void*)&count};
+  DCL_HIP(hipLaunchKernel(kcv, dim3(nb), dim3(256), a2, 0, st));
+  return 0;
+}
+
+// per-step time of configuration (s, occ) over reps steps after one warm-up step
+static int time_config(dcl_batch b, int s, int o, const void* phi0, void* phi, void* kappa, const void* vel,
+                       int reps, hipStream_t st, hipEvent_t e0, hipEvent_t e1, double* per) {
+  DCL_TRY(batch_launch(b, s, o, phi0, phi, kappa, vel, st));  // warm-up (and tile list build)
+  DCL_HIP(hipEventRecord(e0, st));
+  for (int r = 0; r < reps; ++r) DCL_TRY(batch_launch(b, s, o, phi0, phi, kappa, vel, st));
+  DCL_HIP(hipEventRecord(e1, st));
+  DCL_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  DCL_HIP(hipEventElapsedTime(&ms, e0, e1));
+  *per = ms / reps;
+  return 0;
+}
+
+static int autotune(dcl_batch b, const void* phi0, void* phi, void* kappa, const void* vel, int reps,
+                    double* ms_table, double* zc_table, double* best_ms, hipStream_t st, hipEvent_t e0,
+                    hipEvent_t e1) {
   double best = 1e300;
   int bs = b->shape, bo = b->occ;
-  int rc = 0;
-  for (int s = 0; s < NSHAPES && !rc; ++s)
-    for (int o = 0; o < NOCC && !rc; ++o) {
-      rc = batch_launch(b, s, o, phi0, phi, kappa, vel, st);  // warm-up (and tile list build)
-      if (rc) break;
-      if (hipEventRecord(e0, st) != hipSuccess) { rc = set_error(DCL_ERR_LIB, "event record"); break; }
-      for (int r = 0; r < reps && !rc; ++r) rc = batch_launch(b, s, o, phi0, phi, kappa, vel, st);
-      if (rc) break;
-      float ms = 0;
-      if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-        rc = set_error(DCL_ERR_LIB, "event timing failed");
-        break;
-      }
-      const double per = ms / reps;
+  const int nshapes = b->dim == 2 ? 1 : NSHAPES;
+  for (int s = 0; s < NSHAPES; ++s)
+    for (int o = 0; o < NOCC; ++o) {
+      double per = -1;
+      if (s < nshapes) DCL_TRY(time_config(b, s, o, phi0, phi, kappa, vel, reps, st, e0, e1, &per));
       if (ms_table) ms_table[s * NOCC + o] = per;
-      if (per < best) {
+      if (s < nshapes && per < best) {
         best = per;
         bs = s;
         bo = o;
       }
     }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  if (rc) return rc;
   b->shape = bs;
   b->occ = bo;
+  // with ring columns: the z-chunk of each size class, one class at a time
+  // with the others at their current choice
+  for (int k = 0; k < NCLASS; ++k)
+    for (int z = 0; z < NZC; ++z) {
+      if (zc_table) zc_table[k * NZC + z] = -1;
+      if (b->dim != 3 || bs < NBASE || std::count(b->cls.begin(), b->cls.end(), k) == 0) continue;
+      const int keep = b->zc[k];
+      DCL_TRY(set_zc(b, k, ZC_OPT[z]));
+      double per = 0;
+      DCL_TRY(time_config(b, bs, bo, phi0, phi, kappa, vel, reps, st, e0, e1, &per));
+      if (zc_table) zc_table[k * NZC + z] = per;
+      if (per < best) best = per;
+      else DCL_TRY(set_zc(b, k, keep));
+    }
   if (best_ms) *best_ms = best;
   return 0;
+}
+
+extern "C" int dcl_batch_autotune(dcl_batch b, const void* phi0, void* phi, void* kappa, const void* vel,
+                                  int reps, double* ms_table, double* zc_table, double* best_ms, void* stream) {
+  DCL_TRY(check_step(b, phi0, phi, kappa, vel));
+  if (reps < 1) return set_error(DCL_ERR_ARG_SIZ, "reps < 1");
+  hipEvent_t e0, e1;
+  DCL_HIP(hipEventCreate(&e0));
+  DCL_HIP(hipEventCreate(&e1));
+  const int rc = autotune(b, phi0, phi, kappa, vel, reps, ms_table, zc_table, best_ms, as_stream(stream), e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
 }
 
 extern "C" int dcl_batch_graph_create(dcl_batch b, const void* phi0, void* phi, void* kappa, const void* vel,
@@ -466,8 +759,9 @@ extern "C" int dcl_batch_graph_create(dcl_batch b, const void* phi0, void* phi, 
   DCL_TRY(make_plan(b, b->shape, b->occ, p));
   const BCell* cells = b->cells;
   const void* cphi = phi;
-  void* a1[] = {(void*)&phi0, (void*)&phi, (void*)&vel, (void*)&cells, (void*)&p.tsl, (void*)&p.nsl};
-  void* a2[] = {(void*)&cphi, (void*)&kappa, (void*)&cells, (void*)&p.tcv, (void*)&p.ncv};
+  void* a1[] = {(void*)&phi0, (void*)&phi, (void*)&vel, (void*)&cells, (void*)&p.tsl, (void*)&p.bsl_bounds,
+                (void*)&p.qsl};
+  void* a2[] = {(void*)&cphi, (void*)&kappa, (void*)&cells, (void*)&p.tcv, (void*)&p.bcv_bounds, (void*)&p.qcv};
   hipGraph_t g;
   DCL_HIP(hipGraphCreate(&g, 0));
   hipKernelNodeParams p1;

@@ -240,16 +240,17 @@ __device__ __forceinline__ void sl_dep3_ring(rsrc_t rv, const T* vr, const int s
 }
 
 // the SL step of one column: nodes (x0 .. x0+TX-1, y0 .. y0+TY-1, z0 ..
-// z0+ZC-1) of the window, clipped to it; nodes outside [2, n-3] copy phi0.
+// z0+zc-1) of the window, clipped to it; nodes outside [2, n-3] copy phi0.
 // Every thread of the block calls it (it holds barriers) and it ends on a
 // barrier, so a block may run it for one column after another with the same
 // ring (RG_R * G::PLANE values).
-template <typename T, int INTERP, int BT, int ZC, typename G, bool VL>
+template <typename T, int INTERP, int BT, typename G, bool VL>
 __device__ __forceinline__ void ring_column(const T* __restrict__ phi0, T* __restrict__ phi, rsrc_t rp,
-                                            rsrc_t rv, const SLP& P, int x0, int y0, int z0, T* ring, T* vr) {
+                                            rsrc_t rv, const SLP& P, int x0, int y0, int z0, int zc, T* ring,
+                                            T* vr) {
   typedef VelGeom<G::TX, G::TY> V;
   const int nx = P.w.nx, ny = P.w.ny, nz = P.w.nz;
-  const int z1 = min(nz, z0 + ZC);
+  const int z1 = min(nz, z0 + zc);
   const int tid = threadIdx.x + G::TX * threadIdx.y;
   // bytes per plane of phi0 (a third of velocity's); products are taken
   // unsigned, the buffer offsets being 32-bit (windows below 4 GiB)
@@ -322,8 +323,8 @@ __global__ __launch_bounds__(TX * TY) void sl_ring3d(const T* __restrict__ phi0,
     if (per != 0 && lb < per * 8u) lb = (lb % 8u) * per + lb / 8u;
   }
   const unsigned tx = lb % tiles_x, ty = (lb / tiles_x) % tiles_y, zc = lb / (tiles_x * tiles_y);
-  ring_column<T, INTERP, BT, ZC, G, VL>(phi0, phi, make_rsrc(phi0, pbytes), make_rsrc(vel, vbytes), P,
-                                        (int)tx * TX, (int)ty * TY, (int)zc * ZC, ring, vr);
+  ring_column<T, INTERP, BT, G, VL>(phi0, phi, make_rsrc(phi0, pbytes), make_rsrc(vel, vbytes), P, (int)tx * TX,
+                                    (int)ty * TY, (int)zc * ZC, ZC, ring, vr);
 }
 
 }  // namespace dcl

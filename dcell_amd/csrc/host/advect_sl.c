@@ -35,24 +35,36 @@ static void scatter_band(Grid g, const iCoor *band, int len, const PetscReal *va
   }
 }
 
-/* the band loop of LevelSetAdvectSL_2D / _3D (AdvectSL.c:23-89) */
+/* The band loop of LevelSetAdvectSL_2D / _3D (AdvectSL.c:23-89).  Only the
+ * part of phi0 and of the velocity grid that the band's departures can reach
+ * goes to the GPU: the band's bounding box grown by a margin that covers
+ * dt max|v| / dh (max over the uploaded velocity box, which every
+ * interpolated velocity is a convex combination of) plus the widest stencil,
+ * so every stencil the kernels read lies inside the boxes and the results are
+ * those of the whole windows (where a box meets a window edge, it ends there
+ * as the window does). */
 static PetscErrorCode advect_band(LevelSet ls, Grid velgrid, PetscReal dt, const char *where) {
   const int len = ArrayLength(ls->band);
   const iCoor *band = (const iCoor *)ArrayGetData(ls->band);
   void *dphi0, *dvel, *dband, *dout, *hout;
   if (len == 0) return 0;
-  DL_CHECK(where, dl_upload_grid(DL_PHI0, ls->phi0, &dphi0));
-  DL_CHECK(where, dl_upload_grid(DL_VEL, velgrid, &dvel));
+  const int dim = ls->phi0->is2D ? 2 : 3;
+  dl_box bb, pb;
+  dl_bbox(band, len, sizeof(iCoor), dim, &bb);
+  dcl_window vw, pw;
+  int M = 0;
+  DL_CHECK(where, dl_velocity_box(velgrid, &bb, dt, ls->phi0->d, dim, &dvel, &vw, &M));
+  if (!dl_box_around(ls->phi0, &bb, M, 0, &pb)) return dl_fail(where, DCL_ERR_ARG_SIZ);
+  DL_CHECK(where, dl_upload_box(DL_PHI0, ls->phi0, &pb, &dphi0, &pw, NULL, 0));
   DL_CHECK(where, dl_slot(DL_BAND, sizeof(iCoor) * (size_t)len, &dband));
   DL_CHECK(where, dcl_memcpy_htod(dband, band, sizeof(iCoor) * (size_t)len, NULL));
   DL_CHECK(where, dl_slot(DL_OUT, sizeof(PetscReal) * (size_t)len, &dout));
-  const dcl_window w = dl_window(ls->phi0);
   dcl_velocity V;
   V.data = dvel;
-  V.win = dl_window(velgrid);
+  V.win = vw;
   V.dof = velgrid->dof;
   V.udof = 0;
-  DL_CHECK(where, dcl_sl_advect(dphi0, NULL, &w, &V, (const dcl_icoor *)dband, len, dout, dt, DCL_F64,
+  DL_CHECK(where, dcl_sl_advect(dphi0, NULL, &pw, &V, (const dcl_icoor *)dband, len, dout, dt, DCL_F64,
                                 g_interp, g_backtrace, NULL));
   DL_CHECK(where, dl_host(sizeof(PetscReal) * (size_t)len, &hout));
   DL_CHECK(where, dcl_memcpy_dtoh(hout, dout, sizeof(PetscReal) * (size_t)len, NULL));
@@ -64,11 +76,14 @@ PetscErrorCode LevelSetAdvectSL(LevelSet ls, Grid velgrid, PetscReal dt) {
   PetscErrorCode ierr;
   PetscFunctionBegin;
   ierr = PetscLogEventBegin(EVENT_LevelSetAdvectSL, 0, 0, 0, 0); CHKERRQ(ierr);
+  dl_scope_begin();
   if (ls->phi->is2D) {
-    ierr = LevelSetAdvectSL_2D(ls, velgrid, dt); CHKERRQ(ierr);
+    ierr = LevelSetAdvectSL_2D(ls, velgrid, dt);
   } else {
-    ierr = LevelSetAdvectSL_3D(ls, velgrid, dt); CHKERRQ(ierr);
+    ierr = LevelSetAdvectSL_3D(ls, velgrid, dt);
   }
+  dl_scope_end();
+  CHKERRQ(ierr);
   ierr = PetscLogEventEnd(EVENT_LevelSetAdvectSL, 0, 0, 0, 0); CHKERRQ(ierr);
   PetscFunctionReturn(0);
 }
@@ -100,8 +115,9 @@ PetscErrorCode LevelSetAdvectSLRK2FullStep(LevelSet ls, Grid velgrid, PetscReal 
   PetscFunctionReturn(0);
 }
 
-/* Advect.c:33-50 */
-PetscErrorCode LevelSetAdvectAndReinit(LevelSet ls, Grid velgrid, PetscReal dt) {
+/* Advect.c:33-50; the velocity grid is uploaded once for the SL step and
+ * the CFL count */
+static PetscErrorCode advect_and_reinit(LevelSet ls, Grid velgrid, PetscReal dt) {
   PetscErrorCode ierr;
   PetscFunctionBegin;
   ierr = GridCopy(ls->phi, ls->phi0); CHKERRQ(ierr);
@@ -116,6 +132,13 @@ PetscErrorCode LevelSetAdvectAndReinit(LevelSet ls, Grid velgrid, PetscReal dt) 
   PetscFunctionReturn(0);
 }
 
+PetscErrorCode LevelSetAdvectAndReinit(LevelSet ls, Grid velgrid, PetscReal dt) {
+  dl_scope_begin();
+  PetscErrorCode ierr = advect_and_reinit(ls, velgrid, dt);
+  dl_scope_end();
+  return ierr;
+}
+
 /* Advect.c:54-66 */
 PetscErrorCode LevelSetCFLIncrement(LevelSet ls, Grid velgrid, PetscReal dt) {
   Coor dh = ls->phi->d;
@@ -127,7 +150,9 @@ PetscErrorCode LevelSetCFLIncrement(LevelSet ls, Grid velgrid, PetscReal dt) {
   PetscFunctionReturn(0);
 }
 
-/* Advect.c:70-111: max |V| over the irregular nodes, on the GPU */
+/* Advect.c:70-111: max |V| over the irregular nodes, on the GPU; the
+ * velocity grid over the nodes' box (their node velocities read the faces at
+ * the node and one above) */
 PetscErrorCode LevelSet_MaxVelocity(LevelSet ls, Grid velgrid) {
   const int len = ArrayLength(ls->irregularNodes);
   const IrregularNode *nodes = (const IrregularNode *)ArrayGetData(ls->irregularNodes);
@@ -135,19 +160,23 @@ PetscErrorCode LevelSet_MaxVelocity(LevelSet ls, Grid velgrid) {
   double maxVel = 0;
   ls->maxVel = 0;
   if (len == 0) return 0;
+  const int dim = ls->phi->is2D ? 2 : 3;
   DL_CHECK("LevelSet_MaxVelocity", dl_host(sizeof(iCoor) * (size_t)len, &hpos));
   iCoor *pos = (iCoor *)hpos;
   for (int i = 0; i < len; ++i) pos[i] = nodes[i].pos;
   DL_CHECK("LevelSet_MaxVelocity", dl_slot(DL_NODES, sizeof(iCoor) * (size_t)len, &dnodes));
   DL_CHECK("LevelSet_MaxVelocity", dcl_memcpy_htod(dnodes, pos, sizeof(iCoor) * (size_t)len, NULL));
-  DL_CHECK("LevelSet_MaxVelocity", dl_upload_grid(DL_VEL, velgrid, &dvel));
+  dl_box bb;
+  dl_bbox(pos, len, sizeof(iCoor), dim, &bb);
+  dcl_window vw;
+  int M = 0;
+  DL_CHECK("LevelSet_MaxVelocity", dl_velocity_box(velgrid, &bb, 0.0, ls->phi->d, dim, &dvel, &vw, &M));
   dcl_velocity V;
   V.data = dvel;
-  V.win = dl_window(velgrid);
+  V.win = vw;
   V.dof = velgrid->dof;
   V.udof = 0;
-  DL_CHECK("LevelSet_MaxVelocity", dcl_max_velocity(&V, dnodes, len, sizeof(iCoor),
-                                                    ls->phi->is2D ? 2 : 3, DCL_F64, &maxVel, NULL));
+  DL_CHECK("LevelSet_MaxVelocity", dcl_max_velocity(&V, dnodes, len, sizeof(iCoor), dim, DCL_F64, &maxVel, NULL));
   ls->maxVel = maxVel;
   return 0;
 }

@@ -40,6 +40,38 @@ int dl_upload_grid(int slot, Grid g, void **dev);
 /* nodes of a window */
 long dl_nodes(Grid g);
 
+/* An index box of global node coordinates, bounds inclusive. */
+typedef struct {
+  int lo[3], hi[3];
+} dl_box;
+
+/* the bounding box of n coordinate triples (int32, `stride` bytes apart) */
+void dl_bbox(const void *coords, long n, size_t stride, int dim, dl_box *b);
+/* b grown by m nodes below and m + extra_hi above along the grid's axes,
+ * clipped to g's window; returns 0 if the result is empty */
+int dl_box_around(Grid g, const dl_box *b, int m, int extra_hi, dl_box *out);
+/* Upload the box of g (every dof of its nodes) into a slot, packed as a
+ * window of its own (its p = box lo), through page-locked staging with the
+ * copy of each packed chunk overlapping the packing of the next.  *w gets the
+ * box's window (spacing of g).  With maxabs, the largest |value| of each of
+ * the first ncomp components over the box.  Returns when the copies are
+ * enqueued on the default stream. */
+int dl_upload_box(int slot, Grid g, const dl_box *b, void **dev, dcl_window *w, double *maxabs, int ncomp);
+
+/* A scope around a public entry point: within the outermost one, the
+ * velocity grid that dl_velocity_box uploads is reused by later calls (the
+ * reference does not change the velocity grid inside one of its steps). */
+void dl_scope_begin(void);
+void dl_scope_end(void);
+/* The velocity grid over a box around the nodes of b that every departure
+ * of a step of length dt from those nodes stays inside (the box grows until
+ * its margin covers dt max|v| / dh plus the widest stencil), uploaded once
+ * per scope; *M gets the margin used.  dh: the level set's spacing. */
+int dl_velocity_box(Grid velgrid, const dl_box *b, double dt, Coor dh, int dim, void **dev, dcl_window *w,
+                    int *M);
+/* forget the velocity upload (the slot is about to hold something else) */
+void dl_velocity_invalidate(void);
+
 /* report a failed C-ABI call on stderr (the reference's callers drop the
  * return code of ls->Advect, LevelSetMethod/Advect.c:26) and pass it on */
 int dl_fail(const char *where, int rc);

@@ -20,10 +20,17 @@ static PetscErrorCode append_records(LevelSet ls, const char *where) {
   const int len = ArrayLength(ls->band);
   if (len == 0) return 0;
   void *dphi, *dband, *dout, *hrec;
-  DL_CHECK(where, dl_upload_grid(DL_PHI, ls->phi, &dphi));
+  /* phi over the band's box grown by 4 nodes: a record reads the node's
+   * 3x3x3 neighbourhood and its projection the 5x5x5 one, so the box holds
+   * every value read, and a node is as far from the box's edge as from the
+   * window's wherever the box ends inside the window */
+  dl_box bb, pb;
+  dl_bbox(ArrayGetData(ls->band), len, sizeof(iCoor), ls->phi->is2D ? 2 : 3, &bb);
+  if (!dl_box_around(ls->phi, &bb, 4, 0, &pb)) return dl_fail(where, DCL_ERR_ARG_SIZ);
+  dcl_window w;
+  DL_CHECK(where, dl_upload_box(DL_PHI, ls->phi, &pb, &dphi, &w, NULL, 0));
   DL_CHECK(where, dl_slot(DL_BAND, sizeof(iCoor) * (size_t)len, &dband));
   DL_CHECK(where, dcl_memcpy_htod(dband, ArrayGetData(ls->band), sizeof(iCoor) * (size_t)len, NULL));
-  const dcl_window w = dl_window(ls->phi);
   /* at most 1 projection + 3 (2) crossings + 6 (2) face crossings per node */
   const int64_t cap = 10 * (int64_t)len;
   DL_CHECK(where, dl_slot(DL_IRREG, sizeof(dcl_irregular_node) * (size_t)cap, &dout));

@@ -6,6 +6,8 @@
  * through the C ABI (dcell_lsm.h).  struct _Particle {Coor X; PetscReal
  * radius} is dcl_particle, so the particle array is uploaded as it is.
  */
+#include <math.h>
+
 #include "LevelSetMethod.h"
 #include "LSM_private.h"
 #include "dcell_gpu.h"
@@ -45,6 +47,29 @@ static PetscErrorCode upload_particles(ParticleLS pls, void **dev, int *len) {
   return dcl_memcpy_htod(*dev, ArrayGetData(pls->particles), bytes, NULL);
 }
 
+/* the box of the nodes around the particles' positions */
+static void particle_bbox(ParticleLS pls, int len, int dim, dl_box *b) {
+  const struct _Particle *p = (const struct _Particle *)ArrayGetData(pls->particles);
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int i = 0; i < len; ++i) {
+    const double x[3] = {p[i].X.x, p[i].X.y, p[i].X.z};
+    for (int a = 0; a < dim; ++a) {
+      if (x[a] < lo[a]) lo[a] = x[a];
+      if (x[a] > hi[a]) hi[a] = x[a];
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    if (a >= dim || !(lo[a] <= hi[a])) {
+      b->lo[a] = b->hi[a] = 0;
+      continue;
+    }
+    /* clamped into int range; the box is clipped to the window afterwards */
+    const double l = floor(lo[a]), h = floor(hi[a]) + 1;
+    b->lo[a] = l < -1e9 ? -1000000000 : (int)l;
+    b->hi[a] = h > 1e9 ? 1000000000 : (int)h;
+  }
+}
+
 static PetscErrorCode advect_particles(ParticleLS pls, Coor dh, Grid velgrid, PetscReal dt, int dim,
                                        const char *where) {
   void *dpart, *dvel;
@@ -53,10 +78,24 @@ static PetscErrorCode advect_particles(ParticleLS pls, Coor dh, Grid velgrid, Pe
   ierr = PetscLogEventBegin(EVENT_ParticleLS_AdvectParticles, 0, 0, 0, 0); CHKERRQ(ierr);
   DL_CHECK(where, upload_particles(pls, &dpart, &len));
   if (len > 0) {
-    DL_CHECK(where, dl_upload_grid(DL_VEL, velgrid, &dvel));
+    /* the velocity over the particles' box grown by their reach in one step
+     * (each RK2 stage moves a particle by at most dt max|v| / dh) */
+    dcl_window vw;
+    if (dim == 2 && !velgrid->is2D) {
+      /* the 2D particle step over a 3D velocity grid (the RK2 PLS steps, as
+       * in the reference) reads its first plane: the whole grid goes up */
+      dl_velocity_invalidate();
+      DL_CHECK(where, dl_upload_grid(DL_VEL, velgrid, &dvel));
+      vw = dl_window(velgrid);
+    } else {
+      dl_box bb;
+      particle_bbox(pls, len, dim, &bb);
+      int M = 0;
+      DL_CHECK(where, dl_velocity_box(velgrid, &bb, dt, dh, dim, &dvel, &vw, &M));
+    }
     dcl_velocity V;
     V.data = dvel;
-    V.win = dl_window(velgrid);
+    V.win = vw;
     V.dof = velgrid->dof;
     V.udof = 0;
     dcl_coor h = {dh.x, dh.y, dh.z};
@@ -87,13 +126,30 @@ static PetscErrorCode correct(ParticleLS pls, LevelSet ls, int dim, const char *
   ierr = PetscLogEventBegin(EVENT_ParticleLS_ErrorCorrection, 0, 0, 0, 0); CHKERRQ(ierr);
   DL_CHECK(where, upload_particles(pls, &dpart, &len));
   if (blen > 0) {
-    DL_CHECK(where, ensure_ws(dl_nodes(ls->phi), 8L * (len > 0 ? len : 1)));
-    DL_CHECK(where, dl_upload_grid(DL_PHI, ls->phi, &dphi));
+    /* phi over the box of the band and of the particles' corner cubes (with
+     * the reference's A.z = floor(X.y), the z range of the corners is the y
+     * range of the particles), grown by one node */
+    const int flags = (dim == 3 && g_ref_az) ? DCL_PLS_REF_AZ : 0;
+    dl_box bb, pbb, box;
+    dl_bbox(band, blen, sizeof(iCoor), dim, &bb);
+    if (len > 0) {
+      particle_bbox(pls, len, dim, &pbb);
+      if (flags) {
+        if (pbb.lo[1] < pbb.lo[2]) pbb.lo[2] = pbb.lo[1];
+        if (pbb.hi[1] > pbb.hi[2]) pbb.hi[2] = pbb.hi[1];
+      }
+      for (int a = 0; a < 3; ++a) {
+        if (pbb.lo[a] < bb.lo[a]) bb.lo[a] = pbb.lo[a];
+        if (pbb.hi[a] > bb.hi[a]) bb.hi[a] = pbb.hi[a];
+      }
+    }
+    if (!dl_box_around(ls->phi, &bb, 1, 0, &box)) return dl_fail(where, DCL_ERR_ARG_SIZ);
+    dcl_window w;
+    DL_CHECK(where, dl_upload_box(DL_PHI, ls->phi, &box, &dphi, &w, NULL, 0));
+    DL_CHECK(where, ensure_ws((long)w.n.x * w.n.y * w.n.z, 8L * (len > 0 ? len : 1)));
     DL_CHECK(where, dl_slot(DL_BAND, sizeof(iCoor) * (size_t)blen, &dband));
     DL_CHECK(where, dcl_memcpy_htod(dband, band, sizeof(iCoor) * (size_t)blen, NULL));
     DL_CHECK(where, dl_slot(DL_OUT, sizeof(PetscReal) * (size_t)blen, &dout));
-    const dcl_window w = dl_window(ls->phi);
-    const int flags = (dim == 3 && g_ref_az) ? DCL_PLS_REF_AZ : 0;
     DL_CHECK(where, dcl_pls_correct(dpart, len, dphi, &w, (const dcl_icoor *)dband, blen, dout, flags, dim,
                                     DCL_F64, g_ws, NULL));
     DL_CHECK(where, dl_host(sizeof(PetscReal) * (size_t)blen, &hout));
@@ -129,8 +185,12 @@ PetscErrorCode ParticleLS_AdjustRadii(ParticleLS pls, LevelSet ls) {
   ierr = PetscLogEventBegin(EVENT_ParticleLS_AdjustRadii, 0, 0, 0, 0); CHKERRQ(ierr);
   DL_CHECK("ParticleLS_AdjustRadii", upload_particles(pls, &dpart, &len));
   if (len > 0) {
-    DL_CHECK("ParticleLS_AdjustRadii", dl_upload_grid(DL_PHI, ls->phi, &dphi));
-    const dcl_window w = dl_window(ls->phi);
+    /* phi over the particles' box grown by one node (trilinear phi at each) */
+    dl_box bb, box;
+    particle_bbox(pls, len, ls->phi->is2D ? 2 : 3, &bb);
+    if (!dl_box_around(ls->phi, &bb, 1, 0, &box)) return dl_fail("ParticleLS_AdjustRadii", DCL_ERR_ARG_SIZ);
+    dcl_window w;
+    DL_CHECK("ParticleLS_AdjustRadii", dl_upload_box(DL_PHI, ls->phi, &box, &dphi, &w, NULL, 0));
     DL_CHECK("ParticleLS_AdjustRadii", dcl_pls_adjust_radii(dpart, len, dphi, &w, pls->R_MIN, pls->R_MAX,
                                                             ls->phi->is2D ? 2 : 3, DCL_F64, NULL));
     DL_CHECK("ParticleLS_AdjustRadii", dcl_memcpy_dtoh(ArrayGetData(pls->particles), dpart,
@@ -140,8 +200,9 @@ PetscErrorCode ParticleLS_AdjustRadii(ParticleLS pls, LevelSet ls) {
   return 0;
 }
 
-/* ParticleLS.c:299-329 */
-PetscErrorCode LevelSetAdvectPLS(LevelSet ls, Grid velgrid, PetscReal dt) {
+/* ParticleLS.c:299-329; the velocity grid is uploaded once for the
+ * particles, the SL step and the CFL count */
+static PetscErrorCode advect_pls(LevelSet ls, Grid velgrid, PetscReal dt) {
   ParticleLS pls = ls->pls;
   PetscErrorCode ierr;
   PetscFunctionBegin;
@@ -161,9 +222,16 @@ PetscErrorCode LevelSetAdvectPLS(LevelSet ls, Grid velgrid, PetscReal dt) {
   PetscFunctionReturn(0);
 }
 
+PetscErrorCode LevelSetAdvectPLS(LevelSet ls, Grid velgrid, PetscReal dt) {
+  dl_scope_begin();
+  PetscErrorCode ierr = advect_pls(ls, velgrid, dt);
+  dl_scope_end();
+  return ierr;
+}
+
 /* ParticleLS.c:333-350; the particle step is ParticleLS_AdvectParticles2D,
  * as in the reference */
-PetscErrorCode LevelSetAdvectPLSRK2HalfStep(LevelSet ls, Grid velgrid, PetscReal dt) {
+static PetscErrorCode pls_rk2_half(LevelSet ls, Grid velgrid, PetscReal dt) {
   ParticleLS pls = ls->pls;
   PetscErrorCode ierr;
   PetscFunctionBegin;
@@ -179,8 +247,15 @@ PetscErrorCode LevelSetAdvectPLSRK2HalfStep(LevelSet ls, Grid velgrid, PetscReal
   PetscFunctionReturn(0);
 }
 
+PetscErrorCode LevelSetAdvectPLSRK2HalfStep(LevelSet ls, Grid velgrid, PetscReal dt) {
+  dl_scope_begin();
+  PetscErrorCode ierr = pls_rk2_half(ls, velgrid, dt);
+  dl_scope_end();
+  return ierr;
+}
+
 /* ParticleLS.c:353-375 */
-PetscErrorCode LevelSetAdvectPLSRK2FullStep(LevelSet ls, Grid velgrid, PetscReal dt) {
+static PetscErrorCode pls_rk2_full(LevelSet ls, Grid velgrid, PetscReal dt) {
   ParticleLS pls = ls->pls;
   PetscErrorCode ierr;
   PetscFunctionBegin;
@@ -199,4 +274,11 @@ PetscErrorCode LevelSetAdvectPLSRK2FullStep(LevelSet ls, Grid velgrid, PetscReal
   }
   ierr = ParticleLS_AdjustRadii(pls, ls); CHKERRQ(ierr);
   PetscFunctionReturn(0);
+}
+
+PetscErrorCode LevelSetAdvectPLSRK2FullStep(LevelSet ls, Grid velgrid, PetscReal dt) {
+  dl_scope_begin();
+  PetscErrorCode ierr = pls_rk2_full(ls, velgrid, dt);
+  dl_scope_end();
+  return ierr;
 }

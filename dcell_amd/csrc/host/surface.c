@@ -7,6 +7,7 @@
  * reference does inside its loop (the force functions read only their own
  * node).
  */
+#include <math.h>
 #include <string.h>
 
 #include "ImmersedInterfaceMethod.h"
@@ -18,7 +19,26 @@ static PetscErrorCode surface_quantities(IIM iim, LevelSet ls, const char *where
   if (len == 0) return 0;
   IrregularNode *nodes = (IrregularNode *)ArrayGetData(ls->irregularNodes);
   void *dphi, *drec, *dout, *hrec;
-  DL_CHECK(where, dl_upload_grid(DL_PHI, ls->phi, &dphi));
+  /* phi over the box of the records' interface points grown by 4 nodes (each
+   * reads the 4x4x4 neighbourhood of its cell) */
+  const int dim = ls->phi->is2D ? 2 : 3;
+  dl_box bb, pb;
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int i = 0; i < len; ++i) {
+    const double x[3] = {nodes[i].X.x, nodes[i].X.y, nodes[i].X.z};
+    for (int a = 0; a < dim; ++a) {
+      if (x[a] < lo[a]) lo[a] = x[a];
+      if (x[a] > hi[a]) hi[a] = x[a];
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    const int ok = a < dim && lo[a] <= hi[a] && lo[a] > -1e9 && hi[a] < 1e9;
+    bb.lo[a] = ok ? (int)floor(lo[a]) : -1000000000;
+    bb.hi[a] = ok ? (int)floor(hi[a]) + 1 : 1000000000;
+  }
+  if (!dl_box_around(ls->phi, &bb, 4, 0, &pb)) return dl_fail(where, DCL_ERR_ARG_SIZ);
+  dcl_window w;
+  DL_CHECK(where, dl_upload_box(DL_PHI, ls->phi, &pb, &dphi, &w, NULL, 0));
   /* the records' X, packed as the C ABI's irregular-node records */
   const size_t rbytes = sizeof(dcl_irregular_node) * (size_t)len;
   const size_t sbytes = sizeof(dcl_surface) * (size_t)len;
@@ -33,7 +53,6 @@ static PetscErrorCode surface_quantities(IIM iim, LevelSet ls, const char *where
   DL_CHECK(where, dl_slot(DL_IRREG, rbytes, &drec));
   DL_CHECK(where, dcl_memcpy_htod(drec, r, rbytes, NULL));
   DL_CHECK(where, dl_slot(DL_SURF, sbytes, &dout));
-  const dcl_window w = dl_window(ls->phi);
   DL_CHECK(where, dcl_surface_quantities(dphi, &w, (const dcl_irregular_node *)drec, len, DCL_F64,
                                          (dcl_surface *)dout, NULL));
   DL_CHECK(where, dcl_memcpy_dtoh(hrec, dout, sbytes, NULL));

@@ -136,6 +136,24 @@ int dcl_memcpy_dtod(void* dst, const void* src, size_t bytes, void* stream) {
   return 0;
 }
 
+int dcl_host_malloc(void** ptr, size_t bytes) {
+  if (!ptr) return set_error(DCL_ERR_ARG_NULL, "ptr is NULL");
+  *ptr = nullptr;
+  if (bytes == 0) return 0;
+  hipError_t e = hipHostMalloc(ptr, bytes, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    *ptr = nullptr;
+    (void)hipGetLastError();
+    return set_error(DCL_ERR_MEM, "hipHostMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  }
+  return 0;
+}
+
+int dcl_host_free(void* ptr) {
+  if (ptr) DCL_HIP(hipHostFree(ptr));
+  return 0;
+}
+
 int dcl_stream_synchronize(void* stream) {
   DCL_HIP(hipStreamSynchronize(as_stream(stream)));
   return 0;

@@ -97,6 +97,9 @@ int dcl_memcpy_htod(void *dst, const void *src, size_t bytes, void *stream);
 int dcl_memcpy_dtoh(void *dst, const void *src, size_t bytes, void *stream);
 int dcl_memcpy_dtod(void *dst, const void *src, size_t bytes, void *stream);
 int dcl_stream_synchronize(void *stream);
+/* page-locked host memory (staging of asynchronous copies) */
+int dcl_host_malloc(void **ptr, size_t bytes);
+int dcl_host_free(void *ptr);
 
 /* ------------------------------------------------------------- SL advect */
 /* One semi-Lagrangian step phi0 -> phi.
@@ -348,14 +351,22 @@ int dcl_error_norms(const void *a, int prec_a, const void *b, int prec_b, int64_
                     double out[4], void *stream);
 
 /* ---------------------------------------------------------------- batches */
-/* Many independent 3D level sets (the reference's DCells, which each rank
+/* Many independent level sets (the reference's DCells, which each rank
  * advects one after another, DWorld/DCellsArray.c:57-60) updated together:
  * the cells' windows are packed back to back in one phi buffer, one kappa
- * buffer and one velocity buffer (3 components per node, as a velgrid), cut
- * into tiles, and one step is two persistent launches over the packed tile
- * list (SL then curvature, as dcl_levelset_step per cell).  No counterpart in
- * the reference.  The velocity of each cell shares the cell's window. */
+ * buffer and one velocity buffer (dim components per node, as a velgrid),
+ * cut into tiles, and one step is two persistent launches over the packed
+ * tile list (SL then curvature, as dcl_levelset_step per cell): every block
+ * claims tiles from a per-XCD work queue of cost-balanced chunks, so cells of
+ * uneven sizes load the whole chip.  A batch holds 3D cells or 2D cells, not
+ * both.  No counterpart in the reference.  The velocity of each cell shares
+ * the cell's window. */
 typedef struct dcl_batch_s *dcl_batch;
+
+/* cell-size classes of a 3D batch (largest axis <= 48, <= 80, larger) and
+ * the ring z-chunks the autotuner tries for each (16, 32, 64, 128) */
+#define DCL_BATCH_CLASSES 3
+#define DCL_BATCH_ZCHUNKS 4
 
 int dcl_batch_create(int32_t ncells, const dcl_window *wins, int precision, int interp,
                      int backtrace, dcl_batch *batch);
@@ -365,22 +376,40 @@ int dcl_batch_destroy(dcl_batch batch);
 int dcl_batch_layout(dcl_batch batch, int64_t *phi_offsets, int64_t *vel_offsets);
 /* the step size of every cell (host array of ncells); required before stepping */
 int dcl_batch_set_dt(dcl_batch batch, const double *dt);
-/* configuration of the launches: shape 0..5 = curvature and SL tile shape
- * (tile[] of dcl_batch_get_config), shape 6..11 = the curvature tiles of
- * shape - 6 with the SL step over 32 x 8 x 32 columns staged in an LDS
- * ring; occupancy 0..2 -> 2, 4, 8 resident blocks per CU */
+/* configuration of the launches (3D): shape 0..5 = curvature and SL tile
+ * shape (tile[] of dcl_batch_get_config), shape 6..11 = the curvature tiles
+ * of shape - 6 with the SL step over 32 x 8 columns of the cell class's
+ * z-chunk staged in an LDS ring; occupancy 0..2 -> 2, 4, 8 resident blocks
+ * per CU.  2D batches use 64 x 4 node tiles whatever the shape. */
 int dcl_batch_set_config(dcl_batch batch, int shape, int occupancy);
 int dcl_batch_get_config(dcl_batch batch, int *shape, int *occupancy, int32_t tile[3],
                          int *blocks);
+/* ring z-chunk of one size class (default 32) */
+int dcl_batch_set_class_zchunk(dcl_batch batch, int cls, int zchunk);
+/* per size class: the largest cell axis it holds (-1: unbounded), its cell
+ * count and its ring z-chunk; any pointer may be NULL */
+int dcl_batch_get_classes(dcl_batch batch, int32_t max_edge[DCL_BATCH_CLASSES],
+                          int32_t cells[DCL_BATCH_CLASSES], int32_t zchunk[DCL_BATCH_CLASSES]);
 /* one step phi0 -> phi, kappa(phi) for every cell */
 int dcl_batch_step(dcl_batch batch, const void *phi0, void *phi, void *kappa, const void *vel,
                    void *stream);
+/* Band mode, the band loop of LevelSetAdvectSL_2D / _3D
+ * (LevelSetMethod/AdvectSL.c:23-89) for every cell at once: band is a device
+ * array of count entries {x, y, z, cell} (int32; global node coordinates of
+ * that cell's window, z ignored in 2D).  phi gets the SL value and then kappa
+ * the curvature of phi at every listed node; other nodes are left as they
+ * are.  No allocation. */
+int dcl_batch_band_step(dcl_batch batch, const void *phi0, void *phi, void *kappa,
+                        const void *vel, const int32_t *band, int64_t count, void *stream);
 /* Times every configuration (reps steps each, after one warm-up step) on
- * these buffers and keeps the fastest.  ms_table (optional, 36 entries,
- * shape-major) receives the per-step times.  Synchronous. */
+ * these buffers and keeps the fastest, then, when that uses ring columns,
+ * each size class's z-chunk in turn.  ms_table (optional, 36 entries,
+ * shape-major, -1 where not timed) and zc_table (optional, DCL_BATCH_CLASSES
+ * x DCL_BATCH_ZCHUNKS, -1 where not timed) receive the per-step times.
+ * Synchronous. */
 int dcl_batch_autotune(dcl_batch batch, const void *phi0, void *phi, void *kappa,
-                       const void *vel, int reps, double *ms_table, double *best_ms,
-                       void *stream);
+                       const void *vel, int reps, double *ms_table, double *zc_table,
+                       double *best_ms, void *stream);
 /* One step with these buffers recorded as an instantiated hipGraph (two
  * kernel nodes); launch it with dcl_batch_graph_launch. */
 int dcl_batch_graph_create(dcl_batch batch, const void *phi0, void *phi, void *kappa,
@@ -396,6 +425,28 @@ int dcl_batch_graph_destroy(void *graph_exec);
 int dcl_sl_adjoint(const void *phi0, const dcl_window *win, const dcl_velocity *vel,
                    const void *lam_next, void *lam, void *gvel, double dt,
                    int precision, int interp, int backtrace, void *stream);
+
+/* The adjoint of N SL steps with one velocity (a steady velocity, as in the
+ * differentiation form), split so that the velocity work is done once:
+ *   dcl_sl_adjoint_prepare: every interior node's departure stencil (base,
+ *     fractions, clamped axes) into `record` (a device buffer of
+ *     dcl_sl_adjoint_record_bytes bytes; 48 per node);
+ *   dcl_sl_adjoint_step, once per step in reverse order: lam += dL/dphi0 of
+ *     the step from lam_next = dL/dphi, and hsum (3 per node, device, start
+ *     at zero) += the step's departure sensitivities -c lam_next dI/dS;
+ *   dcl_sl_adjoint_finish: gvel += dL/dvel of all the steps, formed from hsum
+ *     with the velocity scatter of dcl_sl_adjoint (MAC corners at the RK2
+ *     midpoint and the node's faces).
+ * The sum of dcl_sl_adjoint over the steps equals these to rounding.  No
+ * counterpart in the reference; the velocity window must equal phi's. */
+int dcl_sl_adjoint_record_bytes(const dcl_window *win, int64_t *bytes);
+int dcl_sl_adjoint_prepare(const dcl_window *win, const dcl_velocity *vel, double dt, int precision,
+                           int interp, int backtrace, void *record, void *stream);
+int dcl_sl_adjoint_step(const void *phi0, const dcl_window *win, const void *record,
+                        const void *lam_next, void *lam, void *hsum, double dt, int precision,
+                        int interp, void *stream);
+int dcl_sl_adjoint_finish(const dcl_window *win, const dcl_velocity *vel, const void *hsum,
+                          void *gvel, double dt, int precision, int backtrace, void *stream);
 
 #ifdef __cplusplus
 }

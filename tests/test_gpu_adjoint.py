@@ -75,6 +75,43 @@ def test_adjoint_rough_velocity(dev, interp, bt):
     assert rel_max(to_np(gvel), gvel_ref) <= 1e-11
 
 
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-12), (torch.float32, 2e-5)])
+@pytest.mark.parametrize("interp,bt", [("cubic", "rk2"), ("linear", "euler"), ("cubic", "euler"),
+                                       ("linear", "rk2")])
+def test_steady_adjoint_equals_per_step(dev, dtype, tol, interp, bt):
+    """The steady-velocity split (record once, record-driven steps that
+    accumulate the departure sensitivities, one velocity scatter at the end)
+    equals the per-step dcl_sl_adjoint summed over a 4-step chain: lam to
+    the rounding of the atomics' order (the same transposed gather), dL/dvel
+    to rounding (the scatter of a sum instead of the sum of scatters)."""
+    rng = np.random.default_rng(31)
+    n = 30
+    h = 1.0 / (n - 1)
+    win = D.GridWindow((2, -1, 3), (n, n, n), (h, h, h))
+    phi = to_dev(rounded(smooth_field(n, 3, rng), dtype), dtype)
+    vel = to_dev(rounded(random_velocity(n, 3, rng, 1.0), dtype), dtype)
+    dt = 2.5 * h
+    states = [phi]
+    for _ in range(3):
+        states.append(D.advect_sl(states[-1], vel, win, dt, interp, bt))
+    lam0 = to_dev(rng.normal(size=(n, n, n)), dtype)
+    lam, gref = lam0.clone(), torch.zeros_like(vel)
+    for s in reversed(range(4)):
+        lam, _ = A.sl_adjoint(states[s], vel, lam, win, dt, interp, bt, gvel=gref)
+    rec = A.adjoint_record(vel, win, dt, interp, bt)
+    hsum = torch.zeros(3 * win.nodes, dtype=dtype, device="cuda:0")
+    lam2 = lam0.clone()
+    for s in reversed(range(4)):
+        nxt = torch.zeros_like(lam2)
+        A.adjoint_step(states[s], rec, lam2, win, dt, nxt, hsum, interp)
+        lam2 = nxt
+    gvel = A.adjoint_finish(vel, hsum, win, dt, torch.zeros_like(vel), bt)
+    assert rel_max(to_np(lam2), to_np(lam)) <= (1e-12 if dtype == torch.float64 else 1e-5)
+    g, r = to_np(gvel), to_np(gref)
+    assert rel_max(g, r) <= tol
+    assert np.linalg.norm(g - r) <= tol * np.linalg.norm(r)
+
+
 def _c5_problem(n, steps, dtype):
     """A scaled-down differentiation form: sphere r 0.2 centred, steady
     velocity of random Fourier modes (k 1..3, seed 5), target from seed 6."""

@@ -1,7 +1,9 @@
 """The batched update (dcl_batch_*): every cell of a packed batch against the
-oracle's dense step of that cell alone; every tile configuration gives the
-same bits; the hipGraph of a step reproduces the eager step over repeated
-replays; the autotuner picks one of the timed configurations.
+oracle's dense step of that cell alone, for 3D and 2D cells; the band-mode
+step against the oracle's band step of every cell, nodes outside the band
+untouched; every tile configuration and z-chunk gives the same bits; the
+hipGraph of a step reproduces the eager step over repeated replays; the
+autotuner picks one of the timed configurations.
 
 FP64: <= 1e-12 of max |phi| (kappa over |phi| < 6 against its max);
 FP32: oracle on float32-rounded inputs, 2e-6 (phi) and 5e-5 (kappa).
@@ -14,7 +16,7 @@ import oracle as O
 import dcell_amd as D
 from dcell_amd import configs
 from dcell_amd.batch import NOCC, NSHAPES, Batch
-from helpers import owin, rel_max, rounded, to_np
+from helpers import owin, random_velocity, rel_max, rounded, smooth_field, to_np
 
 pytestmark = pytest.mark.gpu
 
@@ -67,10 +69,27 @@ def test_batch_configs_identical_graph_replays_autotune(dev):
             kap.zero_()
             b.step(phi0, phi1, kap, vel)
             assert torch.equal(phi1, ref_phi) and torch.equal(kap, ref_kap), (shape, occ)
-    best, table = b.autotune(phi0, phi1, kap, vel, reps=2)
-    assert np.all(table > 0) and best == pytest.approx(table.min())
+    # every ring z-chunk of every size class: the same bits
+    b.set_config(7, 1)
+    for cls in range(3):
+        for zc in (8, 16, 64, 128):
+            b.set_class_zchunk(cls, zc)
+            phi1.zero_()
+            kap.zero_()
+            b.step(phi0, phi1, kap, vel)
+            assert torch.equal(phi1, ref_phi) and torch.equal(kap, ref_kap), (cls, zc)
+        b.set_class_zchunk(cls, 32)
+    best, table, ztable = b.autotune(phi0, phi1, kap, vel, reps=2)
+    assert np.all(table > 0)
     cfg = b.config()
-    assert table[cfg["shape"], cfg["occupancy"]] == pytest.approx(best)
+    assert best <= table[cfg["shape"], cfg["occupancy"]] * (1 + 1e-9) and best <= table.min() * (1 + 1e-9)
+    timed = ztable[ztable > 0]
+    if cfg["shape"] >= 6:
+        assert timed.size > 0 and best == pytest.approx(min(timed.min(), table.min()))
+        for k, c in enumerate(b.classes()):
+            assert c["zchunk"] in (16, 32, 64, 128)
+    else:
+        assert timed.size == 0 and best == pytest.approx(table.min())
     # a graph of one step, replayed with eager work between the replays
     g = b.graph(phi0, phi1, kap, vel)
     for _ in range(4):
@@ -83,9 +102,100 @@ def test_batch_configs_identical_graph_replays_autotune(dev):
     b.close()
 
 
+EDGES2D = [9, 24, 37, 64, 90, 131]
+
+
+def _build2d(dtype, interp, bt, seed=8):
+    rng = np.random.default_rng(seed)
+    wins, phis, vels, dts = [], [], [], []
+    for q, n in enumerate(EDGES2D):
+        h = 1.0 / (n - 1)
+        wins.append(D.GridWindow((q - 2, 3 - q, 0), (n, n + q % 2, 1), (h, h, h)))
+        phi = smooth_field(max(n, n + q % 2), 2, rng)[: n + q % 2, :n]
+        vel = random_velocity(max(n, n + q % 2), 2, rng, 1.0)[: n + q % 2, :n]
+        phis.append(rounded(np.ascontiguousarray(phi), dtype))
+        vels.append(rounded(np.ascontiguousarray(vel), dtype))
+        dts.append(2.5 * h)
+    b = Batch(wins, dtype, interp, bt)
+    phi0, phi1, kap, vel = b.new_field(), b.new_field(), b.new_field(), b.new_field(2)
+    for c in range(len(wins)):
+        b.cell(phi0, c).copy_(torch.from_numpy(phis[c]))
+        b.cell(vel, c, 2).copy_(torch.from_numpy(vels[c]))
+    b.set_dt(dts)
+    return b, (phi0, phi1, kap, vel), wins, phis, vels, dts
+
+
+@pytest.mark.parametrize("dtype,tp,tk", [(torch.float64, 1e-12, 1e-12), (torch.float32, 2e-6, 5e-5)])
+@pytest.mark.parametrize("interp,bt", [("cubic", "rk2"), ("linear", "euler")])
+def test_batch_2d_matches_per_cell_oracle(dev, dtype, tp, tk, interp, bt):
+    b, (phi0, phi1, kap, vel), wins, phis, vels, dts = _build2d(dtype, interp, bt)
+    best, table, ztable = b.autotune(phi0, phi1, kap, vel, reps=1)
+    assert table[0].min() > 0 and np.all(table[1:] == -1) and np.all(ztable == -1)
+    g = b.graph(phi0, phi1, kap, vel)
+    phi1.fill_(7.0)
+    kap.fill_(7.0)
+    Batch.launch(g)
+    for c, w in enumerate(wins):
+        ref = O.sl_advect(phis[c], vels[c], owin(w), dts[c], O.CUBIC if interp == "cubic" else O.LINEAR,
+                          O.RK2 if bt == "rk2" else O.EULER)
+        got = to_np(b.cell(phi1, c))
+        assert rel_max(got, ref) <= tp, c
+        kr = O.curvature(rounded(got, dtype), owin(w))
+        sel = np.abs(ref) < 6
+        assert rel_max(to_np(b.cell(kap, c))[sel], kr[sel], np.max(np.abs(kr[sel]))) <= tk, c
+    b.close()
+
+
+def _band_list(wins, phis, width, rng):
+    """Nodes with |phi| < width of every cell as {x, y, z, cell} in a random
+    order, plus each cell's (n, 3) band."""
+    rows, per = [], []
+    for c, (w, phi) in enumerate(zip(wins, phis)):
+        idx = np.argwhere(np.abs(phi) < width)[:, ::-1]           # x, y(, z) local
+        q = np.zeros((len(idx), 3), dtype=np.int32)
+        q[:, :w.dim] = idx + np.array(w.p[:w.dim])
+        q[:, 2] = q[:, 2] if w.dim == 3 else w.p[2]
+        per.append(q)
+        rows.append(np.c_[q, np.full(len(q), c, dtype=np.int32)])
+    allrows = np.concatenate(rows)
+    return allrows[rng.permutation(len(allrows))].astype(np.int32), per
+
+
+@pytest.mark.parametrize("dim", [3, 2])
+@pytest.mark.parametrize("dtype,tp,tk", [(torch.float64, 1e-12, 1e-12), (torch.float32, 2e-6, 5e-5)])
+@pytest.mark.parametrize("interp,bt", [("cubic", "rk2"), ("linear", "euler")])
+def test_batch_band_step(dev, dim, dtype, tp, tk, interp, bt):
+    if dim == 3:
+        b, (phi0, phi1, kap, vel), wins, phis, vels, dts = _build(dtype, interp, bt)
+    else:
+        b, (phi0, phi1, kap, vel), wins, phis, vels, dts = _build2d(dtype, interp, bt)
+    rng = np.random.default_rng(11)
+    rows, per = _band_list(wins, phis, 4.0, rng)
+    band = torch.from_numpy(rows).to("cuda:0")
+    phi1.fill_(7.0)
+    kap.fill_(7.0)
+    b.band_step(phi0, phi1, kap, vel, band)
+    for c, w in enumerate(wins):
+        q = per[c]
+        ref = O.sl_advect(phis[c], vels[c], owin(w), dts[c], O.CUBIC if interp == "cubic" else O.LINEAR,
+                          O.RK2 if bt == "rk2" else O.EULER, band=q)
+        got = to_np(b.cell(phi1, c))
+        loc = tuple(q[:, a] - w.p[a] for a in range(w.dim))[::-1]
+        assert rel_max(got[loc], ref, np.max(np.abs(phis[c]))) <= tp, c
+        mask = np.zeros(got.shape, dtype=bool)
+        mask[loc] = True
+        assert np.all(got[~mask] == 7.0) and np.all(to_np(b.cell(kap, c))[~mask] == 7.0), c
+        # curvature at the band nodes from the stepped phi as the GPU left it
+        kr = O.curvature(rounded(got, dtype), owin(w), band=q)
+        gk = to_np(b.cell(kap, c))[loc]
+        sel = np.abs(ref) < 2
+        assert rel_max(gk[sel], kr[sel], np.max(np.abs(kr[sel]))) <= tk, c
+    b.close()
+
+
 def test_batch_argument_errors(dev):
     with pytest.raises(D._lib.DclError):
-        Batch([D.GridWindow((0, 0, 0), (9, 9, 1), (0.1, 0.1, 0.1))])        # 2D cell
+        Batch([D.GridWindow.unit_box(9), D.GridWindow((0, 0, 0), (9, 9, 1), (0.1, 0.1, 0.1))])  # 3D and 2D
     b = Batch([D.GridWindow.unit_box(9)])
     f = b.new_field()
     with pytest.raises(D._lib.DclError):
