@@ -52,8 +52,12 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=3, help="full-grid steps of the CPU baseline leg")
     ap.add_argument("--form", choices=sorted(FORMS),
                     help="run one form only (for profiling) and print its JSON")
-    ap.add_argument("--probe", action="store_true",
-                    help="only 5 launches of the headline SL kernel and 5 of a copy kernel, for rocprofv3 --pmc")
+    ap.add_argument("--probe", nargs="?", const="sl64", choices=sorted(PROBES),
+                    help="a short program for rocprofv3 --pmc: sl64 (default) 5 launches of the headline SL "
+                         "kernel and 5 of a copy kernel; sl32, curv, batch, adjoint: the other hot kernels")
+    ap.add_argument("--counter-summary", nargs="+", metavar=("NAME", "PROBE DIR"),
+                    help="NAME PROBE DIR...: write profiles/NAME.counters.txt from the rocprofv3 --pmc output "
+                         "folders of --probe PROBE")
     ap.add_argument("--traffic-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="write the HBM traffic summary from two rocprofv3 --pmc output folders")
     return ap.parse_args()
@@ -111,10 +115,14 @@ def dense_run(n, dtype, interp, bt, steps, warmup, deform=True):
 
 
 def probe(args):
-    """A short program for the counter runs: 5 launches of the headline SL
-    kernel on the headline workload, then 5 launches of a plain copy kernel
-    (torch's elementwise neg over COPY_BYTES of FP32: reads and writes
-    COPY_BYTES), whose counter reading per byte scales the SL kernel's."""
+    """Short programs for the counter runs (rocprofv3 --pmc), one per probe:
+    sl64: 5 launches of the headline SL kernel on the headline workload, then
+    5 launches of a plain copy kernel (torch's elementwise neg over COPY_BYTES
+    of FP32: reads and writes COPY_BYTES), whose counter reading per byte
+    scales the SL kernel's; the others run the kernels named in PROBES."""
+    if args.probe != "sl64":
+        PROBES[args.probe](args)
+        return
     import torch
     import dcell_amd as D
     from dcell_amd import configs
@@ -133,6 +141,134 @@ def probe(args):
         torch.neg(src, out=dst)
     torch.cuda.synchronize()
     print(f"probe done: 5 x SL (RK2 + tricubic, FP64) at {n}^3, 5 x neg over {COPY_BYTES} B", flush=True)
+
+
+def probe_sl32(args):
+    """5 launches of the FP32 dense SL kernel (RK2 + tricubic) at 384^3."""
+    _probe_dense(384, "float32", curv=False)
+
+
+def probe_curv(args):
+    """5 launches of the dense curvature kernel on the 256^3 FP64 sphere and 5
+    on the 384^3 FP32 one."""
+    _probe_dense(256, "float64", sl=False)
+    _probe_dense(384, "float32", sl=False)
+
+
+def _probe_dense(n, dtype, sl=True, curv=True):
+    import torch
+    import dcell_amd as D
+    from dcell_amd import configs
+    dt = getattr(torch, dtype)
+    win = D.GridWindow.unit_box(n)
+    phi = torch.from_numpy(configs.sphere_sdf(n)).to("cuda:0", dt)
+    vel = torch.from_numpy(configs.deformation_velocity(n)).to("cuda:0", dt)
+    out, kap = torch.empty_like(phi), torch.empty_like(phi)
+    for s in range(5):
+        if sl:
+            D.advect_sl(phi, vel, win, configs.deformation_dt(n, s), "cubic", "rk2", out=out)
+        if curv:
+            D.curvature(phi, win, out=kap)
+    torch.cuda.synchronize()
+    print(f"probe done: 5 x {'SL ' if sl else ''}{'curvature ' if curv else ''}at {n}^3 {dtype}", flush=True)
+
+
+def probe_batch(args):
+    """3 steps of the 512-cell FP32 batch (RK2 + tricubic) in the ring
+    configuration the autotuner picks on the benchmark's box (shape 7, 4
+    blocks per CU, static stride, z-chunk 32)."""
+    import torch
+    import dcell_amd as D
+    from dcell_amd import configs
+    from dcell_amd.batch import Batch
+    edges = configs.batch_edges(512, 4)
+    g = configs.philox(4)
+    b = Batch([D.GridWindow.unit_box(int(e)) for e in edges], torch.float32, "cubic", "rk2")
+    phi0, phi1, kap, vel = b.new_field(), b.new_field(), b.new_field(), b.new_field(3)
+    dts = []
+    for c, e in enumerate(edges):
+        phi, v, dt = configs.cell_fields(int(e), g, "cuda:0", torch.float32)
+        b.cell(phi0, c).copy_(phi)
+        b.cell(vel, c, 3).copy_(v)
+        dts.append(dt)
+    b.set_dt(dts)
+    b.set_config(7, 1)
+    for _ in range(3):
+        b.step(phi0, phi1, kap, vel)
+    torch.cuda.synchronize()
+    print("probe done: 3 batch steps, 512 cells", flush=True)
+    b.close()
+
+
+def probe_adjoint(args):
+    """One checkpointed backward of the 128^3 FP32 differentiation form (50
+    adjoint steps, 40 recomputed forward steps)."""
+    r = adjoint_run(128, 50, 10)
+    print(f"probe done: adjoint backward {r['backward_ms']:.1f} ms", flush=True)
+
+
+PROBES = {"sl64": None, "sl32": probe_sl32, "curv": probe_curv, "batch": probe_batch, "adjoint": probe_adjoint}
+
+# the counters of the summaries, one rocprofv3 --pmc pass per line
+COUNTER_PASSES = (
+    "SQ_WAVES SQ_INSTS_VALU SQ_INSTS_VMEM_RD SQ_INSTS_VMEM_WR SQ_INSTS_LDS SQ_INSTS_SALU SQ_INSTS_BRANCH SQ_WAIT_INST_ANY",
+    "VALUBusy VALUUtilization MeanOccupancyPerCU TA_BUSY_avr TD_TD_BUSY_sum SQ_LDS_BANK_CONFLICT SQ_ACTIVE_INST_LDS "
+    "GRBM_GUI_ACTIVE",
+    "FETCH_SIZE",
+    "WRITE_SIZE",
+)
+
+
+def counter_summary(args):
+    """profiles/NAME.counters.txt: per kernel, the mean per dispatch of every
+    counter found in the rocprofv3 --pmc folders given, and per wave the
+    instruction counts (one node per lane in these kernels)."""
+    import collections
+    import csv
+    import glob
+    name, probe_name, dirs = args.counter_summary[0], args.counter_summary[1], args.counter_summary[2:]
+    vals = collections.defaultdict(lambda: collections.defaultdict(list))
+    for d in dirs:
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            per = collections.defaultdict(float)
+            kname = {}
+            with open(f) as fh:
+                for row in csv.DictReader(fh):
+                    key = (row["Dispatch_Id"], row["Counter_Name"])
+                    per[key] += float(row["Counter_Value"])
+                    kname[row["Dispatch_Id"]] = row["Kernel_Name"]
+            for (disp, cn), v in per.items():
+                vals[kname[disp]][cn].append(v)
+    out = os.path.join(ROOT, "profiles", f"{name}.counters.txt")
+    with open(out, "w") as fo:
+        fo.write(f"# rocprofv3 counters: {name}\n# one pass per line of counters (from the repository root):\n")
+        for q, c in enumerate(COUNTER_PASSES):
+            fo.write(f"#   rocprofv3 --pmc {c} -d gpurun_out/pmc_{probe_name}_{q} --output-format csv -- "
+                     f"python3 bench.py --probe {probe_name}\n")
+        fo.write(f"#   python3 bench.py --counter-summary {name} {probe_name} "
+                 + " ".join(f"gpurun_out/pmc_{probe_name}_{q}" for q in range(len(COUNTER_PASSES))) + "\n")
+        fo.write("# values: mean per dispatch; '/wave' = per wavefront of 64 lanes (one node per lane);\n"
+                 "# FETCH_SIZE / WRITE_SIZE in KiB; busy %: TA_BUSY_avr and TD_TD_BUSY_sum / 256 CUs over "
+                 "GRBM_GUI_ACTIVE\n")
+        for k in sorted(vals, key=lambda k: -len(vals[k])):
+            cs = vals[k]
+            if "SQ_WAVES" not in cs and "FETCH_SIZE" not in cs:
+                continue
+            mean = {cn: sum(v) / len(v) for cn, v in cs.items()}
+            fo.write(f"\nkernel: {k[:160]}\n  dispatches: {max(len(v) for v in cs.values())}\n")
+            waves = mean.get("SQ_WAVES")
+            for cn in sorted(mean):
+                line = f"  {cn}: {mean[cn]:.6g}"
+                if waves and cn.startswith("SQ_INSTS"):
+                    line += f"  ({mean[cn] / waves:.2f} /wave)"
+                fo.write(line + "\n")
+            gui = mean.get("GRBM_GUI_ACTIVE")
+            if gui:
+                if "TA_BUSY_avr" in mean:
+                    fo.write(f"  TA busy: {100 * mean['TA_BUSY_avr'] / gui:.1f} %\n")
+                if "TD_TD_BUSY_sum" in mean:
+                    fo.write(f"  TD busy: {100 * mean['TD_TD_BUSY_sum'] / 256 / gui:.1f} %\n")
+    print(open(out).read())
 
 
 def _pmc_per_kernel(folder, counter):
@@ -181,8 +317,8 @@ def traffic_summary(args):
     with open(TRAFFIC_FILE, "w") as f:
         f.write(f"# HBM traffic of the dense SL kernel, {n}^3 FP64 deformation workload (RK2 + tricubic)\n")
         f.write("# commands (from the repository root):\n")
-        f.write(f"#   rocprofv3 --pmc FETCH_SIZE -d {fetch_dir} --output-format csv -- python3 bench.py --probe --n {n}\n")
-        f.write(f"#   rocprofv3 --pmc WRITE_SIZE -d {write_dir} --output-format csv -- python3 bench.py --probe --n {n}\n")
+        f.write(f"#   rocprofv3 --pmc FETCH_SIZE -d {fetch_dir} --output-format csv -- python3 bench.py --probe sl64 --n {n}\n")
+        f.write(f"#   rocprofv3 --pmc WRITE_SIZE -d {write_dir} --output-format csv -- python3 bench.py --probe sl64 --n {n}\n")
         f.write(f"#   python3 bench.py --n {n} --traffic-summary {fetch_dir} {write_dir}\n")
         for ln in lines:
             f.write(f"# {ln}\n")
@@ -332,7 +468,7 @@ def batch_run(ncells, seed, steps, warmup, interp="cubic", bt="rk2", autotune=Tr
     import torch
     import dcell_amd as D
     from dcell_amd import configs
-    from dcell_amd.batch import ZCHUNKS, Batch
+    from dcell_amd.batch import CLAIMS, ZCHUNKS, Batch
 
     dtype = torch.float32 if dtype is None else dtype
     edges = configs.batch_edges(ncells, seed, lo, hi)
@@ -350,9 +486,10 @@ def batch_run(ncells, seed, steps, warmup, interp="cubic", bt="rk2", autotune=Tr
     b.set_dt(dts)
     tune = None
     if autotune:
-        best, table, ztable = b.autotune(phi0, phi1, kap, vel, reps=3)
+        best, table, ztable, ctable = b.autotune(phi0, phi1, kap, vel, reps=3)
         tune = dict(best_ms=best, table_ms=table.round(4).tolist(), zchunks=list(ZCHUNKS),
-                    class_zchunk_ms=ztable.round(4).tolist(), chosen=b.config(), classes=b.classes())
+                    class_zchunk_ms=ztable.round(4).tolist(), tiles_per_claim=list(CLAIMS),
+                    claim_ms=ctable.round(4).tolist(), chosen=b.config(), classes=b.classes())
     graphs = [b.graph(phi0, phi1, kap, vel), b.graph(phi1, phi0, kap, vel)]
     for s in range(warmup):
         Batch.launch(graphs[s % 2])
@@ -906,23 +1043,34 @@ def dropin_run(n=256, calls=10):
 
 
 def copy_ceiling():
-    """The measured HBM ceiling: a device-to-device copy of 1 GiB (torch),
-    best of 5, read + write bytes over its time."""
+    """The measured HBM ceiling: dcl_stream_copy (the library's streaming copy
+    kernel) of 1 GiB, best of 10, read + write bytes over its time, with the
+    same for torch's device copy beside it."""
+    import ctypes as C
     import torch
+    from dcell_amd import _lib as L
+    from dcell_amd.levelset import stream_handle
     src = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")
     dst = torch.empty_like(src)
-    dst.copy_(src)
-    best = 1e30
-    for _ in range(5):
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        dst.copy_(src)
-        e1.record()
-        torch.cuda.synchronize()
-        best = min(best, e0.elapsed_time(e1) * 1e-3)
+
+    def best_of(fn, reps=10):
+        fn()
+        best = 1e30
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            best = min(best, e0.elapsed_time(e1) * 1e-3)
+        return 2 * (1 << 30) / best
+
+    own = best_of(lambda: L.call("dcl_stream_copy", C.c_void_p(dst.data_ptr()), C.c_void_p(src.data_ptr()),
+                                 1 << 30, stream_handle()))
+    torch_copy = best_of(lambda: dst.copy_(src))
     del src, dst
     torch.cuda.empty_cache()
-    return 2 * (1 << 30) / best
+    return own, torch_copy
 
 
 def _f64():
@@ -977,6 +1125,9 @@ def main():
     if args.traffic_summary:
         traffic_summary(args)
         return
+    if args.counter_summary:
+        counter_summary(args)
+        return
     if args.probe:
         probe(args)
         return
@@ -1012,10 +1163,14 @@ def main():
                      "bytes_per_launch": interior * B["sl"],
                      "avg_launch_ms": r["sl_ms"]},
         "step_roofline_frac": r["step_roofline_frac"],
-        "measured_copy_gbs": copy_ceiling() / 1e9,
+        "measured_copy_gbs": None,
         "kernels_ms": {"sl_advect": r["sl_ms"], "curvature": r["curv_ms"]},
         "curvature_gbs": r["curv_gbs"],
     }
+    own, torch_copy = copy_ceiling()
+    line["measured_copy_gbs"] = own / 1e9
+    line["torch_copy_gbs"] = torch_copy / 1e9
+    line["roofline"]["frac_of_measured_copy"] = achieved / own
     if not args.no_forms:
         line["forms"] = forms(args)
     if not args.no_cpu:

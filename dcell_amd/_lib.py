@@ -22,7 +22,7 @@ DCL_PLS_REF_AZ = 1
 EXPORTS = (
     "dcl_abi_version", "dcl_last_error", "dcl_device_count", "dcl_malloc", "dcl_free",
     "dcl_memcpy_htod", "dcl_memcpy_dtoh", "dcl_memcpy_dtod", "dcl_stream_synchronize",
-    "dcl_host_malloc", "dcl_host_free",
+    "dcl_host_malloc", "dcl_host_free", "dcl_stream_copy",
     "dcl_sl_advect", "dcl_curvature", "dcl_levelset_step",
     "dcl_pls_advect", "dcl_pls_ws_create", "dcl_pls_ws_destroy", "dcl_pls_correct",
     "dcl_pls_adjust_radii", "dcl_max_velocity", "dcl_band_compact", "dcl_irregular_nodes",
@@ -32,6 +32,7 @@ EXPORTS = (
     "dcl_sl_adjoint_record_bytes", "dcl_sl_adjoint_prepare", "dcl_sl_adjoint_step", "dcl_sl_adjoint_finish",
     "dcl_batch_create", "dcl_batch_destroy", "dcl_batch_layout", "dcl_batch_set_dt",
     "dcl_batch_set_config", "dcl_batch_get_config", "dcl_batch_set_class_zchunk", "dcl_batch_get_classes",
+    "dcl_batch_set_claim", "dcl_batch_get_claim",
     "dcl_batch_step", "dcl_batch_band_step", "dcl_batch_autotune",
     "dcl_batch_graph_create", "dcl_batch_graph_launch", "dcl_batch_graph_destroy",
 )
@@ -86,6 +87,7 @@ def _declare(L):
         "dcl_stream_synchronize": ([vp], i32),
         "dcl_host_malloc": ([C.POINTER(vp), C.c_size_t], i32),
         "dcl_host_free": ([vp], i32),
+        "dcl_stream_copy": ([vp, vp, C.c_size_t, vp], i32),
         "dcl_sl_advect": ([vp, vp, W, V, vp, i64, vp, dbl, i32, i32, i32, vp], i32),
         "dcl_curvature": ([vp, vp, W, vp, i64, vp, i32, i32, vp], i32),
         "dcl_levelset_step": ([vp, vp, vp, W, V, dbl, i32, i32, i32, vp], i32),
@@ -120,7 +122,9 @@ def _declare(L):
         "dcl_batch_get_classes": ([vp, vp, vp, vp], i32),
         "dcl_batch_step": ([vp, vp, vp, vp, vp, vp], i32),
         "dcl_batch_band_step": ([vp, vp, vp, vp, vp, vp, i64, vp], i32),
-        "dcl_batch_autotune": ([vp, vp, vp, vp, vp, i32, vp, vp, vp, vp], i32),
+        "dcl_batch_autotune": ([vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, vp], i32),
+        "dcl_batch_set_claim": ([vp, i32], i32),
+        "dcl_batch_get_claim": ([vp, C.POINTER(i32)], i32),
         "dcl_batch_graph_create": ([vp, vp, vp, vp, vp, C.POINTER(vp)], i32),
         "dcl_batch_graph_launch": ([vp, vp], i32),
         "dcl_batch_graph_destroy": ([vp], i32),

@@ -22,6 +22,7 @@ NSHAPES, NOCC = 12, 3  # 6 tile shapes x 2 SL kernels (tile gather, LDS ring), x
 NCONFIG = NSHAPES * NOCC
 NCLASS, NZC = 3, 4     # cell-size classes x ring z-chunks tried (16, 32, 64, 128)
 ZCHUNKS = (16, 32, 64, 128)
+CLAIMS = (0, 1, 4, 16)  # tiles per claim tried (0: static stride)
 
 
 class Batch:
@@ -63,10 +64,14 @@ class Batch:
         L.call("dcl_batch_set_config", self.handle, shape, occupancy)
 
     def config(self):
-        s, o, b = C.c_int(), C.c_int(), C.c_int()
+        s, o, b, k = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         tile = (C.c_int32 * 3)()
         L.call("dcl_batch_get_config", self.handle, C.byref(s), C.byref(o), tile, C.byref(b))
-        return dict(shape=s.value, occupancy=o.value, tile=list(tile), blocks=b.value)
+        L.call("dcl_batch_get_claim", self.handle, C.byref(k))
+        return dict(shape=s.value, occupancy=o.value, tile=list(tile), blocks=b.value, tiles_per_claim=k.value)
+
+    def set_claim(self, tiles_per_claim: int):
+        L.call("dcl_batch_set_claim", self.handle, tiles_per_claim)
 
     def classes(self):
         """Per size class: the largest cell axis it holds (None: unbounded), its
@@ -104,15 +109,17 @@ class Batch:
 
     def autotune(self, phi0, phi, kappa, vel, reps: int = 3):
         """Returns (best ms per step, the shape x occupancy table, the size
-        class x z-chunk table); -1 where a configuration was not timed."""
+        class x z-chunk table, the tiles-per-claim table); -1 where a
+        configuration was not timed."""
         self._check(phi0, phi, kappa, vel)
         table = np.zeros(NCONFIG)
         ztable = np.zeros(NCLASS * NZC)
+        ctable = np.zeros(len(CLAIMS))
         best = C.c_double()
         L.call("dcl_batch_autotune", self.handle, _p(phi0), _p(phi), _p(kappa), _p(vel), reps,
-               table.ctypes.data_as(C.c_void_p), ztable.ctypes.data_as(C.c_void_p), C.byref(best),
-               stream_handle())
-        return best.value, table.reshape(NSHAPES, NOCC), ztable.reshape(NCLASS, NZC)
+               table.ctypes.data_as(C.c_void_p), ztable.ctypes.data_as(C.c_void_p),
+               ctable.ctypes.data_as(C.c_void_p), C.byref(best), stream_handle())
+        return best.value, table.reshape(NSHAPES, NOCC), ztable.reshape(NCLASS, NZC), ctable
 
     def graph(self, phi0, phi, kappa, vel):
         self._check(phi0, phi, kappa, vel)

@@ -18,6 +18,7 @@
 // phi = phi0 there, so dL/dphi0 += lam_next.
 #include <algorithm>
 
+#include "dcl_buf.hpp"
 #include "dcl_host.hpp"
 
 namespace dcl {
@@ -504,11 +505,28 @@ __global__ __launch_bounds__(256) void adj_step_k(const T* __restrict__ phi0, co
     up_m[d] = key != 0 && ku != 0 && key + d == ku;
   }
   T G[3] = {T(0), T(0), T(0)};
-  const T* src = phi0 + (b[2] + OFF) * Z + (b[1] + OFF) * Y + (b[0] + OFF);
+  // the stencil rows with buffer loads (a row of 4 values is one 16-byte
+  // load in FP32, two in FP64; inactive lanes read the window's first row),
+  // the rows of two stencil planes in flight at a time in FP32, of one in FP64
+  const rsrc_t rp = make_rsrc(phi0, (unsigned)((int64_t)nx * ny * nz * (int64_t)sizeof(T)));
+  const int vo = act ? ((b[2] + OFF) * Z + (b[1] + OFF) * Y + (b[0] + OFF)) * (int)sizeof(T) : 0;
+  constexpr int KP = sizeof(T) == 4 ? 2 : 1;
 #pragma unroll 1
-  for (int kk = 0; kk < NP; ++kk)
+  for (int k0 = 0; k0 < NP; k0 += KP) {
+    T rows[KP][4][4];
+#pragma unroll
+    for (int dk = 0; dk < KP; ++dk)
+#pragma unroll
+      for (int jj = 0; jj < NP; ++jj) {
+        const int so = ((k0 + dk) * Z + jj * Y) * (int)sizeof(T);
+        if (NP == 4) BL<T>::four(rp, vo, so, rows[dk][jj]);
+        else BL<T>::two(rp, vo, so, rows[dk][jj]);
+      }
+#pragma unroll
+    for (int dk = 0; dk < KP; ++dk)
 #pragma unroll
     for (int jj = 0; jj < NP; ++jj) {
+      const int kk = k0 + dk;
       const T wyz = w[1][jj] * w[2][kk];
       T add[4];
 #pragma unroll
@@ -520,11 +538,10 @@ __global__ __launch_bounds__(256) void adj_step_k(const T* __restrict__ phi0, co
         if (lo_m[d]) word0 += o;
       }
       if (act) {
-        const T* r = src + kk * Z + jj * Y;
         T rx = T(0), rd = T(0);
 #pragma unroll
         for (int ii = 0; ii < NP; ++ii) {
-          const T f = r[ii];
+          const T f = rows[dk][jj][ii];
           rx += w[0][ii] * f;
           rd += dw[0][ii] * f;
           if (in_box) {
@@ -542,6 +559,7 @@ __global__ __launch_bounds__(256) void adj_step_k(const T* __restrict__ phi0, co
         G[2] += rx * w[1][jj] * dw[2][kk];
       }
     }
+  }
   if (act) {
     T* hs = hsum + 3 * (int64_t)id;
 #pragma unroll

@@ -65,13 +65,17 @@ constexpr int NCLASS = DCL_BATCH_CLASSES;
 constexpr int CLASS_EDGE[NCLASS] = {48, 80, 1 << 30};
 constexpr int NZC = DCL_BATCH_ZCHUNKS;
 constexpr int ZC_OPT[NZC] = {16, 32, 64, 128};
+// tiles per claim tried by the autotuner (0: static stride, no atomics)
+constexpr int NCLAIM = DCL_BATCH_CLAIMS;
+constexpr int CLAIM_OPT[NCLAIM] = {0, 1, 4, 16};
 
-// Work queue of one persistent launch: the tile list is cut into 8
+// Work distribution of one persistent launch: the tile list is cut into 8
 // contiguous chunks of about equal cost, one per XCD (blocks are dealt to the
-// XCDs round-robin, so block b runs on XCD b mod 8), and the blocks of an XCD
-// claim the tiles of its chunk one at a time with an atomic counter, so that
-// neighbouring columns (which share halo rows) run in the same L2; a block
-// whose chunk is exhausted takes tiles from the other chunks.  The last block
+// XCDs round-robin, so block b runs on XCD b mod 8), so that neighbouring
+// columns (which share halo rows) run in the same L2.  With k = 0 the blocks
+// of an XCD take its chunk's tiles in a fixed stride (no atomics); with k > 0
+// they claim k tiles at a time with an atomic counter per chunk and, when
+// their chunk is exhausted, take tiles from the other chunks.  The last block
 // to finish resets the counters, so the next launch (or graph replay) starts
 // from zero.
 struct WorkQ {
@@ -81,21 +85,27 @@ struct WorkQ {
 };
 
 template <typename F>
-__device__ __forceinline__ void claim_tiles(WorkQ* __restrict__ q, const int* __restrict__ bounds, F body) {
+__device__ __forceinline__ void claim_tiles(WorkQ* __restrict__ q, const int* __restrict__ bounds, int k, F body) {
+  const int xcd = blockIdx.x & 7;
+  if (k <= 0) {
+    const int nslot = gridDim.x >> 3, slot = blockIdx.x >> 3;
+    for (int t = bounds[xcd] + slot; t < bounds[xcd + 1]; t += nslot) body(t);
+    return;
+  }
   __shared__ int s_tile;
   const bool lead = threadIdx.x == 0 && threadIdx.y == 0;
-  const int xcd = blockIdx.x & 7;
   for (int s = 0; s < 8; ++s) {
     const int x = (xcd + s) & 7;
     const int lo = bounds[x], hi = bounds[x + 1];
     if (lo >= hi) continue;
     for (;;) {
-      if (lead) s_tile = lo + (int)atomicAdd(&q->claim[x], 1u);
+      if (lead) s_tile = lo + k * (int)atomicAdd(&q->claim[x], 1u);
       __syncthreads();
-      const int t = s_tile;
+      const int t0 = s_tile;
       __syncthreads();
-      if (t >= hi) break;
-      body(t);
+      if (t0 >= hi) break;
+      const int t1 = min(t0 + k, hi);
+      for (int t = t0; t < t1; ++t) body(t);
     }
   }
   if (lead && atomicAdd(&q->done, 1u) == gridDim.x - 1) {
@@ -129,9 +139,9 @@ __device__ __forceinline__ KWin cell_win(const BCell& c) {
 template <typename T, int INTERP, int BT, int S>
 __global__ __launch_bounds__(shape_of(S).tx * shape_of(S).ty) void batch_sl_k(
     const T* __restrict__ phi0, T* __restrict__ phi, const T* __restrict__ vel, const BCell* __restrict__ cells,
-    const int4* __restrict__ tiles, const int* __restrict__ bounds, WorkQ* __restrict__ q) {
+    const int4* __restrict__ tiles, const int* __restrict__ bounds, WorkQ* __restrict__ q, int k) {
   constexpr TileShape sh = shape_of(S);
-  claim_tiles(q, bounds, [&](int t) {
+  claim_tiles(q, bounds, k, [&](int t) {
     const int4 tl = tiles[t];
     const BCell c = cells[tl.x];
     const SLP P = cell_slp<3>(c);
@@ -149,13 +159,13 @@ template <typename T, int INTERP, int BT>
 __global__ __launch_bounds__(256) void batch_ring_k(const T* __restrict__ phi0, T* __restrict__ phi,
                                                       const T* __restrict__ vel, const BCell* __restrict__ cells,
                                                       const int4* __restrict__ tiles,
-                                                      const int* __restrict__ bounds, WorkQ* __restrict__ q) {
+                                                      const int* __restrict__ bounds, WorkQ* __restrict__ q, int k) {
   typedef RingGeom<RING_TX, RING_TY> G;
   typedef VelGeom<RING_TX, RING_TY> V;
   constexpr bool VL = sizeof(T) == 4;
   __shared__ T ring[RG_R * G::PLANE];
   __shared__ T vr[VL ? VR_R * 3 * V::PLANE : 1];
-  claim_tiles(q, bounds, [&](int t) {
+  claim_tiles(q, bounds, k, [&](int t) {
     const int4 tl = tiles[t];
     const BCell c = cells[tl.x];
     const SLP P = cell_slp<3>(c);
@@ -170,15 +180,17 @@ __global__ __launch_bounds__(256) void batch_ring_k(const T* __restrict__ phi0, 
 template <typename T, int S>
 __global__ __launch_bounds__(shape_of(S).tx * shape_of(S).ty) void batch_curv_k(
     const T* __restrict__ phi, T* __restrict__ kappa, const BCell* __restrict__ cells,
-    const int4* __restrict__ tiles, const int* __restrict__ bounds, WorkQ* __restrict__ q) {
+    const int4* __restrict__ tiles, const int* __restrict__ bounds, WorkQ* __restrict__ q, int k) {
   constexpr TileShape sh = shape_of(S);
-  __shared__ T pl[4 * (sh.tx + 2) * (sh.ty + 2)];
-  claim_tiles(q, bounds, [&](int t) {
+  constexpr int PF = ((sh.tx + 2) * (sh.ty + 2) + sh.tx * sh.ty - 1) / (sh.tx * sh.ty);
+  __shared__ T pl[4 * PF * sh.tx * sh.ty];
+  claim_tiles(q, bounds, k, [&](int t) {
     const int4 tl = tiles[t];
     const BCell c = cells[tl.x];
     const unsigned nodes = (unsigned)c.nx * c.ny * c.nz;
-    curv_column<T, sh.tx, sh.ty, sh.tz>(make_rsrc(phi + c.phi_off, nodes * (unsigned)sizeof(T)), kappa + c.phi_off,
-                                        cell_win(c), curv_recips<T>(c.dx, c.dy, c.dz), tl.y, tl.z, tl.w, pl);
+    curv_column2<T, sh.tx, sh.ty, 1, sh.tz>(make_rsrc(phi + c.phi_off, nodes * (unsigned)sizeof(T)),
+                                            kappa + c.phi_off, cell_win(c), curv_recips<T>(c.dx, c.dy, c.dz), tl.y,
+                                            tl.z, tl.w, pl);
   });
 }
 
@@ -188,8 +200,8 @@ template <typename T, int INTERP, int BT>
 __global__ __launch_bounds__(T2X * T2Y) void batch2d_sl_k(const T* __restrict__ phi0, T* __restrict__ phi,
                                                          const T* __restrict__ vel, const BCell* __restrict__ cells,
                                                          const int4* __restrict__ tiles,
-                                                         const int* __restrict__ bounds, WorkQ* __restrict__ q) {
-  claim_tiles(q, bounds, [&](int t) {
+                                                         const int* __restrict__ bounds, WorkQ* __restrict__ q, int k) {
+  claim_tiles(q, bounds, k, [&](int t) {
     const int4 tl = tiles[t];
     const BCell c = cells[tl.x];
     const int i = tl.y + threadIdx.x, j = tl.z + threadIdx.y;
@@ -219,8 +231,8 @@ template <typename T>
 __global__ __launch_bounds__(T2X * T2Y) void batch2d_curv_k(const T* __restrict__ phi, T* __restrict__ kappa,
                                                            const BCell* __restrict__ cells,
                                                            const int4* __restrict__ tiles,
-                                                           const int* __restrict__ bounds, WorkQ* __restrict__ q) {
-  claim_tiles(q, bounds, [&](int t) {
+                                                           const int* __restrict__ bounds, WorkQ* __restrict__ q, int k) {
+  claim_tiles(q, bounds, k, [&](int t) {
     const int4 tl = tiles[t];
     const BCell c = cells[tl.x];
     const int i = tl.y + threadIdx.x, j = tl.z + threadIdx.y;
@@ -333,6 +345,7 @@ struct dcl_batch_s {
   int ntiles[NBASE + 1] = {0};
   WorkQ* queues = nullptr;              // [0] the SL launch, [1] the curvature launch
   int shape = 1, occ = 1;
+  int claim = 0;                        // tiles per claim of both launches (0: static stride)
   int zc[NCLASS] = {32, 32, 32};        // ring z-chunk per size class
   int cus = 256;
   bool dt_set = false;
@@ -434,11 +447,13 @@ struct Plan {
   const int* bcv_bounds;
   WorkQ* qsl;
   WorkQ* qcv;
+  int k;
 };
 
 static int make_plan(dcl_batch b, int s, int occ, Plan& p) {
   const bool f64 = b->precision == DCL_F64;
   p.grid = dim3(grid_blocks(b, occ));
+  p.k = b->claim;
   p.qsl = b->queues;
   p.qcv = b->queues + 1;
   if (b->dim == 2) {
@@ -481,10 +496,11 @@ static int batch_launch(dcl_batch b, int s, int occ, const void* phi0, void* phi
   DCL_TRY(make_plan(b, s, occ, p));
   const BCell* cells = b->cells;
   void* a1[] = {(void*)&phi0, (void*)&phi, (void*)&vel, (void*)&cells, (void*)&p.tsl, (void*)&p.bsl_bounds,
-                (void*)&p.qsl};
+                (void*)&p.qsl, (void*)&p.k};
   DCL_HIP(hipLaunchKernel(p.ksl, p.grid, p.bsl, a1, 0, st));
   const void* cphi = phi;
-  void* a2[] = {(void*)&cphi, (void*)&kappa, (void*)&cells, (void*)&p.tcv, (void*)&p.bcv_bounds, (void*)&p.qcv};
+  void* a2[] = {(void*)&cphi, (void*)&kappa, (void*)&cells, (void*)&p.tcv, (void*)&p.bcv_bounds, (void*)&p.qcv,
+                (void*)&p.k};
   DCL_HIP(hipLaunchKernel(p.kcv, p.grid, p.bcv, a2, 0, st));
   return 0;
 }
@@ -620,6 +636,20 @@ extern "C" int dcl_batch_get_config(dcl_batch b, int* shape, int* occupancy, int
   return 0;
 }
 
+extern "C" int dcl_batch_set_claim(dcl_batch b, int tiles_per_claim) {
+  if (!b) return set_error(DCL_ERR_ARG_NULL, "batch is NULL");
+  if (tiles_per_claim < 0 || tiles_per_claim > 4096)
+    return set_error(DCL_ERR_ARG_WRONG, "tiles per claim %d out of range", tiles_per_claim);
+  b->claim = tiles_per_claim;
+  return 0;
+}
+
+extern "C" int dcl_batch_get_claim(dcl_batch b, int* tiles_per_claim) {
+  if (!b || !tiles_per_claim) return set_error(DCL_ERR_ARG_NULL, "batch or tiles_per_claim is NULL");
+  *tiles_per_claim = b->claim;
+  return 0;
+}
+
 static int set_zc(dcl_batch b, int cls, int zc) {
   if (b->zc[cls] == zc) return 0;
   b->zc[cls] = zc;
@@ -702,9 +732,10 @@ static int time_config(dcl_batch b, int s, int o, const void* phi0, void* phi, v
 }
 
 static int autotune(dcl_batch b, const void* phi0, void* phi, void* kappa, const void* vel, int reps,
-                    double* ms_table, double* zc_table, double* best_ms, hipStream_t st, hipEvent_t e0,
-                    hipEvent_t e1) {
+                    double* ms_table, double* zc_table, double* claim_table, double* best_ms, hipStream_t st,
+                    hipEvent_t e0, hipEvent_t e1) {
   double best = 1e300;
+  b->claim = 0;
   int bs = b->shape, bo = b->occ;
   const int nshapes = b->dim == 2 ? 1 : NSHAPES;
   for (int s = 0; s < NSHAPES; ++s)
@@ -734,18 +765,33 @@ static int autotune(dcl_batch b, const void* phi0, void* phi, void* kappa, const
       if (per < best) best = per;
       else DCL_TRY(set_zc(b, k, keep));
     }
+  // the work distribution: static stride or claims of 1, 4, 16 tiles
+  int bc = 0;
+  for (int c = 0; c < NCLAIM; ++c) {
+    b->claim = CLAIM_OPT[c];
+    double per = 0;
+    DCL_TRY(time_config(b, bs, bo, phi0, phi, kappa, vel, reps, st, e0, e1, &per));
+    if (claim_table) claim_table[c] = per;
+    if (per < best) {
+      best = per;
+      bc = CLAIM_OPT[c];
+    }
+  }
+  b->claim = bc;
   if (best_ms) *best_ms = best;
   return 0;
 }
 
 extern "C" int dcl_batch_autotune(dcl_batch b, const void* phi0, void* phi, void* kappa, const void* vel,
-                                  int reps, double* ms_table, double* zc_table, double* best_ms, void* stream) {
+                                  int reps, double* ms_table, double* zc_table, double* claim_table, double* best_ms,
+                                  void* stream) {
   DCL_TRY(check_step(b, phi0, phi, kappa, vel));
   if (reps < 1) return set_error(DCL_ERR_ARG_SIZ, "reps < 1");
   hipEvent_t e0, e1;
   DCL_HIP(hipEventCreate(&e0));
   DCL_HIP(hipEventCreate(&e1));
-  const int rc = autotune(b, phi0, phi, kappa, vel, reps, ms_table, zc_table, best_ms, as_stream(stream), e0, e1);
+  const int rc =
+      autotune(b, phi0, phi, kappa, vel, reps, ms_table, zc_table, claim_table, best_ms, as_stream(stream), e0, e1);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return rc;
@@ -760,8 +806,9 @@ extern "C" int dcl_batch_graph_create(dcl_batch b, const void* phi0, void* phi, 
   const BCell* cells = b->cells;
   const void* cphi = phi;
   void* a1[] = {(void*)&phi0, (void*)&phi, (void*)&vel, (void*)&cells, (void*)&p.tsl, (void*)&p.bsl_bounds,
-                (void*)&p.qsl};
-  void* a2[] = {(void*)&cphi, (void*)&kappa, (void*)&cells, (void*)&p.tcv, (void*)&p.bcv_bounds, (void*)&p.qcv};
+                (void*)&p.qsl, (void*)&p.k};
+  void* a2[] = {(void*)&cphi, (void*)&kappa, (void*)&cells, (void*)&p.tcv, (void*)&p.bcv_bounds, (void*)&p.qcv,
+                (void*)&p.k};
   hipGraph_t g;
   DCL_HIP(hipGraphCreate(&g, 0));
   hipKernelNodeParams p1;

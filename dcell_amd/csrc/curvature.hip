@@ -49,21 +49,23 @@ __global__ __launch_bounds__(256) void curv_dense(const T* __restrict__ phi, T* 
   kappa[(k * P.w.ny + j) * P.w.nx + i] = curv_node<T, DIM, ST>(phi, P, i, j, k);
 }
 
-// Dense 3D curvature, 2.5D tiling: a block owns a 64 x 4 x 16 column of
-// nodes and walks it with curv_column (a four-plane LDS ring of phi with a
-// one-node halo, one barrier per plane).
-constexpr int CTX = 64, CTY = 4, CTZ = 16;
+// Dense 3D curvature, 2.5D tiling: a block of 64 x 4 threads owns a
+// 64 x 8 x 16 column of nodes (two rows per thread) and walks it with
+// curv_column2 (a four-plane LDS ring of phi with a one-node halo, two planes
+// of prefetch in flight, one barrier per plane).
+constexpr int CTX = 64, CTY = 4, CRY = 2, CTZ = 16;
+constexpr int CPF = ((CTX + 2) * (CTY * CRY + 2) + CTX * CTY - 1) / (CTX * CTY);
 
 template <typename T>
 __global__ __launch_bounds__(256) void curv_tiled3d(const T* __restrict__ phi, T* __restrict__ kappa,
                                                     CurvP P, int ntx, int nty, int ntz, unsigned pbytes) {
-  __shared__ T pl[4 * (CTX + 2) * (CTY + 2)];
+  __shared__ T pl[4 * CPF * CTX * CTY];
   const unsigned nb = (unsigned)(ntx * nty * ntz);
   const unsigned per = nb / 8u;
   const unsigned lb = (per == 0 || blockIdx.x >= per * 8u) ? blockIdx.x : (blockIdx.x % 8u) * per + blockIdx.x / 8u;
   const int tx = lb % ntx, ty = (lb / ntx) % nty, tz = lb / (ntx * nty);
-  curv_column<T, CTX, CTY, CTZ>(make_rsrc(phi, pbytes), kappa, P.w, curv_recips<T>(P.dx, P.dy, P.dz), tx * CTX,
-                                ty * CTY, tz * CTZ, pl);
+  curv_column2<T, CTX, CTY, CRY, CTZ>(make_rsrc(phi, pbytes), kappa, P.w, curv_recips<T>(P.dx, P.dy, P.dz),
+                                      tx * CTX, ty * CTY * CRY, tz * CTZ, pl);
 }
 
 template <typename T, int DIM, int ST, bool COMPACT>
@@ -86,7 +88,8 @@ static int launch_curv(const void* phi, void* kappa, const CurvP& P, const dcl_i
   const T* p = static_cast<const T*>(phi);
   const long long pbytes = (long long)P.w.nx * P.w.ny * P.w.nz * (long long)sizeof(T);
   if (!band && DIM == 3 && use_tiled_curv() && buf_ok(pbytes, 0)) {
-    const int ntx = (P.w.nx + CTX - 1) / CTX, nty = (P.w.ny + CTY - 1) / CTY, ntz = (P.w.nz + CTZ - 1) / CTZ;
+    const int ntx = (P.w.nx + CTX - 1) / CTX, nty = (P.w.ny + CTY * CRY - 1) / (CTY * CRY),
+              ntz = (P.w.nz + CTZ - 1) / CTZ;
     hipLaunchKernelGGL((curv_tiled3d<T>), dim3(ntx * nty * ntz), dim3(CTX, CTY), 0, s, p,
                        static_cast<T*>(kappa), P, ntx, nty, ntz, (unsigned)pbytes);
     DCL_LAUNCHED("curv_tiled3d");

@@ -7,8 +7,8 @@
 // gathers hit L1/L2 well and the update is bound by its instruction count.)
 //
 // Curvature: a rolling four-plane LDS ring of the new phi with a one-node
-// halo; the 19-point stencil reads come from there (curv_column, also the
-// dense 3D curvature kernel).
+// halo and two planes of prefetch; the 19-point stencil reads come from
+// there (curv_column2, also the dense 3D curvature kernel).
 #pragma once
 
 #include "dcl_buf.hpp"
@@ -34,66 +34,6 @@ __device__ __forceinline__ void sl_tile(rsrc_t rp, rsrc_t rv, const T* __restric
   }
 }
 
-// Curvature of one TX x TY x TZ column (TX TY threads, one node column
-// each): the planes k-1, k, k+1 of the new phi with a one-node halo in x and
-// y sit in a four-slot LDS ring (pl, 4 (TX+2)(TY+2) values) while plane k+2
-// is fetched into registers; one barrier per plane.  The staged values'
-// in-plane offsets are computed once and the plane offset is a uniform
-// buffer offset.  kappa is 0 where the stencil leaves the window.  Every
-// thread of the block calls it, and it ends on a barrier.
-template <typename T, int TX, int TY, int TZ>
-__device__ __forceinline__ void curv_column(rsrc_t rp, T* __restrict__ kappa, const KWin& w, const CurvR<T>& r,
-                                            int x0, int y0, int z0, T* pl) {
-  constexpr int PX = TX + 2, PY = TY + 2, PLANE = PX * PY, NT = TX * TY, PF = (PLANE + NT - 1) / NT;
-  const int nx = w.nx, ny = w.ny, nz = w.nz;
-  const int tid = threadIdx.x + TX * threadIdx.y;
-  const unsigned pz = (unsigned)(nx * ny) * (unsigned)sizeof(T);
-  int xo[PF];
-#pragma unroll
-  for (int q = 0; q < PF; ++q) {
-    const int e = tid + q * NT;
-    const int bx = e % PX, by = e / PX;
-    const int gx = clampi(x0 - 1 + bx, 0, nx - 1), gy = clampi(y0 - 1 + by, 0, ny - 1);
-    xo[q] = e < PLANE ? (gy * nx + gx) * (int)sizeof(T) : -1;
-  }
-  auto fetch = [&](int k, T v[PF]) {
-    const int zo = (int)((unsigned)clampi(k, 0, nz - 1) * pz);
-#pragma unroll
-    for (int q = 0; q < PF; ++q)
-      if (xo[q] >= 0) v[q] = BL<T>::one(rp, xo[q], zo);
-  };
-  auto store = [&](int slot, const T v[PF]) {
-#pragma unroll
-    for (int q = 0; q < PF; ++q)
-      if (xo[q] >= 0) pl[slot * PLANE + tid + q * NT] = v[q];
-  };
-  for (int m = 0; m < 3; ++m) {  // planes z0-1, z0, z0+1 in slots 0, 1, 2
-    T v[PF];
-    fetch(z0 - 1 + m, v);
-    store(m, v);
-  }
-  __syncthreads();
-  const int i = x0 + threadIdx.x, j = y0 + threadIdx.y;
-  const bool in_xy = i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2;
-  const int lo = (threadIdx.y + 1) * PX + threadIdx.x + 1;
-  for (int kz = 0; kz < TZ; ++kz) {
-    const int k = z0 + kz;
-    if (k >= nz) break;  // uniform across the block
-    const bool pre = kz + 1 < TZ && k + 1 < nz;
-    T v[PF];
-    if (pre) fetch(k + 2, v);
-    if (i < nx && j < ny) {
-      T c = T(0);
-      if (in_xy && k >= 1 && k <= nz - 2)
-        c = curv3d_planes<T>(pl + (kz & 3) * PLANE + lo, pl + ((kz + 1) & 3) * PLANE + lo,
-                             pl + ((kz + 2) & 3) * PLANE + lo, PX, r);
-      kappa[((size_t)k * ny + j) * nx + i] = c;
-    }
-    if (pre) store((kz + 3) & 3, v);
-    __syncthreads();
-  }
-}
-
 // Curvature of one TX x (TY RY) x TZ column worked by TX x TY threads with
 // RY node rows each (rows ty, ty + TY, ...): planes k-1, k, k+1 of phi with a
 // one-node halo sit in a four-slot LDS ring (pl, 4 PF TX TY values, PF =
@@ -102,8 +42,8 @@ __device__ __forceinline__ void curv_column(rsrc_t rp, T* __restrict__ kappa, co
 // values load a clamped in-window value into the slot's padding, so the
 // loads and LDS stores of the steady state are unconditional and the wait
 // for plane k+2 leaves plane k+3's loads in flight.  kappa is 0 where the
-// stencil leaves the window; the arithmetic is curv3d_planes, as in
-// curv_column.  Every thread of the block calls it, and it ends on a barrier.
+// stencil leaves the window; the arithmetic is curv3d_planes (dcl_curv.hpp).
+// Every thread of the block calls it, and it ends on a barrier.
 template <typename T, int TX, int TY, int RY, int TZ>
 __device__ __forceinline__ void curv_column2(rsrc_t rp, T* __restrict__ kappa, const KWin& w, const CurvR<T>& r,
                                              int x0, int y0, int z0, T* pl) {

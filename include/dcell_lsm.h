@@ -97,6 +97,10 @@ int dcl_memcpy_htod(void *dst, const void *src, size_t bytes, void *stream);
 int dcl_memcpy_dtoh(void *dst, const void *src, size_t bytes, void *stream);
 int dcl_memcpy_dtod(void *dst, const void *src, size_t bytes, void *stream);
 int dcl_stream_synchronize(void *stream);
+/* device-to-device copy by a streaming kernel (16-byte non-temporal loads
+ * and stores, several in flight per thread); 16-byte aligned buffers and
+ * size.  The benchmark's measured HBM ceiling.  Asynchronous. */
+int dcl_stream_copy(void *dst, const void *src, size_t bytes, void *stream);
 /* page-locked host memory (staging of asynchronous copies) */
 int dcl_host_malloc(void **ptr, size_t bytes);
 int dcl_host_free(void *ptr);
@@ -356,8 +360,8 @@ int dcl_error_norms(const void *a, int prec_a, const void *b, int prec_b, int64_
  * the cells' windows are packed back to back in one phi buffer, one kappa
  * buffer and one velocity buffer (dim components per node, as a velgrid),
  * cut into tiles, and one step is two persistent launches over the packed
- * tile list (SL then curvature, as dcl_levelset_step per cell): every block
- * claims tiles from a per-XCD work queue of cost-balanced chunks, so cells of
+ * tile list (SL then curvature, as dcl_levelset_step per cell), each XCD
+ * working through a chunk of the list of about equal cost, so cells of
  * uneven sizes load the whole chip.  A batch holds 3D cells or 2D cells, not
  * both.  No counterpart in the reference.  The velocity of each cell shares
  * the cell's window. */
@@ -367,6 +371,8 @@ typedef struct dcl_batch_s *dcl_batch;
  * the ring z-chunks the autotuner tries for each (16, 32, 64, 128) */
 #define DCL_BATCH_CLASSES 3
 #define DCL_BATCH_ZCHUNKS 4
+/* tiles per claim the autotuner tries: 0 (static stride), 1, 4, 16 */
+#define DCL_BATCH_CLAIMS 4
 
 int dcl_batch_create(int32_t ncells, const dcl_window *wins, int precision, int interp,
                      int backtrace, dcl_batch *batch);
@@ -384,6 +390,13 @@ int dcl_batch_set_dt(dcl_batch batch, const double *dt);
 int dcl_batch_set_config(dcl_batch batch, int shape, int occupancy);
 int dcl_batch_get_config(dcl_batch batch, int *shape, int *occupancy, int32_t tile[3],
                          int *blocks);
+/* Work distribution of both launches: the tile list is cut into one chunk
+ * of about equal cost per XCD; with tiles_per_claim 0 (the default) the
+ * blocks of an XCD take its chunk's tiles in a fixed stride, with k > 0 they
+ * claim k tiles at a time from a per-chunk atomic counter and then take
+ * tiles from the other chunks */
+int dcl_batch_set_claim(dcl_batch batch, int tiles_per_claim);
+int dcl_batch_get_claim(dcl_batch batch, int *tiles_per_claim);
 /* ring z-chunk of one size class (default 32) */
 int dcl_batch_set_class_zchunk(dcl_batch batch, int cls, int zchunk);
 /* per size class: the largest cell axis it holds (-1: unbounded), its cell
@@ -402,14 +415,15 @@ int dcl_batch_step(dcl_batch batch, const void *phi0, void *phi, void *kappa, co
 int dcl_batch_band_step(dcl_batch batch, const void *phi0, void *phi, void *kappa,
                         const void *vel, const int32_t *band, int64_t count, void *stream);
 /* Times every configuration (reps steps each, after one warm-up step) on
- * these buffers and keeps the fastest, then, when that uses ring columns,
- * each size class's z-chunk in turn.  ms_table (optional, 36 entries,
- * shape-major, -1 where not timed) and zc_table (optional, DCL_BATCH_CLASSES
- * x DCL_BATCH_ZCHUNKS, -1 where not timed) receive the per-step times.
- * Synchronous. */
+ * these buffers with the static stride and keeps the fastest, then, when that
+ * uses ring columns, each size class's z-chunk in turn, then the tiles per
+ * claim.  ms_table (optional, 36 entries, shape-major, -1 where not timed),
+ * zc_table (optional, DCL_BATCH_CLASSES x DCL_BATCH_ZCHUNKS, -1 where not
+ * timed) and claim_table (optional, DCL_BATCH_CLAIMS entries) receive the
+ * per-step times.  Synchronous. */
 int dcl_batch_autotune(dcl_batch batch, const void *phi0, void *phi, void *kappa,
                        const void *vel, int reps, double *ms_table, double *zc_table,
-                       double *best_ms, void *stream);
+                       double *claim_table, double *best_ms, void *stream);
 /* One step with these buffers recorded as an instantiated hipGraph (two
  * kernel nodes); launch it with dcl_batch_graph_launch. */
 int dcl_batch_graph_create(dcl_batch batch, const void *phi0, void *phi, void *kappa,
