@@ -25,7 +25,7 @@ EXPORTS = (
     "dcl_host_malloc", "dcl_host_free", "dcl_stream_copy",
     "dcl_sl_advect", "dcl_curvature", "dcl_levelset_step",
     "dcl_pls_advect", "dcl_pls_ws_create", "dcl_pls_ws_destroy", "dcl_pls_correct",
-    "dcl_pls_adjust_radii", "dcl_max_velocity", "dcl_band_compact", "dcl_irregular_nodes",
+    "dcl_pls_adjust_radii", "dcl_max_velocity", "dcl_band_compact", "dcl_band_compact_static", "dcl_irregular_nodes",
     "dcl_surface_quantities", "dcl_get_velocity", "dcl_reinitialize", "dcl_reinitialize_parallel",
     "dcl_real64_pack", "dcl_irregular_node_rows",
     "dcl_interface_metrics", "dcl_error_norms", "dcl_sl_adjoint",
@@ -98,6 +98,7 @@ def _declare(L):
         "dcl_pls_adjust_radii": ([vp, i64, vp, W, dbl, dbl, i32, i32, vp], i32),
         "dcl_max_velocity": ([V, vp, i64, i64, i32, i32, C.POINTER(dbl), vp], i32),
         "dcl_band_compact": ([vp, W, dbl, i32, vp, i64, C.POINTER(i64), vp], i32),
+        "dcl_band_compact_static": ([vp, W, dbl, i32, vp, i64, C.POINTER(i64), vp], i32),
         "dcl_irregular_nodes": ([vp, W, vp, i64, i32, vp, i64, C.POINTER(i64), vp], i32),
         "dcl_surface_quantities": ([vp, W, vp, i64, i32, vp, vp], i32),
         "dcl_real64_pack": ([vp, i32, i64, vp, vp], i32),

@@ -9,6 +9,8 @@
 //   dcl_error_norms       max / L2 differences
 // Sums are reduced in a fixed order (block partials, then one block), so the
 // results do not depend on scheduling.
+#include <mutex>
+
 #include "dcl_host.hpp"
 
 namespace dcl {
@@ -238,15 +240,44 @@ extern "C" int dcl_max_velocity(const dcl_velocity* vel, const void* nodes, int6
   return fetch(max_out, part + nb, 1, s);
 }
 
+// scratch of a band compaction: total | tile sums | block counts
+static size_t band_scratch_bytes(int64_t n) {
+  const int64_t nb = grid1d(n, RB);
+  return 16 + sizeof(unsigned long long) * scan_tiles(nb) + sizeof(unsigned) * nb;
+}
+
+// The allocation-free band compaction's scratch: one array per device, held
+// until the process ends, sized for windows of up to BAND_STATIC_MAX_NODES
+// nodes.  Host code uses the device address that hipGetSymbolAddress returns
+// for it on each device, never the array's name.
+constexpr int64_t BAND_STATIC_MAX_NODES = (int64_t)1 << 28;
+constexpr size_t BAND_STATIC_BYTES = (4u << 20) + (64u << 10);
+static __device__ __align__(256) unsigned char band_static_scratch[BAND_STATIC_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_static_mu;
+static void* g_static_ptr[MAX_DEVICES];
+
+static int band_static_scratch_ptr(void** ptr) {
+  int dev = 0;
+  DCL_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) return set_error(DCL_ERR_SUP, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_static_mu);
+  if (!g_static_ptr[dev]) {
+    void* p = nullptr;
+    DCL_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(band_static_scratch)));
+    g_static_ptr[dev] = p;
+  }
+  *ptr = g_static_ptr[dev];
+  return 0;
+}
+
 template <typename T>
 static int band_compact_t(const void* phi, const dcl_window* win, double width, dcl_icoor* out,
-                          int64_t cap, int64_t* count_out, hipStream_t s) {
+                          int64_t cap, int64_t* count_out, void* sc, hipStream_t s) {
   const int64_t n = win_nodes(*win);
   const int64_t nb = grid1d(n, RB);
-  // scratch: total | tile sums | block counts
   const int64_t ntiles = scan_tiles(nb);
-  void* sc = nullptr;
-  DCL_TRY(scratch(16 + sizeof(unsigned long long) * ntiles + sizeof(unsigned) * nb, &sc));
   unsigned long long* total = static_cast<unsigned long long*>(sc);
   unsigned long long* sums = total + 2;
   unsigned* counts = reinterpret_cast<unsigned*>(sums + ntiles);
@@ -267,18 +298,42 @@ static int band_compact_t(const void* phi, const dcl_window* win, double width, 
   return 0;
 }
 
-extern "C" int dcl_band_compact(const void* phi, const dcl_window* win, double width, int precision,
-                                dcl_icoor* band_out, int64_t capacity, int64_t* count_out,
-                                void* stream) {
+static int check_band_compact(const void* phi, const dcl_window* win, int precision, const dcl_icoor* band_out,
+                              int64_t capacity, const int64_t* count_out) {
   DCL_TRY(check_precision(precision));
   if (!phi) return set_error(DCL_ERR_ARG_NULL, "phi is NULL");
   if (!count_out) return set_error(DCL_ERR_ARG_NULL, "count_out is NULL");
   if (capacity < 0) return set_error(DCL_ERR_ARG_SIZ, "capacity < 0");
   if (capacity > 0 && !band_out) return set_error(DCL_ERR_ARG_NULL, "band_out is NULL");
-  DCL_TRY(check_window(win, "phi", 1));
+  return check_window(win, "phi", 1);
+}
+
+extern "C" int dcl_band_compact(const void* phi, const dcl_window* win, double width, int precision,
+                                dcl_icoor* band_out, int64_t capacity, int64_t* count_out,
+                                void* stream) {
+  DCL_TRY(check_band_compact(phi, win, precision, band_out, capacity, count_out));
   hipStream_t s = as_stream(stream);
-  if (precision == DCL_F64) return band_compact_t<double>(phi, win, width, band_out, capacity, count_out, s);
-  return band_compact_t<float>(phi, win, width, band_out, capacity, count_out, s);
+  void* sc = nullptr;
+  DCL_TRY(scratch(band_scratch_bytes(win_nodes(*win)), &sc));
+  if (precision == DCL_F64) return band_compact_t<double>(phi, win, width, band_out, capacity, count_out, sc, s);
+  return band_compact_t<float>(phi, win, width, band_out, capacity, count_out, sc, s);
+}
+
+extern "C" int dcl_band_compact_static(const void* phi, const dcl_window* win, double width, int precision,
+                                       dcl_icoor* band_out, int64_t capacity, int64_t* count_out,
+                                       void* stream) {
+  DCL_TRY(check_band_compact(phi, win, precision, band_out, capacity, count_out));
+  const int64_t n = win_nodes(*win);
+  if (n > BAND_STATIC_MAX_NODES || band_scratch_bytes(n) > BAND_STATIC_BYTES)
+    return set_error(DCL_ERR_ARG_SIZ,
+                     "window of %lld nodes: the allocation-free band compaction takes windows of up to "
+                     "%lld nodes (its static scratch holds %zu bytes)",
+                     (long long)n, (long long)BAND_STATIC_MAX_NODES, BAND_STATIC_BYTES);
+  void* sc = nullptr;
+  DCL_TRY(band_static_scratch_ptr(&sc));
+  hipStream_t s = as_stream(stream);
+  if (precision == DCL_F64) return band_compact_t<double>(phi, win, width, band_out, capacity, count_out, sc, s);
+  return band_compact_t<float>(phi, win, width, band_out, capacity, count_out, sc, s);
 }
 
 extern "C" int dcl_interface_metrics(const void* phi, const void* kappa, const dcl_window* win,

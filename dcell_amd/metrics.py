@@ -28,13 +28,27 @@ def max_velocity(vel, vel_win: GridWindow, nodes, stride_bytes: int = 12, udof=0
     return out.value
 
 
-def band_compact(phi, win: GridWindow, width: float, capacity: int | None = None):
-    """Nodes with |phi| < width in raster order, as an (n, 3) int32 device tensor."""
+def band_compact(phi, win: GridWindow, width: float, capacity: int | None = None, out=None):
+    """Nodes with |phi| < width in raster order, as an (n, 3) int32 device
+    tensor.  With ``out`` (an int32 device tensor shaped (capacity, 3)) the
+    nodes are written there and the allocation-free entry point runs
+    (dcl_band_compact_static: no device allocation, windows up to 2^28 nodes)."""
     _dev(phi, "phi")
-    cap = win.nodes if capacity is None else capacity
-    out = torch.empty((max(cap, 1), 3), dtype=torch.int32, device=phi.device)
+    precision_of(phi)
+    if phi.numel() != win.nodes:
+        raise ValueError(f"phi has {phi.numel()} elements, window needs {win.nodes}")
+    fn = "dcl_band_compact"
+    if out is not None:
+        _dev(out, "out", torch.int32)
+        if out.dim() != 2 or out.shape[1] != 3:
+            raise ValueError("out must be shaped (capacity, 3)")
+        cap = out.shape[0]
+        fn = "dcl_band_compact_static"
+    else:
+        cap = win.nodes if capacity is None else capacity
+        out = torch.empty((max(cap, 1), 3), dtype=torch.int32, device=phi.device)
     count = C.c_int64()
-    L.call("dcl_band_compact", _p(phi), C.byref(win.c()), float(width), precision_of(phi), _p(out),
+    L.call(fn, _p(phi), C.byref(win.c()), float(width), precision_of(phi), _p(out),
            cap, C.byref(count), stream_handle())
     return out[:min(count.value, cap)], count.value
 

@@ -203,6 +203,19 @@ int dcl_band_compact(const void *phi, const dcl_window *win, double width,
                      int precision, dcl_icoor *band_out, int64_t capacity,
                      int64_t *count_out, void *stream);
 
+/* dcl_band_compact without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, results and synchronisation as dcl_band_compact; its
+ * per-block counts and scan sums live in a static device array of
+ * 4 MiB + 64 KiB per GPU (held until the process ends), so windows of up to
+ * 2^28 nodes are accepted and larger ones fail with DCL_ERR_ARG_SIZ.  Every
+ * call on one GPU shares that array: calls that overlap on one GPU (other
+ * host threads, other streams) race and must be serialised by the caller. */
+int dcl_band_compact_static(const void *phi, const dcl_window *win, double width,
+                            int precision, dcl_icoor *band_out, int64_t capacity,
+                            int64_t *count_out, void *stream);
+
 /* One record of the irregular-node list: the fields that
  * LevelSetUpdateIrregularNodeList sets (LevelSetMethod/IrregularNodes3D.c:
  * 53-113, IrregularNodes.c:69-137); the reference's IrregularNode
