@@ -156,7 +156,8 @@ __global__ __launch_bounds__(shape_of(S).tx * shape_of(S).ty) void batch_sl_k(
 constexpr int RING_TX = 32, RING_TY = 8;
 
 template <typename T, int INTERP, int BT>
-__global__ __launch_bounds__(256) void batch_ring_k(const T* __restrict__ phi0, T* __restrict__ phi,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ring_waves<T, sizeof(T) == 4>(), 8))) void
+batch_ring_k(const T* __restrict__ phi0, T* __restrict__ phi,
                                                       const T* __restrict__ vel, const BCell* __restrict__ cells,
                                                       const int4* __restrict__ tiles,
                                                       const int* __restrict__ bounds, WorkQ* __restrict__ q, int k) {

@@ -31,16 +31,25 @@ __host__ __device__ inline CurvR<T> curv_recips(double dx, double dy, double dz)
 // (pointers at node (i, j) of each plane, row stride Y): the 7-point
 // derivatives, the 12 edge points of the cross terms, NaN -> 0.  Divisions by
 // the spacings are multiplications by their reciprocals.
-// c / sqrt(g^3), g = |grad phi|^2: in FP64 the IEEE quotient and square root
-// of the reference expression; in FP32 c * rsq(g)^3 with the hardware
-// reciprocal square root (a few ulp; 0 * inf = NaN at g = 0 as before)
+// c / sqrt(g^3), g = |grad phi|^2, as c y^3 with y = 1/sqrt(g): in FP32 the
+// hardware reciprocal square root (a few ulp); in FP64 its estimate refined
+// by two Newton steps y <- y + y (1 - g y^2) / 2 (within an ulp or two of the
+// reference's quotient and square root, a quarter of their instructions).
+// At g = 0 the result is NaN (0 * inf), which the callers map to 0 as the
+// reference maps its 0 / 0; c is 0 there, every term carrying a gradient
+// component.
 template <typename T>
 __device__ __forceinline__ T grad_pow3_div(T c, T g) {
   if constexpr (sizeof(T) == 4) {
     const T q = __builtin_amdgcn_rsqf(g);
     return c * (q * q * q);
   } else {
-    return c / sqrt(g * g * g);
+    double y = __builtin_amdgcn_rsq(g);
+    double e = fma(-(g * y), y, 1.0);
+    y = fma(0.5 * y, e, y);
+    e = fma(-(g * y), y, 1.0);
+    y = fma(0.5 * y, e, y);
+    return c * (y * y * y);
   }
 }
 

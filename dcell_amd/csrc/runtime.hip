@@ -86,7 +86,7 @@ int scratch(size_t bytes, void** ptr) {
 // Streaming device copy: 16-byte elements, each thread keeps SC_U loads in
 // flight (non-temporal loads and stores: the data is touched once), grid
 // stride over the buffer.
-constexpr int SC_U = 4, SC_BLOCKS = 8192;
+constexpr int SC_U = 4, SC_BLOCKS = 16384;
 typedef unsigned int sc_u32x4 __attribute__((ext_vector_type(4)));
 
 __global__ __launch_bounds__(256) void stream_copy_k(const sc_u32x4* __restrict__ src, sc_u32x4* __restrict__ dst,
