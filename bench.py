@@ -175,8 +175,8 @@ def _probe_dense(n, dtype, sl=True, curv=True):
 
 def probe_batch(args):
     """3 steps of the 512-cell FP32 batch (RK2 + tricubic) in the ring
-    configuration the autotuner picks on the benchmark's box (shape 7, 4
-    blocks per CU, static stride, z-chunk 32)."""
+    configuration the autotuner picks for it (shape 7, occupancy index 2:
+    8 blocks per CU, static stride, the default z-chunk per size class)."""
     import torch
     import dcell_amd as D
     from dcell_amd import configs
@@ -192,7 +192,8 @@ def probe_batch(args):
         b.cell(vel, c, 3).copy_(v)
         dts.append(dt)
     b.set_dt(dts)
-    b.set_config(7, 1)
+    b.set_config(7, 2)
+    b.set_claim(0)
     for _ in range(3):
         b.step(phi0, phi1, kap, vel)
     torch.cuda.synchronize()
@@ -210,6 +211,7 @@ def probe_adjoint(args):
 PROBES = {"sl64": None, "sl32": probe_sl32, "curv": probe_curv, "batch": probe_batch, "adjoint": probe_adjoint}
 
 # the counters of the summaries, one rocprofv3 --pmc pass per line
+XCDS = 8  # MI355X: accelerator complex dies, each with its own GRBM_GUI_ACTIVE
 COUNTER_PASSES = (
     "SQ_WAVES SQ_INSTS_VALU SQ_INSTS_VMEM_RD SQ_INSTS_VMEM_WR SQ_INSTS_LDS SQ_INSTS_SALU SQ_INSTS_BRANCH SQ_WAIT_INST_ANY",
     "VALUBusy VALUUtilization MeanOccupancyPerCU TA_BUSY_avr TD_TD_BUSY_sum SQ_LDS_BANK_CONFLICT SQ_ACTIVE_INST_LDS "
@@ -247,12 +249,15 @@ def counter_summary(args):
                      f"python3 bench.py --probe {probe_name}\n")
         fo.write(f"#   python3 bench.py --counter-summary {name} {probe_name} "
                  + " ".join(f"gpurun_out/pmc_{probe_name}_{q}" for q in range(len(COUNTER_PASSES))) + "\n")
-        fo.write("# values: mean per dispatch; '/wave' = per wavefront of 64 lanes (one node per lane);\n"
-                 "# FETCH_SIZE / WRITE_SIZE in KiB; busy %: TA_BUSY_avr and TD_TD_BUSY_sum / 256 CUs over "
-                 "GRBM_GUI_ACTIVE\n")
+        fo.write("# values: mean per dispatch; '/wave' = per wavefront of 64 lanes;\n"
+                 "# FETCH_SIZE / WRITE_SIZE in KiB (raw: the copy kernel reads 2x its FETCH_SIZE, see the\n"
+                 f"# traffic summary); GRBM_GUI_ACTIVE sums the {XCDS} XCDs' busy cycles, so the kernel's\n"
+                 f"# cycles are GRBM_GUI_ACTIVE / {XCDS} (about 2.2 GHz x its duration under the profiler);\n"
+                 "# busy %: TA_BUSY_avr, TD_TD_BUSY_sum / 256 CUs and SQ_LDS_BANK_CONFLICT / 256 CUs over\n"
+                 "# those cycles; VALUBusy as rocprofv3 derives it\n")
         for k in sorted(vals, key=lambda k: -len(vals[k])):
             cs = vals[k]
-            if "SQ_WAVES" not in cs and "FETCH_SIZE" not in cs:
+            if not any(c in cs for c in ("SQ_WAVES", "FETCH_SIZE", "GRBM_GUI_ACTIVE")):
                 continue
             mean = {cn: sum(v) / len(v) for cn, v in cs.items()}
             fo.write(f"\nkernel: {k[:160]}\n  dispatches: {max(len(v) for v in cs.values())}\n")
@@ -264,10 +269,14 @@ def counter_summary(args):
                 fo.write(line + "\n")
             gui = mean.get("GRBM_GUI_ACTIVE")
             if gui:
+                gui /= XCDS
+                fo.write(f"  kernel cycles (GRBM_GUI_ACTIVE / {XCDS}): {gui:.6g}\n")
                 if "TA_BUSY_avr" in mean:
                     fo.write(f"  TA busy: {100 * mean['TA_BUSY_avr'] / gui:.1f} %\n")
                 if "TD_TD_BUSY_sum" in mean:
                     fo.write(f"  TD busy: {100 * mean['TD_TD_BUSY_sum'] / 256 / gui:.1f} %\n")
+                if "SQ_LDS_BANK_CONFLICT" in mean:
+                    fo.write(f"  LDS bank-conflict cycles: {100 * mean['SQ_LDS_BANK_CONFLICT'] / 256 / gui:.1f} %\n")
     print(open(out).read())
 
 
@@ -343,6 +352,38 @@ def read_traffic(n):
     if int(kv["n"]) != n:
         return None, None
     return kv["kernel"], float(kv["fetch_bytes_per_launch"]) + float(kv["write_bytes_per_launch"])
+
+
+COUNTERS_FILE = os.path.join(ROOT, "profiles", "sl_f64_cubic_rk2_256.counters.txt")
+
+
+def read_limiter(kernel_prefix="dcl::sl_ring3d<double, 1, 1,"):
+    """The unit that limits the headline SL kernel, from the committed counter
+    summary: the busiest of the vector ALU, the texture address unit and the
+    texture data unit (percent of the kernel's cycles), or None without a
+    summary."""
+    try:
+        with open(COUNTERS_FILE) as f:
+            lines = f.read().splitlines()
+    except OSError:
+        return None
+    vals, inside = {}, False
+    for ln in lines:
+        if ln.startswith("kernel:"):
+            inside = kernel_prefix in ln
+        elif inside and ":" in ln:
+            k, v = ln.strip().split(":", 1)
+            vals[k] = v.strip()
+    busy = {}
+    for unit, key in (("VALU", "VALUBusy"), ("TA (texture address)", "TA busy"),
+                      ("TD (texture data)", "TD busy")):
+        if key in vals:
+            busy[unit] = float(vals[key].split()[0])
+    if not busy:
+        return None
+    top = max(busy, key=busy.get)
+    return {"unit": top, "busy_pct": busy,
+            "source": os.path.relpath(COUNTERS_FILE, ROOT)}
 
 
 def host_cpu():
@@ -990,10 +1031,12 @@ def reinit_run(n=256, runs=5, exact=True):
 def dropin_run(n=256, calls=10):
     """The C drop-in as DCell calls it: LevelSetAdvectSL on an n^3 FP64 level
     set (the headline sphere) over its 6-cell band, with a full n^3 velocity
-    grid in host memory.  Each call uploads phi0 and the velocity window,
-    advects the band nodes on the GPU and writes them back into the host grid,
-    so this is the PCIe-inclusive rate; the same band update through the C ABI
-    with everything resident in HBM is timed beside it."""
+    grid in host memory.  Each call uploads the boxes of phi0 and of the
+    velocity grid that the band's departures reach, advects the band nodes on
+    the GPU and writes them back into the host grid, so this is the
+    PCIe-inclusive rate (bytes moved per call from the drop-in's own
+    counters); the same band update through the C ABI with everything
+    resident in HBM is timed beside it."""
     import ctypes as C
     import torch
     import dcell_amd as D
@@ -1016,11 +1059,15 @@ def dropin_run(n=256, calls=10):
     vel[:] = configs.deformation_velocity(n)
     dt = configs.deformation_dt(n, 0)
     DI.check(L.LevelSetAdvectSL(ls, vg, C.c_double(dt)), "LevelSetAdvectSL")  # warm-up
+    up0, down0 = C.c_ulonglong(), C.c_ulonglong()
+    up1, down1 = C.c_ulonglong(), C.c_ulonglong()
+    DI.check(L.LevelSetGPUTransferStats(C.byref(up0), C.byref(down0)), "LevelSetGPUTransferStats")
     ts = []
     for _ in range(calls):
         t0 = time.perf_counter()
         DI.check(L.LevelSetAdvectSL(ls, vg, C.c_double(dt)), "LevelSetAdvectSL")
         ts.append(time.perf_counter() - t0)
+    DI.check(L.LevelSetGPUTransferStats(C.byref(up1), C.byref(down1)), "LevelSetGPUTransferStats")
     t_drop = float(np.median(ts))
     # device-resident: the same band update through the C ABI
     dvel = torch.from_numpy(vel.copy()).to("cuda:0")
@@ -1036,10 +1083,11 @@ def dropin_run(n=256, calls=10):
     t_dev = e0.elapsed_time(e1) * 1e-3 / calls
     L.GridDestroy(vg)
     L.LevelSetDestroy(ls)
-    moved = (n ** 3) * 8 * 4  # phi0 and the 3-component velocity uploaded per call
     return dict(n=n, band_nodes=nb, dropin_ms_per_call=t_drop * 1e3, dropin_band_updates_per_s=nb / t_drop,
-                host_bytes_uploaded_per_call=moved, device_resident_ms_per_call=t_dev * 1e3,
-                device_resident_band_updates_per_s=nb / t_dev)
+                host_bytes_uploaded_per_call=(up1.value - up0.value) // calls,
+                host_bytes_downloaded_per_call=(down1.value - down0.value) // calls,
+                whole_grids_bytes=(n ** 3) * 8 * 4, host_threads=int(os.environ.get("DCELL_GPU_HOST_THREADS", 8)),
+                device_resident_ms_per_call=t_dev * 1e3, device_resident_band_updates_per_s=nb / t_dev)
 
 
 def copy_ceiling():
@@ -1171,6 +1219,7 @@ def main():
     line["measured_copy_gbs"] = own / 1e9
     line["torch_copy_gbs"] = torch_copy / 1e9
     line["roofline"]["frac_of_measured_copy"] = achieved / own
+    line["roofline"]["limiter"] = read_limiter()
     if not args.no_forms:
         line["forms"] = forms(args)
     if not args.no_cpu:

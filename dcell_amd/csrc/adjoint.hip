@@ -17,6 +17,7 @@
 // of their MAC velocity scatter before the HBM atomic.  Outer two layers:
 // phi = phi0 there, so dL/dphi0 += lam_next.
 #include <algorithm>
+#include <type_traits>
 
 #include "dcl_buf.hpp"
 #include "dcl_host.hpp"
@@ -376,19 +377,29 @@ __global__ __launch_bounds__(256) void sl_adjoint_k(const T* __restrict__ phi0, 
 //   finish (once):   dL/dvel += the MAC / node-velocity scatter of hsum, the
 //                    same scatter as sl_adjoint_k's per step.
 // The record holds {bx, by, bz, flags} (int4, flags: bit a = axis a clamped,
-// bit 3 = interior node) and the three fractions in double (a double4), from
-// which the weights are formed exactly as the forward forms them.
+// bit 3 = interior node) and the three fractions, from which the weights are
+// formed exactly as the forward forms them: in double (a double4), except
+// for the FP32 tricubic, whose weights start from the fractions rounded to
+// float (a float4: the record is sized for the double4 and this uses half
+// of its fraction area).
 constexpr size_t REC_BYTES = sizeof(int4) + sizeof(double4);
+template <typename T, int INTERP>
+using Frac4 = typename std::conditional<sizeof(T) == 4 && INTERP == DCL_CUBIC, float4, double4>::type;
 
 template <typename T, int INTERP, int BT>
 __global__ __launch_bounds__(256) void adj_prepare_k(const T* __restrict__ vel, AdjP P, int4* __restrict__ rb,
-                                                     double4* __restrict__ rf) {
+                                                     void* __restrict__ rfv) {
+  // the fraction record's type depends on INTERP; it is passed untyped so the
+  // kernel's mangled name holds no expression over DCL_CUBIC (an enumerator
+  // of an unnamed enum, whose internal name the host and device compilations
+  // number differently)
+  Frac4<T, INTERP>* __restrict__ rf = static_cast<Frac4<T, INTERP>*>(rfv);
   const int64_t n = (int64_t)P.w.nx * P.w.ny * P.w.nz;
   for (int64_t id = (int64_t)blockIdx.x * 256 + threadIdx.x; id < n; id += (int64_t)gridDim.x * 256) {
     const int i = (int)(id % P.w.nx), j = (int)((id / P.w.nx) % P.w.ny), k = (int)(id / ((int64_t)P.w.nx * P.w.ny));
     const bool inner = i >= 2 && i <= P.w.nx - 3 && j >= 2 && j <= P.w.ny - 3 && k >= 2 && k <= P.w.nz - 3;
     int4 q = make_int4(0, 0, 0, 0);
-    double4 f4 = make_double4(0.0, 0.0, 0.0, 0.0);
+    Frac4<T, INTERP> f4 = {0, 0, 0, 0};
     if (inner) {
       const int X = P.w.px + i, Yg = P.w.py + j, Zg = P.w.pz + k;
       const double Xd[3] = {(double)X, (double)Yg, (double)Zg};
@@ -438,9 +449,10 @@ __global__ __launch_bounds__(256) void adj_prepare_k(const T* __restrict__ vel, 
 // box, lane combining and flush as sl_adjoint_k.
 template <typename T, int INTERP>
 __global__ __launch_bounds__(256) void adj_step_k(const T* __restrict__ phi0, const int4* __restrict__ rb,
-                                                  const double4* __restrict__ rf,
+                                                  const void* __restrict__ rfv,
                                                   const T* __restrict__ lam_next, T* __restrict__ lam,
                                                   T* __restrict__ hsum, AdjP P) {
+  const Frac4<T, INTERP>* __restrict__ rf = static_cast<const Frac4<T, INTERP>*>(rfv);
   __shared__ T box[LBZ * LBY * LBX];
   const int tid = threadIdx.x + ABX * (threadIdx.y + ABY * threadIdx.z);
   for (int i = tid; i < LBZ * LBY * LBX; i += 256) box[i] = T(0);
@@ -457,13 +469,18 @@ __global__ __launch_bounds__(256) void adj_step_k(const T* __restrict__ phi0, co
     const T g = lam_next[id];
     if (g != T(0)) atomic_add(&lam[id], g);
   }
-  T g = T(0);
+  // the node's record, its lam_next and its sensitivity sum so far, loaded
+  // together (hsum's latency overlaps the others')
+  T g = T(0), h0[3] = {T(0), T(0), T(0)};
   int4 q = make_int4(0, 0, 0, 0);
-  double4 f4 = make_double4(0.0, 0.0, 0.0, 0.0);
+  Frac4<T, INTERP> f4 = {0, 0, 0, 0};
+  T* hs = hsum + 3 * (int64_t)id;
   if (inner) {
     g = lam_next[id];
     q = rb[id];
     f4 = rf[id];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) h0[a] = hs[a];
   }
   const bool act = inner && g != T(0);
   const int NP = INTERP == DCL_CUBIC ? 4 : 2;
@@ -473,7 +490,7 @@ __global__ __launch_bounds__(256) void adj_step_k(const T* __restrict__ phi0, co
   int bx0 = 0, by0 = 0, bz0 = 0;
   bool in_box = false;
   if (act) {
-    const double fr[3] = {f4.x, f4.y, f4.z};
+    const double fr[3] = {(double)f4.x, (double)f4.y, (double)f4.z};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       if (INTERP == DCL_CUBIC) {
@@ -561,10 +578,9 @@ __global__ __launch_bounds__(256) void adj_step_k(const T* __restrict__ phi0, co
     }
   }
   if (act) {
-    T* hs = hsum + 3 * (int64_t)id;
 #pragma unroll
     for (int a = 0; a < 3; ++a)
-      if (!(q.w & (1 << a))) hs[a] += (T)(-P.c[a]) * g * G[a];
+      if (!(q.w & (1 << a))) hs[a] = h0[a] + (T)(-P.c[a]) * g * G[a];
   }
   __syncthreads();
   for (int t = tid; t < LBZ * LBY * LBX; t += 256) {
@@ -705,10 +721,10 @@ template <typename T>
 static int adj_prepare_t(int interp, int bt, const void* vel, const AdjP& P, void* record, hipStream_t s) {
   const int64_t n = (int64_t)P.w.nx * P.w.ny * P.w.nz;
   int4* rb = static_cast<int4*>(record);
-  double4* rf = reinterpret_cast<double4*>(rb + n);
   const unsigned nb = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
   const T* v = static_cast<const T*>(vel);
-#define DCL_PREP(I, B) hipLaunchKernelGGL((adj_prepare_k<T, I, B>), dim3(nb), dim3(256), 0, s, v, P, rb, rf)
+#define DCL_PREP(I, B)                                                                  \
+  hipLaunchKernelGGL((adj_prepare_k<T, I, B>), dim3(nb), dim3(256), 0, s, v, P, rb, static_cast<void*>(rb + n))
   if (interp == DCL_CUBIC && bt == DCL_RK2) DCL_PREP(DCL_CUBIC, DCL_RK2);
   else if (interp == DCL_CUBIC) DCL_PREP(DCL_CUBIC, DCL_EULER);
   else if (bt == DCL_RK2) DCL_PREP(DCL_LINEAR, DCL_RK2);
@@ -735,16 +751,15 @@ static int adj_step_t(int interp, const void* phi0, const void* record, const vo
                       void* hsum, const AdjP& P, hipStream_t s) {
   const int64_t n = (int64_t)P.w.nx * P.w.ny * P.w.nz;
   const int4* rb = static_cast<const int4*>(record);
-  const double4* rf = reinterpret_cast<const double4*>(rb + n);
   dim3 grid((P.w.nx + ABX - 1) / ABX, (P.w.ny + ABY - 1) / ABY, (P.w.nz + ABZ - 1) / ABZ);
   const T* p0 = static_cast<const T*>(phi0);
   const T* ln = static_cast<const T*>(lam_next);
   if (interp == DCL_CUBIC)
-    hipLaunchKernelGGL((adj_step_k<T, DCL_CUBIC>), grid, dim3(ABX, ABY, ABZ), 0, s, p0, rb, rf, ln,
-                       static_cast<T*>(lam), static_cast<T*>(hsum), P);
+    hipLaunchKernelGGL((adj_step_k<T, DCL_CUBIC>), grid, dim3(ABX, ABY, ABZ), 0, s, p0, rb,
+                       static_cast<const void*>(rb + n), ln, static_cast<T*>(lam), static_cast<T*>(hsum), P);
   else
-    hipLaunchKernelGGL((adj_step_k<T, DCL_LINEAR>), grid, dim3(ABX, ABY, ABZ), 0, s, p0, rb, rf, ln,
-                       static_cast<T*>(lam), static_cast<T*>(hsum), P);
+    hipLaunchKernelGGL((adj_step_k<T, DCL_LINEAR>), grid, dim3(ABX, ABY, ABZ), 0, s, p0, rb,
+                       static_cast<const void*>(rb + n), ln, static_cast<T*>(lam), static_cast<T*>(hsum), P);
   DCL_LAUNCHED("adj_step");
   return 0;
 }

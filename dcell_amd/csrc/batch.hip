@@ -162,10 +162,10 @@ batch_ring_k(const T* __restrict__ phi0, T* __restrict__ phi,
                                                       const int4* __restrict__ tiles,
                                                       const int* __restrict__ bounds, WorkQ* __restrict__ q, int k) {
   typedef RingGeom<RING_TX, RING_TY> G;
-  typedef VelGeom<RING_TX, RING_TY> V;
+  typedef VelGeom<RING_TX, RING_TY, G::VRP, G::VPP> V;
   constexpr bool VL = sizeof(T) == 4;
-  __shared__ T ring[RG_R * G::PLANE];
-  __shared__ T vr[VL ? VR_R * 3 * V::PLANE : 1];
+  __shared__ T ring[RG_R * G::PP];
+  __shared__ T vr[VL ? VR_R * 3 * V::PP : 1];
   claim_tiles(q, bounds, k, [&](int t) {
     const int4 tl = tiles[t];
     const BCell c = cells[tl.x];

@@ -29,6 +29,10 @@ int check_precision(int precision);
 
 inline unsigned grid1d(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
+// compute units of the current device (queried once per device; 256 if the
+// query fails)
+int device_cus();
+
 // Device scratch owned by the calling host thread on the current device
 // (grown on demand, kept until the process ends).  Only the synchronous
 // entry points (reductions with host results) use it, and they synchronize

@@ -22,26 +22,34 @@ namespace dcl {
 
 constexpr int RG_H = 3, RG_R = 8;  // halo, ring slots (2 RG_H + 2)
 
-// geometry of a TX x TY column
-template <int TX_, int TY_>
+// geometry of a TX x TY column: the box of BX x BY values per plane sits in
+// LDS with row pitch RP and plane (slot) pitch PP, by default packed
+template <int TX_, int TY_, int RP_ = TX_ + 2 * RG_H, int PP_ = RP_ * (TY_ + 2 * RG_H), int VRP_ = TX_ + 3,
+          int VPP_ = VRP_ * (TY_ + 3)>
 struct RingGeom {
   static constexpr int TX = TX_, TY = TY_;
   static constexpr int BX = TX + 2 * RG_H, BY = TY + 2 * RG_H;
-  static constexpr int PLANE = BX * BY;
+  static constexpr int PLANE = BX * BY;  // staged values per plane
+  static constexpr int RP = RP_, PP = PP_;
+  static constexpr int VRP = VRP_, VPP = VPP_;  // the velocity ring's pitches
+  static_assert(RP >= BX && PP >= RP * BY, "ring pitches too small for the box");
   static constexpr int NT = TX * TY;
   static constexpr int PF = (PLANE + NT - 1) / NT;  // staged values per thread and plane
 };
 
 // byte offsets within a plane of the box values this thread stages (x, y
-// clamped to the window), -1 past the box; the same for every plane
+// clamped to the window), -1 past the box, and their LDS positions within a
+// slot; the same for every plane
 template <typename T, typename G>
-__device__ __forceinline__ void ring_offsets(const KWin& w, int x0, int y0, int tid, int xo[G::PF]) {
+__device__ __forceinline__ void ring_offsets(const KWin& w, int x0, int y0, int tid, int xo[G::PF],
+                                             int so[G::PF]) {
 #pragma unroll
   for (int q = 0; q < G::PF; ++q) {
     const int e = tid + q * G::NT;
     const int bx = e % G::BX, by = e / G::BX;
     const int gx = clampi(x0 - RG_H + bx, 0, w.nx - 1), gy = clampi(y0 - RG_H + by, 0, w.ny - 1);
     xo[q] = e < G::PLANE ? (gy * w.nx + gx) * (int)sizeof(T) : -1;
+    so[q] = G::RP == G::BX ? e : by * G::RP + bx;
   }
 }
 
@@ -54,11 +62,11 @@ __device__ __forceinline__ void ring_fetch(rsrc_t rp, const int xo[G::PF], int z
 }
 
 template <typename T, typename G>
-__device__ __forceinline__ void ring_store(T* ring, int slot, int tid, const T pf[G::PF]) {
+__device__ __forceinline__ void ring_store(T* ring, int slot, int tid, const int so[G::PF], const T pf[G::PF]) {
 #pragma unroll
   for (int q = 0; q < G::PF; ++q) {
     const int e = tid + q * G::NT;
-    if (e < G::PLANE) ring[slot * G::PLANE + e] = pf[q];
+    if (e < G::PLANE) ring[slot * G::PP + so[q]] = pf[q];
   }
 }
 
@@ -79,10 +87,10 @@ __device__ __forceinline__ T gather3_ring(rsrc_t rp, const T* ring, const KWin& 
     const int lx = bx - 1 - (x0 - RG_H), ly = by - 1 - (y0 - RG_H), lz = bz - 1 - (k - RG_H);
     if ((unsigned)lx <= (unsigned)(G::BX - 4) && (unsigned)ly <= (unsigned)(G::BY - 4) &&
         (unsigned)lz <= (unsigned)(2 * RG_H - 3)) {
-      const T* b = ring + ly * G::BX + lx;
+      const T* b = ring + ly * G::RP + lx;
       const int zs = bz - 1 - z0 + RG_H;
       return cubic_sum<T>(wx, wy, wz, [&](int j, int kk, T r[4]) {
-        const T* q = b + ((zs + kk) & (RG_R - 1)) * G::PLANE + j * G::BX;
+        const T* q = b + ((zs + kk) & (RG_R - 1)) * G::PP + j * G::RP;
         r[0] = q[0];
         r[1] = q[1];
         r[2] = q[2];
@@ -103,16 +111,16 @@ __device__ __forceinline__ T gather3_ring(rsrc_t rp, const T* ring, const KWin& 
     if ((unsigned)lx <= (unsigned)(G::BX - 2) && (unsigned)ly <= (unsigned)(G::BY - 2) &&
         (unsigned)lz <= (unsigned)(2 * RG_H - 1)) {
       const int zs = bz - z0 + RG_H;
-      const T* p0 = ring + (zs & (RG_R - 1)) * G::PLANE + ly * G::BX + lx;
-      const T* p1 = ring + ((zs + 1) & (RG_R - 1)) * G::PLANE + ly * G::BX + lx;
+      const T* p0 = ring + (zs & (RG_R - 1)) * G::PP + ly * G::RP + lx;
+      const T* p1 = ring + ((zs + 1) & (RG_R - 1)) * G::PP + ly * G::RP + lx;
       a00[0] = p0[0];
       a00[1] = p0[1];
-      a10[0] = p0[G::BX];
-      a10[1] = p0[G::BX + 1];
+      a10[0] = p0[G::RP];
+      a10[1] = p0[G::RP + 1];
       a01[0] = p1[0];
       a01[1] = p1[1];
-      a11[0] = p1[G::BX];
-      a11[1] = p1[G::BX + 1];
+      a11[0] = p1[G::RP];
+      a11[1] = p1[G::RP + 1];
     } else {
       const int Yr = w.nx, Zr = w.nx * w.ny, e = (int)sizeof(T);
       const int vo = (bz * Zr + by * Yr + bx) * e;
@@ -135,22 +143,28 @@ __device__ __forceinline__ T gather3_ring(rsrc_t rp, const T* ring, const KWin& 
 // them from memory.
 constexpr int VR_R = 5;
 
-template <int TX_, int TY_>
+// the velocity box: VX x VY nodes per plane, each component's values in LDS
+// with row pitch RP and plane pitch PP
+template <int TX_, int TY_, int RP_ = TX_ + 3, int PP_ = RP_ * (TY_ + 3)>
 struct VelGeom {
   static constexpr int VX = TX_ + 3, VY = TY_ + 3;
   static constexpr int PLANE = VX * VY;
+  static constexpr int RP = RP_, PP = PP_;
+  static_assert(RP >= VX && PP >= RP * VY, "velocity ring pitches too small for the box");
   static constexpr int NT = TX_ * TY_;
   static constexpr int PF = (PLANE + NT - 1) / NT;
 };
 
 template <typename T, typename V>
-__device__ __forceinline__ void vring_offsets(const KWin& w, int x0, int y0, int tid, int xo[V::PF]) {
+__device__ __forceinline__ void vring_offsets(const KWin& w, int x0, int y0, int tid, int xo[V::PF],
+                                              int so[V::PF]) {
 #pragma unroll
   for (int q = 0; q < V::PF; ++q) {
     const int e = tid + q * V::NT;
     const int bx = e % V::VX, by = e / V::VX;
     const int gx = clampi(x0 - 1 + bx, 0, w.nx - 1), gy = clampi(y0 - 1 + by, 0, w.ny - 1);
     xo[q] = e < V::PLANE ? (gy * w.nx + gx) * 3 * (int)sizeof(T) : -1;
+    so[q] = V::RP == V::VX ? e : by * V::RP + bx;
   }
 }
 
@@ -176,13 +190,13 @@ __device__ __forceinline__ void vring_fetch(rsrc_t rv, const int xo[V::PF], int 
 }
 
 template <typename T, typename V>
-__device__ __forceinline__ void vring_store(T* vr, int slot, int tid, const T pf[V::PF][3]) {
+__device__ __forceinline__ void vring_store(T* vr, int slot, int tid, const int so[V::PF], const T pf[V::PF][3]) {
 #pragma unroll
   for (int q = 0; q < V::PF; ++q) {
     const int e = tid + q * V::NT;
     if (e < V::PLANE) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) vr[(slot * 3 + c) * V::PLANE + e] = pf[q][c];
+      for (int c = 0; c < 3; ++c) vr[(slot * 3 + c) * V::PP + so[q]] = pf[q][c];
     }
   }
 }
@@ -194,12 +208,12 @@ __device__ __forceinline__ void sl_dep3_ring(rsrc_t rv, const T* vr, const int s
                                              int k, int x0, int y0, double S[3]) {
   // node velocity: faces at (i, i+1), (j, j+1), (k, k+1); interior nodes need no clamp
   const int bx = i - (x0 - 1), by = j - (y0 - 1);
-  const T* p0 = vr + sl[1] * 3 * V::PLANE + by * V::VX + bx;
-  const T* p1 = vr + sl[2] * 3 * V::PLANE + by * V::VX + bx;
+  const T* p0 = vr + sl[1] * 3 * V::PP + by * V::RP + bx;
+  const T* p1 = vr + sl[2] * 3 * V::PP + by * V::RP + bx;
   T Vn[3];
   Vn[0] = vmid(p0[0], p0[1]);
-  Vn[1] = vmid(p0[V::PLANE], p0[V::PLANE + V::VX]);
-  Vn[2] = vmid(p0[2 * V::PLANE], p1[2 * V::PLANE]);
+  Vn[1] = vmid(p0[V::PP], p0[V::PP + V::RP]);
+  Vn[2] = vmid(p0[2 * V::PP], p1[2 * V::PP]);
   const int X = P.w.px + i, Y = P.w.py + j, Z = P.w.pz + k;
   const double Xd[3] = {(double)X, (double)Y, (double)Z};
   if (BT != DCL_RK2) {
@@ -227,10 +241,10 @@ __device__ __forceinline__ void sl_dep3_ring(rsrc_t rv, const T* vr, const int s
       const int rel = iz - (k - 1);  // 0 .. 2
       const int sa = rel == 0 ? sl[0] : rel == 1 ? sl[1] : sl[2];
       const int sb = rel == 0 ? sl[1] : rel == 1 ? sl[2] : sl[3];
-      const int o = (iy - lo[1]) * V::VX + (ix - lo[0]) + c * V::PLANE;
-      const T* qa = vr + sa * 3 * V::PLANE + o;
-      const T* qb = vr + sb * 3 * V::PLANE + o;
-      V2[c] = vtri(qa[0], qa[1], qa[V::VX], qa[V::VX + 1], qb[0], qb[1], qb[V::VX], qb[V::VX + 1], x1, y1, z1);
+      const int o = (iy - lo[1]) * V::RP + (ix - lo[0]) + c * V::PP;
+      const T* qa = vr + sa * 3 * V::PP + o;
+      const T* qb = vr + sb * 3 * V::PP + o;
+      V2[c] = vtri(qa[0], qa[1], qa[V::RP], qa[V::RP + 1], qb[0], qb[1], qb[V::RP], qb[V::RP + 1], x1, y1, z1);
     }
   } else {
     mac_vel_buf_split<T>(rv, P.v, b0, b1, f0, f1, V2);
@@ -243,31 +257,31 @@ __device__ __forceinline__ void sl_dep3_ring(rsrc_t rv, const T* vr, const int s
 // z0+zc-1) of the window, clipped to it; nodes outside [2, n-3] copy phi0.
 // Every thread of the block calls it (it holds barriers) and it ends on a
 // barrier, so a block may run it for one column after another with the same
-// ring (RG_R * G::PLANE values).
+// ring (RG_R * G::PP values).
 template <typename T, int INTERP, int BT, typename G, bool VL>
 __device__ __forceinline__ void ring_column(const T* __restrict__ phi0, T* __restrict__ phi, rsrc_t rp,
                                             rsrc_t rv, const SLP& P, int x0, int y0, int z0, int zc, T* ring,
                                             T* vr) {
-  typedef VelGeom<G::TX, G::TY> V;
+  typedef VelGeom<G::TX, G::TY, G::VRP, G::VPP> V;
   const int nx = P.w.nx, ny = P.w.ny, nz = P.w.nz;
   const int z1 = min(nz, z0 + zc);
   const int tid = threadIdx.x + G::TX * threadIdx.y;
   // bytes per plane of phi0 (a third of velocity's); products are taken
   // unsigned, the buffer offsets being 32-bit (windows below 4 GiB)
   const unsigned pz = (unsigned)(nx * ny) * (unsigned)sizeof(T);
-  int xo[G::PF], vxo[V::PF];
-  ring_offsets<T, G>(P.w, x0, y0, tid, xo);
-  if (VL) vring_offsets<T, V>(P.w, x0, y0, tid, vxo);
+  int xo[G::PF], so[G::PF], vxo[V::PF], vso[V::PF];
+  ring_offsets<T, G>(P.w, x0, y0, tid, xo, so);
+  if (VL) vring_offsets<T, V>(P.w, x0, y0, tid, vxo, vso);
   for (int z = max(z0 - RG_H, 0); z <= min(z0 + RG_H, nz - 1); ++z) {
     T pf[G::PF];
     ring_fetch<T, G>(rp, xo, (int)((unsigned)z * pz), pf);
-    ring_store<T, G>(ring, (z - z0 + RG_H) & (RG_R - 1), tid, pf);
+    ring_store<T, G>(ring, (z - z0 + RG_H) & (RG_R - 1), tid, so, pf);
   }
   if (VL) {
     for (int z = max(z0 - 1, 0); z <= min(z0 + 2, nz - 1); ++z) {
       T vf[V::PF][3];
       vring_fetch<T, V>(rv, vxo, (int)(3u * (unsigned)z * pz), vf);
-      vring_store<T, V>(vr, (z - z0 + 1) % VR_R, tid, vf);
+      vring_store<T, V>(vr, (z - z0 + 1) % VR_R, tid, vso, vf);
     }
   }
   __syncthreads();
@@ -298,8 +312,8 @@ __device__ __forceinline__ void ring_column(const T* __restrict__ phi0, T* __res
         phi[id] = gather3_ring<T, INTERP, G>(rp, ring, P.w, S, x0, y0, z0, k);
       }
     }
-    if (pre) ring_store<T, G>(ring, (zn - z0 + RG_H) & (RG_R - 1), tid, pf);
-    if (vpre) vring_store<T, V>(vr, (k + 3 - z0 + 1) % VR_R, tid, vf);
+    if (pre) ring_store<T, G>(ring, (zn - z0 + RG_H) & (RG_R - 1), tid, so, pf);
+    if (vpre) vring_store<T, V>(vr, (k + 3 - z0 + 1) % VR_R, tid, vso, vf);
     __syncthreads();
   }
 }
@@ -314,17 +328,16 @@ constexpr int ring_waves() {
 }
 
 // grid: tiles_x * tiles_y * nzc blocks of TX x TY threads (XCD-remapped so
-// each XCD walks neighbouring columns).  VL: velocity staged too (velocity
-// window = phi window, 3 components, udof 0).
-template <typename T, int INTERP, int BT, int ZC, int TX, int TY, bool VL>
-__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(ring_waves<T, VL>(), 8))) void sl_ring3d(const T* __restrict__ phi0, T* __restrict__ phi,
-                                                     const T* __restrict__ vel, SLP P, unsigned tiles_x,
-                                                     unsigned tiles_y, unsigned nzc, unsigned pbytes,
-                                                     unsigned vbytes) {
+// each XCD walks neighbouring columns), each walking zch planes.  VL:
+// velocity staged too (velocity window = phi window, 3 components, udof 0).
+template <typename T, int INTERP, int BT, int TX, int TY, bool VL>
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(ring_waves<T, VL>(), 8))) void sl_ring3d(
+    const T* __restrict__ phi0, T* __restrict__ phi, const T* __restrict__ vel, SLP P, unsigned tiles_x,
+    unsigned tiles_y, unsigned nzc, int zch, unsigned pbytes, unsigned vbytes) {
   typedef RingGeom<TX, TY> G;
-  typedef VelGeom<TX, TY> V;
-  __shared__ T ring[RG_R * G::PLANE];
-  __shared__ T vr[VL ? VR_R * 3 * V::PLANE : 1];
+  typedef VelGeom<TX, TY, G::VRP, G::VPP> V;
+  __shared__ T ring[RG_R * G::PP];
+  __shared__ T vr[VL ? VR_R * 3 * V::PP : 1];
   const unsigned nb = tiles_x * tiles_y * nzc;
   unsigned lb = blockIdx.x;
   {
@@ -333,7 +346,7 @@ __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(ring_wa
   }
   const unsigned tx = lb % tiles_x, ty = (lb / tiles_x) % tiles_y, zc = lb / (tiles_x * tiles_y);
   ring_column<T, INTERP, BT, G, VL>(phi0, phi, make_rsrc(phi0, pbytes), make_rsrc(vel, vbytes), P, (int)tx * TX,
-                                    (int)ty * TY, (int)zc * ZC, ZC, ring, vr);
+                                    (int)ty * TY, (int)zc * zch, zch, ring, vr);
 }
 
 }  // namespace dcl

@@ -25,9 +25,13 @@ PetscErrorCode LevelSetAdvectSLSetScheme(int interp, int backtrace) {
   return 0;
 }
 
-/* write compact band values into g at the band nodes that lie in its window */
+/* write compact band values into g at the band nodes that lie in its window,
+ * split over threads (a node listed twice gets the same value from both of
+ * its entries, so the order of the writes does not matter) */
 static void scatter_band(Grid g, const iCoor *band, int len, const PetscReal *vals) {
   const int nz = g->is2D ? 1 : g->n.z;
+  const int nt = dl_threads();
+#pragma omp parallel for num_threads(nt) if (nt > 1 && len > 65536) schedule(static)
   for (int b = 0; b < len; ++b) {
     const int x = band[b].x - g->p.x, y = band[b].y - g->p.y, z = g->is2D ? 0 : band[b].z - g->p.z;
     if (x < 0 || x >= g->n.x || y < 0 || y >= g->n.y || z < 0 || z >= nz) continue;
@@ -57,7 +61,7 @@ static PetscErrorCode advect_band(LevelSet ls, Grid velgrid, PetscReal dt, const
   if (!dl_box_around(ls->phi0, &bb, M, 0, &pb)) return dl_fail(where, DCL_ERR_ARG_SIZ);
   DL_CHECK(where, dl_upload_box(DL_PHI0, ls->phi0, &pb, &dphi0, &pw, NULL, 0));
   DL_CHECK(where, dl_slot(DL_BAND, sizeof(iCoor) * (size_t)len, &dband));
-  DL_CHECK(where, dcl_memcpy_htod(dband, band, sizeof(iCoor) * (size_t)len, NULL));
+  DL_CHECK(where, dl_htod(dband, band, sizeof(iCoor) * (size_t)len, NULL));
   DL_CHECK(where, dl_slot(DL_OUT, sizeof(PetscReal) * (size_t)len, &dout));
   dcl_velocity V;
   V.data = dvel;
@@ -67,7 +71,7 @@ static PetscErrorCode advect_band(LevelSet ls, Grid velgrid, PetscReal dt, const
   DL_CHECK(where, dcl_sl_advect(dphi0, NULL, &pw, &V, (const dcl_icoor *)dband, len, dout, dt, DCL_F64,
                                 g_interp, g_backtrace, NULL));
   DL_CHECK(where, dl_host(sizeof(PetscReal) * (size_t)len, &hout));
-  DL_CHECK(where, dcl_memcpy_dtoh(hout, dout, sizeof(PetscReal) * (size_t)len, NULL));
+  DL_CHECK(where, dl_dtoh(hout, dout, sizeof(PetscReal) * (size_t)len, NULL));
   scatter_band(ls->phi, band, len, (const PetscReal *)hout);
   return 0;
 }
@@ -165,7 +169,7 @@ PetscErrorCode LevelSet_MaxVelocity(LevelSet ls, Grid velgrid) {
   iCoor *pos = (iCoor *)hpos;
   for (int i = 0; i < len; ++i) pos[i] = nodes[i].pos;
   DL_CHECK("LevelSet_MaxVelocity", dl_slot(DL_NODES, sizeof(iCoor) * (size_t)len, &dnodes));
-  DL_CHECK("LevelSet_MaxVelocity", dcl_memcpy_htod(dnodes, pos, sizeof(iCoor) * (size_t)len, NULL));
+  DL_CHECK("LevelSet_MaxVelocity", dl_htod(dnodes, pos, sizeof(iCoor) * (size_t)len, NULL));
   dl_box bb;
   dl_bbox(pos, len, sizeof(iCoor), dim, &bb);
   dcl_window vw;
@@ -191,7 +195,7 @@ PetscErrorCode LevelSetCurvature(LevelSet ls, Grid kappa) {
   DL_CHECK("LevelSetCurvature", dl_slot(DL_KAPPA, sizeof(PetscReal) * (size_t)phi->SIZE, &dk));
   const dcl_window w = dl_window(phi);
   DL_CHECK("LevelSetCurvature", dcl_curvature(dphi, dk, &w, NULL, 0, NULL, DCL_F64, DCL_CURV_STD, NULL));
-  DL_CHECK("LevelSetCurvature", dcl_memcpy_dtoh(kappa->v1, dk, sizeof(PetscReal) * (size_t)phi->SIZE, NULL));
+  DL_CHECK("LevelSetCurvature", dl_dtoh(kappa->v1, dk, sizeof(PetscReal) * (size_t)phi->SIZE, NULL));
   return 0;
 }
 

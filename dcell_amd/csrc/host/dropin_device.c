@@ -2,11 +2,41 @@
  * dropin_device.c -- device buffers and host staging of the C drop-in.
  */
 #include <math.h>
+#include <omp.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
 #include "dropin.h"
+#include "dcell_gpu.h"
+
+static unsigned long long g_htod_bytes, g_dtoh_bytes;
+
+int dl_htod(void *dst, const void *src, size_t bytes, void *stream) {
+  g_htod_bytes += bytes;
+  return dcl_memcpy_htod(dst, src, bytes, stream);
+}
+
+int dl_dtoh(void *dst, const void *src, size_t bytes, void *stream) {
+  g_dtoh_bytes += bytes;
+  return dcl_memcpy_dtoh(dst, src, bytes, stream);
+}
+
+PetscErrorCode LevelSetGPUTransferStats(unsigned long long *htod_bytes, unsigned long long *dtoh_bytes) {
+  if (htod_bytes) *htod_bytes = g_htod_bytes;
+  if (dtoh_bytes) *dtoh_bytes = g_dtoh_bytes;
+  return 0;
+}
+
+int dl_threads(void) {
+  static int n;
+  if (!n) {
+    const char *e = getenv("DCELL_GPU_HOST_THREADS");
+    int v = e ? atoi(e) : 8;
+    n = v < 1 ? 1 : v > 64 ? 64 : v;
+  }
+  return n;
+}
 
 static void *g_dev[DL_NSLOTS];
 static size_t g_dev_bytes[DL_NSLOTS];
@@ -82,6 +112,8 @@ static struct {
   void *dev;
   dcl_window w;
   double maxabs[3];
+  Grid last_grid; /* the grid of the last successful dl_velocity_box */
+  int last_m;     /* and the margin it needed */
 } g_vel;
 
 int dl_release(void) {
@@ -90,6 +122,7 @@ int dl_release(void) {
   g_pin = NULL;
   g_pin_off = 0;
   g_vel.valid = 0;
+  g_vel.last_grid = NULL;
   for (int i = 0; i < DL_NSLOTS; ++i) {
     if (g_dev[i]) dcl_free(g_dev[i]);
     g_dev[i] = NULL;
@@ -121,7 +154,7 @@ int dl_upload_grid(int slot, Grid g, void **dev) {
   const size_t bytes = sizeof(PetscReal) * (size_t)g->SIZE;
   int rc = dl_slot(slot, bytes, dev);
   if (rc) return rc;
-  return dcl_memcpy_htod(*dev, g->v1, bytes, NULL);
+  return dl_htod(*dev, g->v1, bytes, NULL);
 }
 
 int dl_fail(const char *where, int rc) {
@@ -130,17 +163,21 @@ int dl_fail(const char *where, int rc) {
 }
 
 void dl_bbox(const void *coords, long n, size_t stride, int dim, dl_box *b) {
-  for (int a = 0; a < 3; ++a) {
-    b->lo[a] = 2147483647;
-    b->hi[a] = -2147483647 - 1;
-  }
+  int lo[3] = {2147483647, 2147483647, 2147483647}, hi[3] = {-2147483647 - 1, -2147483647 - 1, -2147483647 - 1};
   const char *p = (const char *)coords;
-  for (long i = 0; i < n; ++i, p += stride) {
-    const int *q = (const int *)p;
+  const int nt = dl_threads();
+#pragma omp parallel for num_threads(nt) if (nt > 1 && n > 65536) schedule(static) \
+    reduction(min : lo[:3]) reduction(max : hi[:3])
+  for (long i = 0; i < n; ++i) {
+    const int *q = (const int *)(p + i * stride);
     for (int a = 0; a < dim; ++a) {
-      if (q[a] < b->lo[a]) b->lo[a] = q[a];
-      if (q[a] > b->hi[a]) b->hi[a] = q[a];
+      if (q[a] < lo[a]) lo[a] = q[a];
+      if (q[a] > hi[a]) hi[a] = q[a];
     }
+  }
+  for (int a = 0; a < 3; ++a) {
+    b->lo[a] = lo[a];
+    b->hi[a] = hi[a];
   }
   if (dim == 2) b->lo[2] = b->hi[2] = 0;
 }
@@ -172,35 +209,51 @@ int dl_upload_box(int slot, Grid g, const dl_box *b, void **dev, dcl_window *w, 
   int rc = dl_slot(slot, total, dev);
   if (rc) return rc;
   double mx[4] = {0, 0, 0, 0};
+  int nan = 0;
   if (ncomp > 4) ncomp = 4;
+  if (!maxabs) ncomp = 0;
   const long gx = g->n.x, gy = g->n.y;
   const long rows_per_chunk = DL_CHUNK / (long)row;
   const long nrows = by * bz;
+  const int nt = dl_threads();
   size_t sent = 0;
   for (long r0 = 0; r0 < nrows; r0 += rows_per_chunk) {
     const long r1 = r0 + rows_per_chunk < nrows ? r0 + rows_per_chunk : nrows;
     char *st;
     rc = dl_stage(row * (size_t)(r1 - r0), &st);
     if (rc) return rc;
-    for (long r = r0; r < r1; ++r) {
-      const long y = b->lo[1] + r % by, z = b->lo[2] + r / by;
-      const long lz = g->is2D ? 0 : z - g->p.z;
-      const PetscReal *src = g->v1 + ((lz * gy + (y - g->p.y)) * gx + (b->lo[0] - g->p.x)) * dof;
-      memcpy(st + (r - r0) * row, src, row);
-      if (maxabs) {
-        for (long i = 0; i < bx; ++i)
-          for (int c = 0; c < ncomp; ++c) {
+#pragma omp parallel num_threads(nt) if (nt > 1 && r1 - r0 > 1)
+    {
+      double lm[4] = {0, 0, 0, 0};
+      int ln = 0;
+#pragma omp for schedule(static)
+      for (long r = r0; r < r1; ++r) {
+        const long y = b->lo[1] + r % by, z = b->lo[2] + r / by;
+        const long lz = g->is2D ? 0 : z - g->p.z;
+        const PetscReal *src = g->v1 + ((lz * gy + (y - g->p.y)) * gx + (b->lo[0] - g->p.x)) * dof;
+        memcpy(st + (r - r0) * row, src, row);
+        for (int c = 0; c < ncomp; ++c) {
+          double m = lm[c];
+          for (long i = 0; i < bx; ++i) {
             const double v = fabs(src[i * dof + c]);
-            if (!(v <= mx[c])) mx[c] = v; /* NaN propagates */
+            ln |= v != v;
+            m = v > m ? v : m;
           }
+          lm[c] = m;
+        }
+      }
+#pragma omp critical(dl_maxabs)
+      {
+        for (int c = 0; c < ncomp; ++c)
+          if (lm[c] > mx[c]) mx[c] = lm[c];
+        nan |= ln;
       }
     }
-    rc = dcl_memcpy_htod((char *)*dev + sent, st, row * (size_t)(r1 - r0), NULL);
+    rc = dl_htod((char *)*dev + sent, st, row * (size_t)(r1 - r0), NULL);
     if (rc) return rc;
     sent += row * (size_t)(r1 - r0);
   }
-  if (maxabs)
-    for (int c = 0; c < ncomp; ++c) maxabs[c] = mx[c];
+  for (int c = 0; c < ncomp; ++c) maxabs[c] = nan ? NAN : mx[c];
   w->p.x = b->lo[0];
   w->p.y = b->lo[1];
   w->p.z = b->lo[2];
@@ -236,6 +289,7 @@ int dl_velocity_box(Grid velgrid, const dl_box *b, double dt, Coor dh, int dim, 
                     int *M) {
   const double d[3] = {dh.x, dh.y, dh.z};
   int m = 4;
+  if (g_vel.last_grid == velgrid && g_vel.last_m > m) m = g_vel.last_m;
   for (int iter = 0; iter < 16; ++iter) {
     dl_box vb;
     if (!dl_box_around(velgrid, b, m, 1, &vb)) return DCL_ERR_ARG_SIZ;
@@ -264,6 +318,8 @@ int dl_velocity_box(Grid velgrid, const dl_box *b, double dt, Coor dh, int dim, 
       *dev = g_vel.dev;
       *w = g_vel.w;
       *M = m;
+      g_vel.last_grid = velgrid;
+      g_vel.last_m = need;
       return 0;
     }
     m = need;

@@ -30,7 +30,7 @@ static PetscErrorCode append_records(LevelSet ls, const char *where) {
   dcl_window w;
   DL_CHECK(where, dl_upload_box(DL_PHI, ls->phi, &pb, &dphi, &w, NULL, 0));
   DL_CHECK(where, dl_slot(DL_BAND, sizeof(iCoor) * (size_t)len, &dband));
-  DL_CHECK(where, dcl_memcpy_htod(dband, ArrayGetData(ls->band), sizeof(iCoor) * (size_t)len, NULL));
+  DL_CHECK(where, dl_htod(dband, ArrayGetData(ls->band), sizeof(iCoor) * (size_t)len, NULL));
   /* at most 1 projection + 3 (2) crossings + 6 (2) face crossings per node */
   const int64_t cap = 10 * (int64_t)len;
   DL_CHECK(where, dl_slot(DL_IRREG, sizeof(dcl_irregular_node) * (size_t)cap, &dout));
@@ -39,7 +39,7 @@ static PetscErrorCode append_records(LevelSet ls, const char *where) {
                                       (dcl_irregular_node *)dout, cap, &count, NULL));
   if (count > cap) return dl_fail(where, DCL_ERR_ARG_SIZ);
   DL_CHECK(where, dl_host(sizeof(dcl_irregular_node) * (size_t)count, &hrec));
-  DL_CHECK(where, dcl_memcpy_dtoh(hrec, dout, sizeof(dcl_irregular_node) * (size_t)count, NULL));
+  DL_CHECK(where, dl_dtoh(hrec, dout, sizeof(dcl_irregular_node) * (size_t)count, NULL));
   const int base = ArrayLength(ls->irregularNodes);
   ierr = ArraySetSize(ls->irregularNodes, base + (int)count); CHKERRQ(ierr);
   IrregularNode *nodes = (IrregularNode *)ArrayGetData(ls->irregularNodes) + base;

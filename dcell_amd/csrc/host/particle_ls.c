@@ -44,7 +44,7 @@ static PetscErrorCode upload_particles(ParticleLS pls, void **dev, int *len) {
   const size_t bytes = sizeof(struct _Particle) * (size_t)*len;
   int rc = dl_slot(DL_PART, bytes ? bytes : 1, dev);
   if (rc) return rc;
-  return dcl_memcpy_htod(*dev, ArrayGetData(pls->particles), bytes, NULL);
+  return dl_htod(*dev, ArrayGetData(pls->particles), bytes, NULL);
 }
 
 /* the box of the nodes around the particles' positions */
@@ -100,7 +100,7 @@ static PetscErrorCode advect_particles(ParticleLS pls, Coor dh, Grid velgrid, Pe
     V.udof = 0;
     dcl_coor h = {dh.x, dh.y, dh.z};
     DL_CHECK(where, dcl_pls_advect(dpart, len, &V, h, dt, dim, DCL_F64, NULL));
-    DL_CHECK(where, dcl_memcpy_dtoh(ArrayGetData(pls->particles), dpart,
+    DL_CHECK(where, dl_dtoh(ArrayGetData(pls->particles), dpart,
                                     sizeof(struct _Particle) * (size_t)len, NULL));
   }
   ierr = PetscLogEventEnd(EVENT_ParticleLS_AdvectParticles, 0, 0, 0, 0); CHKERRQ(ierr);
@@ -148,12 +148,12 @@ static PetscErrorCode correct(ParticleLS pls, LevelSet ls, int dim, const char *
     DL_CHECK(where, dl_upload_box(DL_PHI, ls->phi, &box, &dphi, &w, NULL, 0));
     DL_CHECK(where, ensure_ws((long)w.n.x * w.n.y * w.n.z, 8L * (len > 0 ? len : 1)));
     DL_CHECK(where, dl_slot(DL_BAND, sizeof(iCoor) * (size_t)blen, &dband));
-    DL_CHECK(where, dcl_memcpy_htod(dband, band, sizeof(iCoor) * (size_t)blen, NULL));
+    DL_CHECK(where, dl_htod(dband, band, sizeof(iCoor) * (size_t)blen, NULL));
     DL_CHECK(where, dl_slot(DL_OUT, sizeof(PetscReal) * (size_t)blen, &dout));
     DL_CHECK(where, dcl_pls_correct(dpart, len, dphi, &w, (const dcl_icoor *)dband, blen, dout, flags, dim,
                                     DCL_F64, g_ws, NULL));
     DL_CHECK(where, dl_host(sizeof(PetscReal) * (size_t)blen, &hout));
-    DL_CHECK(where, dcl_memcpy_dtoh(hout, dout, sizeof(PetscReal) * (size_t)blen, NULL));
+    DL_CHECK(where, dl_dtoh(hout, dout, sizeof(PetscReal) * (size_t)blen, NULL));
     Grid g = ls->phi;
     const int nz = g->is2D ? 1 : g->n.z;
     const PetscReal *v = (const PetscReal *)hout;
@@ -193,7 +193,7 @@ PetscErrorCode ParticleLS_AdjustRadii(ParticleLS pls, LevelSet ls) {
     DL_CHECK("ParticleLS_AdjustRadii", dl_upload_box(DL_PHI, ls->phi, &box, &dphi, &w, NULL, 0));
     DL_CHECK("ParticleLS_AdjustRadii", dcl_pls_adjust_radii(dpart, len, dphi, &w, pls->R_MIN, pls->R_MAX,
                                                             ls->phi->is2D ? 2 : 3, DCL_F64, NULL));
-    DL_CHECK("ParticleLS_AdjustRadii", dcl_memcpy_dtoh(ArrayGetData(pls->particles), dpart,
+    DL_CHECK("ParticleLS_AdjustRadii", dl_dtoh(ArrayGetData(pls->particles), dpart,
                                                        sizeof(struct _Particle) * (size_t)len, NULL));
   }
   ierr = PetscLogEventEnd(EVENT_ParticleLS_AdjustRadii, 0, 0, 0, 0); CHKERRQ(ierr);

@@ -44,7 +44,7 @@ PetscErrorCode LevelSetReinitialize(LevelSet ls) {
     r[i].op.z = nodes[i].op.z;
   }
   DL_CHECK(where, dl_slot(DL_IRREG, rbytes, &drec));
-  DL_CHECK(where, dcl_memcpy_htod(drec, r, rbytes, NULL));
+  DL_CHECK(where, dl_htod(drec, r, rbytes, NULL));
   const long nodes_w = dl_nodes(ls->phi);
   DL_CHECK(where, dl_slot(DL_BAND, sizeof(iCoor) * (size_t)nodes_w, &dband));
   dcl_reinit_cell cell;
@@ -63,10 +63,10 @@ PetscErrorCode LevelSetReinitialize(LevelSet ls) {
   } else {
     DL_CHECK(where, dcl_reinitialize_parallel(&cell, 1, 0, NULL, NULL));
   }
-  DL_CHECK(where, dcl_memcpy_dtoh(ls->phi->v1, dphi, sizeof(double) * (size_t)nodes_w, NULL));
+  DL_CHECK(where, dl_dtoh(ls->phi->v1, dphi, sizeof(double) * (size_t)nodes_w, NULL));
   ierr = ArraySetSize(ls->band, (int)cell.band_count); CHKERRQ(ierr);
   if (cell.band_count > 0) {
-    DL_CHECK(where, dcl_memcpy_dtoh(ArrayGetData(ls->band), dband, sizeof(iCoor) * (size_t)cell.band_count, NULL));
+    DL_CHECK(where, dl_dtoh(ArrayGetData(ls->band), dband, sizeof(iCoor) * (size_t)cell.band_count, NULL));
   }
   ierr = PetscLogEventEnd(EVENT_LevelSetReinitialize, 0, 0, 0, 0); CHKERRQ(ierr);
   PetscFunctionReturn(0);

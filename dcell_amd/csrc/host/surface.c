@@ -51,11 +51,11 @@ static PetscErrorCode surface_quantities(IIM iim, LevelSet ls, const char *where
     r[i].X.z = nodes[i].X.z;
   }
   DL_CHECK(where, dl_slot(DL_IRREG, rbytes, &drec));
-  DL_CHECK(where, dcl_memcpy_htod(drec, r, rbytes, NULL));
+  DL_CHECK(where, dl_htod(drec, r, rbytes, NULL));
   DL_CHECK(where, dl_slot(DL_SURF, sbytes, &dout));
   DL_CHECK(where, dcl_surface_quantities(dphi, &w, (const dcl_irregular_node *)drec, len, DCL_F64,
                                          (dcl_surface *)dout, NULL));
-  DL_CHECK(where, dcl_memcpy_dtoh(hrec, dout, sbytes, NULL));
+  DL_CHECK(where, dl_dtoh(hrec, dout, sbytes, NULL));
   const dcl_surface *q = (const dcl_surface *)hrec;
   const int is2D = ls->phi->is2D;
   for (int i = 0; i < len; ++i) {

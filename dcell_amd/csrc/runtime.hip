@@ -1,5 +1,6 @@
 // runtime.hip -- error reporting, argument checks and the memory helpers of
 // the C ABI (dcell_lsm.h).
+#include <atomic>
 #include <stdio.h>
 #include <string.h>
 
@@ -51,6 +52,17 @@ int check_precision(int precision) {
   if (precision != DCL_F64 && precision != DCL_F32)
     return set_error(DCL_ERR_ARG_WRONG, "precision %d is neither DCL_F64 nor DCL_F32", precision);
   return 0;
+}
+
+int device_cus() {
+  static std::atomic<int> cached[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  int c = cached[dev].load(std::memory_order_relaxed);
+  if (c > 0) return c;
+  if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0) c = 256;
+  cached[dev].store(c, std::memory_order_relaxed);
+  return c;
 }
 
 struct ScratchSlot {

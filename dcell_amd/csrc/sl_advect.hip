@@ -100,19 +100,23 @@ static int launch_sl_b(const void* phi0, void* phi, const void* vel, const SLP& 
     // velocity staged too where it can be (same window, 3 components) and
     // measured faster: FP32, or FP64 with the trilinear stencil (the FP64
     // tricubic kernel is register-bound with it).  DCL_RING_VEL=0|1 overrides.
-    constexpr int ZC = 32;
+    // z-chunks of 32 planes, halved (down to 8) while the grid has fewer
+    // than 4 blocks per CU: small windows (128^3 fills 256 blocks at 32)
+    // need the blocks more than the shorter chunks' extra halo planes
     const char* ge = getenv("DCL_RING_TILE");
     const int g = ge ? atoi(ge) : 1;
     const char* ve = getenv("DCL_RING_VEL");
     const bool same = P.v.w.px == P.w.px && P.v.w.py == P.w.py && P.v.w.pz == P.w.pz && P.v.w.nx == P.w.nx &&
                       P.v.w.ny == P.w.ny && P.v.w.nz == P.w.nz && P.v.dof == 3 && P.v.udof == 0;
     const bool vl = same && (ve ? atoi(ve) != 0 : (sizeof(T) == 4 || INTERP == DCL_LINEAR));
-#define DCL_RING(GI, TXV, TYV, VLV)                                                                          \
-  if (g == GI && vl == VLV) {                                                                                \
-    const unsigned tx = (P.w.nx + TXV - 1) / TXV, ty = (P.w.ny + TYV - 1) / TYV;                             \
-    const unsigned nzc = (P.w.nz + ZC - 1) / ZC;                                                             \
-    hipLaunchKernelGGL((sl_ring3d<T, INTERP, BT, ZC, TXV, TYV, VLV>), dim3(tx * ty * nzc), dim3(TXV, TYV), 0, \
-                       s, p0, static_cast<T*>(phi), v, P, tx, ty, nzc, pb, vb);                              \
+#define DCL_RING(GI, TXV, TYV, VLV)                                                                      \
+  if (g == GI && vl == VLV) {                                                                            \
+    const unsigned tx = (P.w.nx + TXV - 1) / TXV, ty = (P.w.ny + TYV - 1) / TYV;                         \
+    int zc = 32;                                                                                         \
+    while (zc > 8 && (int64_t)tx * ty * ((P.w.nz + zc - 1) / zc) < 4 * (int64_t)device_cus()) zc /= 2;   \
+    const unsigned nzc = (P.w.nz + zc - 1) / zc;                                                         \
+    hipLaunchKernelGGL((sl_ring3d<T, INTERP, BT, TXV, TYV, VLV>), dim3(tx * ty * nzc), dim3(TXV, TYV), 0, \
+                       s, p0, static_cast<T*>(phi), v, P, tx, ty, nzc, zc, pb, vb);                      \
   }
     DCL_RING(0, 16, 16, false)
     DCL_RING(1, 32, 8, false)

@@ -147,6 +147,7 @@ def lib():
         L.LevelSetPLSSetReferenceCornerZ.argtypes = [i32]
         L.ParticleLS_AdjustRadii.argtypes = [ParticleLSP, LevelSetP]
         L.LevelSetGPUFinalize.argtypes = []
+        L.LevelSetGPUTransferStats.argtypes = [C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]
         _L = L
     return _L
 

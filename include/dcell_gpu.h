@@ -33,6 +33,10 @@ PetscErrorCode LevelSetCurvature(LevelSet ls, Grid kappa);
 /* Frees the device buffers the drop-in keeps between calls. */
 PetscErrorCode LevelSetGPUFinalize(void);
 
+/* Bytes the drop-in has copied host -> device and device -> host since the
+ * library was loaded (running totals; either pointer may be NULL). */
+PetscErrorCode LevelSetGPUTransferStats(unsigned long long *htod_bytes, unsigned long long *dtoh_bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,6 +25,19 @@ def test_library_exports_every_declared_symbol():
     assert sorted(L.EXPORTS) == declared
 
 
+def test_kernel_names_mention_no_unnamed_type():
+    """A kernel template whose mangled name refers to an unnamed type (an
+    enumerator of an anonymous enum inside a dependent parameter type) gets
+    that type numbered differently by the host and the device compilations,
+    and its launch then finds no device function.  The host stubs' kernel
+    names are the mangled strings in the library's data."""
+    blob = open(L.LIB_PATH, "rb").read()
+    names = set(re.findall(rb"_ZN3dcl[0-9A-Za-z_$]+", blob))
+    assert names, "no dcl kernel names found"
+    bad = sorted(n.decode() for n in names if b"$_" in n)
+    assert not bad, bad
+
+
 def test_abi_version_and_argument_errors():
     lib = L.lib()
     assert lib.dcl_abi_version() == 1

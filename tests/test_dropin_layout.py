@@ -71,5 +71,6 @@ def test_dropin_libraries_export_reference_names():
                  "LevelSetCFLIncrement", "LevelSet_MaxVelocity", "LevelSetCurvature",
                  "LevelSetUpdateIrregularNodeList", "LevelSetUpdateIrregularNodeList_2D", "LevelSetReinitialize",
                  "LevelSetUpdateIrregularNodeList_3D", "IIMUpdateSurfaceQuantities_2D",
-                 "IIMUpdateSurfaceQuantities_3D", "LevelSetAdvectSLSetScheme", "LevelSetPLSSetReferenceCornerZ", "LevelSetGPUFinalize"):
+                 "IIMUpdateSurfaceQuantities_3D", "LevelSetAdvectSLSetScheme", "LevelSetPLSSetReferenceCornerZ", "LevelSetGPUFinalize",
+                 "LevelSetGPUTransferStats"):
         assert f" T {name}\n" in syms, name
