@@ -161,7 +161,7 @@ batch_ring_k(const T* __restrict__ phi0, T* __restrict__ phi,
                                                       const T* __restrict__ vel, const BCell* __restrict__ cells,
                                                       const int4* __restrict__ tiles,
                                                       const int* __restrict__ bounds, WorkQ* __restrict__ q, int k) {
-  typedef RingGeom<RING_TX, RING_TY> G;
+  typedef typename RingFor<T, RING_TX, RING_TY>::type G;
   typedef VelGeom<RING_TX, RING_TY, G::VRP, G::VPP> V;
   constexpr bool VL = sizeof(T) == 4;
   __shared__ T ring[RG_R * G::PP];
