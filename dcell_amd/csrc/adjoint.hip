@@ -445,127 +445,49 @@ __global__ __launch_bounds__(256) void adj_prepare_k(const T* __restrict__ vel, 
   }
 }
 
-// One adjoint step from the record.  Blocks of 16 x 4 x 8 nodes, each thread
-// the z pair (k, k+1) of its (i, j); the LDS box of lam holds the block plus
-// the reach AR.  Where every lane of a wave with an active node has both nodes
-// active, inside the box, and node k+1's stencil one plane above node k's
-// (a smooth flow: nearly every wave), the pair's two stencils share planes
-// 1..NP-1 of node k's: the wave walks NP + 1 planes instead of 2 NP, adding
-// both nodes' entries into one word before the LDS atomic and reading each
-// shared stencil row once (the LDS float atomics are this kernel's cost).
-// Other waves scatter the two nodes one after the other.  Within a 16-lane
-// row the x-neighbour lanes combine their shared words as in sl_adjoint_k.
-constexpr int SBZ = 2 * ABZ, SLBZ = SBZ + 2 * AR;  // node planes per block, box planes
-
-// the weights of one interpolation axis at fraction f (double for the
-// trilinear, whose weights the forward forms from 1 - f in double)
-template <typename T, int INTERP>
-__device__ __forceinline__ void axis_weights(double f, T w[4], T dw[4]) {
-  if (INTERP == DCL_CUBIC) {
-    cr_weights<T>((T)f, w);
-    cr_dweights<T>((T)f, dw);
-  } else {
-    w[0] = (T)(1.0 - f);
-    w[1] = (T)f;
-    w[2] = w[3] = T(0);
-    dw[0] = T(-1);
-    dw[1] = T(1);
-    dw[2] = dw[3] = T(0);
-  }
-}
-
-// the scatter of one node (every lane of the wave calls it; act false: no
-// entries): the transposed gather of g into the box (or lam where the
-// stencil leaves the box) and the node's departure sensitivity G
-template <typename T, int INTERP>
-__device__ __forceinline__ void adj_scatter_one(rsrc_t rp, T* box, T* __restrict__ lam, int Y, int Z, bool act,
-                                                T g, const int b[3], const T (&w)[3][4], const T (&dw)[3][4],
-                                                int ox, int oy, int oz, T G[3]) {
-  constexpr int NP = INTERP == DCL_CUBIC ? 4 : 2;
-  constexpr int OFF = INTERP == DCL_CUBIC ? -1 : 0;
-  const int bx0 = b[0] + OFF - (ox - AR), by0 = b[1] + OFF - (oy - AR), bz0 = b[2] + OFF - (oz - AR);
-  const bool in_box =
-      act && bx0 >= 0 && bx0 + NP <= LBX && by0 >= 0 && by0 + NP <= LBY && bz0 >= 0 && bz0 + NP <= SLBZ;
-  const int key = in_box ? (bz0 * LBY + by0) * LBX + bx0 + 1 : 0;
-  bool lo_m[4] = {false, false, false, false}, up_m[4] = {false, false, false, false};
-#pragma unroll
-  for (int d = 1; d < NP; ++d) {
-    const int kl = lane_from_lower(key, d), ku = lane_from_upper(key, d);
-    lo_m[d] = key != 0 && kl != 0 && kl + d == key;
-    up_m[d] = key != 0 && ku != 0 && key + d == ku;
-  }
-  const int vo = act ? ((b[2] + OFF) * Z + (b[1] + OFF) * Y + (b[0] + OFF)) * (int)sizeof(T) : 0;
-#pragma unroll
-  for (int kk = 0; kk < NP; ++kk) {
-    T rows[4][4];
-#pragma unroll
-    for (int jj = 0; jj < NP; ++jj) {
-      const int so = (kk * Z + jj * Y) * (int)sizeof(T);
-      if (NP == 4) BL<T>::four(rp, vo, so, rows[jj]);
-      else BL<T>::two(rp, vo, so, rows[jj]);
-    }
-#pragma unroll
-    for (int jj = 0; jj < NP; ++jj) {
-      const T wyz = w[1][jj] * w[2][kk];
-      T add[4];
-#pragma unroll
-      for (int ii = 0; ii < NP; ++ii) add[ii] = g * w[0][ii] * wyz;
-      T word0 = add[0];
-#pragma unroll
-      for (int d = 1; d < NP; ++d) {
-        const T o = lane_from_lower(add[d], d);
-        if (lo_m[d]) word0 += o;
-      }
-      if (act) {
-        T rx = T(0), rd = T(0);
-#pragma unroll
-        for (int ii = 0; ii < NP; ++ii) {
-          const T f = rows[jj][ii];
-          rx += w[0][ii] * f;
-          rd += dw[0][ii] * f;
-          if (in_box) {
-            T* bw = &box[((bz0 + kk) * LBY + (by0 + jj)) * LBX + (bx0 + ii)];
-            if (ii == 0)
-              atomicAdd(bw, word0);
-            else if (!up_m[ii])
-              atomicAdd(bw, add[ii]);
-          } else {
-            atomic_add(&lam[(b[2] + OFF + kk) * Z + (b[1] + OFF + jj) * Y + (b[0] + OFF + ii)], add[ii]);
-          }
-        }
-        G[0] += rd * wyz;
-        G[1] += rx * dw[1][jj] * w[2][kk];
-        G[2] += rx * w[1][jj] * dw[2][kk];
-      }
-    }
-  }
-}
+// One adjoint step from the record.  A block is one wave: its 16 x 4 lanes
+// own a 16 x 4 column of nodes and walk AWZ planes of it, scattering into an
+// LDS box of lam (the column plus the reach WR) that no other wave touches.
+// So the scatter is a plain LDS read-add-write per lane: a float atomic in
+// LDS costs about 3 cycles per active lane (a 64-lane ds_add_f32 takes ~190
+// cycles of a CU's LDS, a ds_write 5), and that was this step's cost.  Two
+// lanes of one instruction may write the same word only when their stencil
+// corners coincide (departures that converge); such lanes are found per
+// plane with a tag table and take LDS atomics instead.  x-neighbour lanes
+// whose corners are 1..3 words apart combine their shared words with DPP row
+// shifts before writing (as in sl_adjoint_k).  Stencils that leave the box
+// (departures of 2 cells or more) add into lam with HBM atomics.  The box and
+// tag accesses are volatile: the lanes read words other lanes wrote, which
+// the compiler must not forward from a lane's own earlier store.
+constexpr int AWZ = 4, WR = 3;  // node planes per wave, box reach
+constexpr int WBX = ABX + 2 * WR, WBY = ABY + 2 * WR, WBZ = AWZ + 2 * WR, WBOX = WBX * WBY * WBZ;
+constexpr int NTAG = 1024;
 
 template <typename T, int INTERP>
-__global__ __launch_bounds__(256) void adj_step_k(const T* __restrict__ phi0, const int4* __restrict__ rb,
-                                                  const void* __restrict__ rfv,
-                                                  const T* __restrict__ lam_next, T* __restrict__ lam,
-                                                  T* __restrict__ hsum, AdjP P) {
+__global__ __launch_bounds__(64) void adj_step_k(const T* __restrict__ phi0, const int4* __restrict__ rb,
+                                                 const void* __restrict__ rfv,
+                                                 const T* __restrict__ lam_next, T* __restrict__ lam,
+                                                 T* __restrict__ hsum, AdjP P) {
   constexpr int NP = INTERP == DCL_CUBIC ? 4 : 2;
   constexpr int OFF = INTERP == DCL_CUBIC ? -1 : 0;
   const Frac4<T, INTERP>* __restrict__ rf = static_cast<const Frac4<T, INTERP>*>(rfv);
-  __shared__ T box[SLBZ * LBY * LBX];
-  const int tid = threadIdx.x + ABX * (threadIdx.y + ABY * threadIdx.z);
-  for (int t = tid; t < SLBZ * LBY * LBX; t += 256) box[t] = T(0);
-  __syncthreads();
+  __shared__ T box_s[WBOX];
+  __shared__ int tag_s[NTAG];
+  typedef __attribute__((address_space(3))) T lds_T;
+  typedef __attribute__((address_space(3))) int lds_int;
+  volatile lds_T* box = (volatile lds_T*)box_s;
+  volatile lds_int* tag = (volatile lds_int*)tag_s;
+  const int lane = threadIdx.x;
+  for (int t = lane; t < WBOX; t += 64) box[t] = T(0);
 
-  const int ox = blockIdx.x * ABX, oy = blockIdx.y * ABY, oz = blockIdx.z * SBZ;
-  const int i = ox + threadIdx.x, j = oy + threadIdx.y, k0 = oz + 2 * threadIdx.z;
+  const int ox = blockIdx.x * ABX, oy = blockIdx.y * ABY, oz = blockIdx.z * AWZ;
+  const int i = ox + (lane & (ABX - 1)), j = oy + lane / ABX;
   const int nx = P.w.nx, ny = P.w.ny, nz = P.w.nz;
   const int Y = nx, Z = nx * ny;
-  // each node's record, lam_next and sensitivity sum so far, loaded together
-  T g[2] = {T(0), T(0)}, h0[2][3];
-  int b[2][3], flags[2] = {0, 0};
-  bool act[2];
-  double fr[2][3];
-#pragma unroll
-  for (int m = 0; m < 2; ++m) {
-    const int k = k0 + m;
+  const rsrc_t rp = make_rsrc(phi0, (unsigned)((int64_t)nx * ny * nz * (int64_t)sizeof(T)));
+#pragma unroll 1
+  for (int kz = 0; kz < AWZ; ++kz) {
+    const int k = oz + kz;
     const bool valid = i < nx && j < ny && k < nz;
     const bool inner = valid && i >= 2 && i <= nx - 3 && j >= 2 && j <= ny - 3 && k >= 2 && k <= nz - 3;
     const int id = k * Z + j * Y + i;
@@ -573,47 +495,51 @@ __global__ __launch_bounds__(256) void adj_step_k(const T* __restrict__ phi0, co
       const T gb = lam_next[id];
       if (gb != T(0)) atomic_add(&lam[id], gb);
     }
+    // the node's record, lam_next and sensitivity sum so far, loaded together
+    T g = T(0), h0[3] = {T(0), T(0), T(0)};
     int4 q = make_int4(0, 0, 0, 0);
     Frac4<T, INTERP> f4 = {0, 0, 0, 0};
-    h0[m][0] = h0[m][1] = h0[m][2] = T(0);
+    T* hs = hsum + 3 * (int64_t)id;
     if (inner) {
-      g[m] = lam_next[id];
+      g = lam_next[id];
       q = rb[id];
       f4 = rf[id];
-      const T* hs = hsum + 3 * (int64_t)id;
 #pragma unroll
-      for (int a = 0; a < 3; ++a) h0[m][a] = hs[a];
+      for (int a = 0; a < 3; ++a) h0[a] = hs[a];
     }
-    act[m] = inner && g[m] != T(0);
-    b[m][0] = q.x;
-    b[m][1] = q.y;
-    b[m][2] = q.z;
-    flags[m] = q.w;
-    fr[m][0] = (double)f4.x;
-    fr[m][1] = (double)f4.y;
-    fr[m][2] = (double)f4.z;
-  }
-  T w[2][3][4], dw[2][3][4];
-#pragma unroll
-  for (int m = 0; m < 2; ++m)
+    const bool act = inner && g != T(0);
+    const int b[3] = {q.x, q.y, q.z};
+    T w[3][4], dw[3][4];
+    const double fr[3] = {(double)f4.x, (double)f4.y, (double)f4.z};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      if (act[m]) {
-        axis_weights<T, INTERP>(fr[m][a], w[m][a], dw[m][a]);
+      if (act) {
+        if (INTERP == DCL_CUBIC) {
+          cr_weights<T>((T)fr[a], w[a]);
+          cr_dweights<T>((T)fr[a], dw[a]);
+        } else {
+          w[a][0] = (T)(1.0 - fr[a]);
+          w[a][1] = (T)fr[a];
+          dw[a][0] = T(-1);
+          dw[a][1] = T(1);
+        }
       } else {
 #pragma unroll
-        for (int qq = 0; qq < 4; ++qq) w[m][a][qq] = dw[m][a][qq] = T(0);
+        for (int qq = 0; qq < 4; ++qq) w[a][qq] = dw[a][qq] = T(0);
       }
     }
-  const rsrc_t rp = make_rsrc(phi0, (unsigned)((int64_t)nx * ny * nz * (int64_t)sizeof(T)));
-  T G[2][3] = {{T(0), T(0), T(0)}, {T(0), T(0), T(0)}};
-  // node 0's stencil corner in the box
-  const int bx0 = b[0][0] + OFF - (ox - AR), by0 = b[0][1] + OFF - (oy - AR), bz0 = b[0][2] + OFF - (oz - AR);
-  const bool paired = act[0] && act[1] && b[1][0] == b[0][0] && b[1][1] == b[0][1] && b[1][2] == b[0][2] + 1 &&
-                      bx0 >= 0 && bx0 + NP <= LBX && by0 >= 0 && by0 + NP <= LBY && bz0 >= 0 &&
-                      bz0 + NP + 1 <= SLBZ;
-  if (__all(paired || (!act[0] && !act[1]))) {
-    const int key = paired ? (bz0 * LBY + by0) * LBX + bx0 + 1 : 0;
+    const int bx0 = b[0] + OFF - (ox - WR), by0 = b[1] + OFF - (oy - WR), bz0 = b[2] + OFF - (oz - WR);
+    const bool in_box =
+        act && bx0 >= 0 && bx0 + NP <= WBX && by0 >= 0 && by0 + NP <= WBY && bz0 >= 0 && bz0 + NP <= WBZ;
+    const int key = in_box ? (bz0 * WBY + by0) * WBX + bx0 + 1 : 0;
+    // lanes sharing a stencil corner: each tags its slot, lanes that lose
+    // mark the slot, and every lane whose slot ends marked takes atomics
+    const int slot = key & (NTAG - 1);
+    if (in_box) tag[slot] = lane;
+    const bool lost = in_box && tag[slot] != lane;
+    if (lost) tag[slot] = -1;
+    const bool shared = in_box && tag[slot] == -1;
+    const bool plain = in_box && !shared;
     bool lo_m[4] = {false, false, false, false}, up_m[4] = {false, false, false, false};
 #pragma unroll
     for (int d = 1; d < NP; ++d) {
@@ -621,90 +547,73 @@ __global__ __launch_bounds__(256) void adj_step_k(const T* __restrict__ phi0, co
       lo_m[d] = key != 0 && kl != 0 && kl + d == key;
       up_m[d] = key != 0 && ku != 0 && key + d == ku;
     }
-    const int vo = paired ? ((b[0][2] + OFF) * Z + (b[0][1] + OFF) * Y + (b[0][0] + OFF)) * (int)sizeof(T) : 0;
-    // plane p of the pair's NP + 1: node 0's stencil plane p (p < NP), node
-    // 1's plane p - 1 (p > 0); the z weights are shifted down a slot per plane
-    T z0[NP + 1], zd0[NP + 1], z1[NP + 1], zd1[NP + 1];
-#pragma unroll
-    for (int p = 0; p <= NP; ++p) {
-      z0[p] = p < NP ? w[0][2][p] : T(0);
-      zd0[p] = p < NP ? dw[0][2][p] : T(0);
-      z1[p] = p > 0 ? w[1][2][p - 1] : T(0);
-      zd1[p] = p > 0 ? dw[1][2][p - 1] : T(0);
-    }
+    T G[3] = {T(0), T(0), T(0)};
+    // the stencil rows with buffer loads (a row of 4 values is one 16-byte
+    // load in FP32, two in FP64; inactive lanes read the window's first row),
+    // two stencil planes in flight at a time in FP32, one in FP64
+    const int vo = act ? ((b[2] + OFF) * Z + (b[1] + OFF) * Y + (b[0] + OFF)) * (int)sizeof(T) : 0;
+    constexpr int KP = sizeof(T) == 4 ? 2 : 1;
 #pragma unroll 1
-    for (int p = 0; p <= NP; ++p) {
-      T rows[4][4];
+    for (int k0 = 0; k0 < NP; k0 += KP) {
+      T rows[KP][4][4];
 #pragma unroll
-      for (int jj = 0; jj < NP; ++jj) {
-        const int so = (p * Z + jj * Y) * (int)sizeof(T);
-        if (NP == 4) BL<T>::four(rp, vo, so, rows[jj]);
-        else BL<T>::two(rp, vo, so, rows[jj]);
-      }
-      const T a0 = z0[0], ad0 = zd0[0], a1 = z1[0], ad1 = zd1[0];
+      for (int dk = 0; dk < KP; ++dk)
 #pragma unroll
-      for (int q = 0; q < NP; ++q) {
-        z0[q] = z0[q + 1];
-        zd0[q] = zd0[q + 1];
-        z1[q] = z1[q + 1];
-        zd1[q] = zd1[q + 1];
-      }
-#pragma unroll
-      for (int jj = 0; jj < NP; ++jj) {
-        const T wyz0 = w[0][1][jj] * a0, wyz1 = w[1][1][jj] * a1;
-        T add[4];
-#pragma unroll
-        for (int ii = 0; ii < NP; ++ii) add[ii] = g[0] * w[0][0][ii] * wyz0 + g[1] * w[1][0][ii] * wyz1;
-        T word0 = add[0];
-#pragma unroll
-        for (int d = 1; d < NP; ++d) {
-          const T o = lane_from_lower(add[d], d);
-          if (lo_m[d]) word0 += o;
+        for (int jj = 0; jj < NP; ++jj) {
+          const int so = ((k0 + dk) * Z + jj * Y) * (int)sizeof(T);
+          if (NP == 4) BL<T>::four(rp, vo, so, rows[dk][jj]);
+          else BL<T>::two(rp, vo, so, rows[dk][jj]);
         }
-        if (paired) {
-          T rx0 = T(0), rd0 = T(0), rx1 = T(0), rd1 = T(0);
+#pragma unroll
+      for (int dk = 0; dk < KP; ++dk)
+#pragma unroll
+        for (int jj = 0; jj < NP; ++jj) {
+          const int kk = k0 + dk;
+          const T wyz = w[1][jj] * w[2][kk];
+          T add[4];
+#pragma unroll
+          for (int ii = 0; ii < NP; ++ii) add[ii] = g * w[0][ii] * wyz;
+          T word0 = add[0];
+#pragma unroll
+          for (int d = 1; d < NP; ++d) {
+            const T o = lane_from_lower(add[d], d);
+            if (lo_m[d]) word0 += o;
+          }
+          if (act) {
+            T rx = T(0), rd = T(0);
+#pragma unroll
+            for (int ii = 0; ii < NP; ++ii) {
+              const T f = rows[dk][jj][ii];
+              rx += w[0][ii] * f;
+              rd += dw[0][ii] * f;
+            }
+            G[0] += rd * wyz;
+            G[1] += rx * dw[1][jj] * w[2][kk];
+            G[2] += rx * w[1][jj] * dw[2][kk];
+          }
+          const int wo = ((bz0 + kk) * WBY + (by0 + jj)) * WBX + bx0;
 #pragma unroll
           for (int ii = 0; ii < NP; ++ii) {
-            const T f = rows[jj][ii];
-            rx0 += w[0][0][ii] * f;
-            rd0 += dw[0][0][ii] * f;
-            rx1 += w[1][0][ii] * f;
-            rd1 += dw[1][0][ii] * f;
-            T* bw = &box[((bz0 + p) * LBY + (by0 + jj)) * LBX + (bx0 + ii)];
-            if (ii == 0)
-              atomicAdd(bw, word0);
-            else if (!up_m[ii])
-              atomicAdd(bw, add[ii]);
+            const T v = ii == 0 ? word0 : add[ii];
+            const bool mine = ii == 0 || !up_m[ii];
+            if (plain && mine) box[wo + ii] = box[wo + ii] + v;
+            if (shared && mine) atomicAdd(&box_s[wo + ii], v);
+            if (act && !in_box)
+              atomic_add(&lam[(b[2] + OFF + kk) * Z + (b[1] + OFF + jj) * Y + (b[0] + OFF + ii)], add[ii]);
           }
-          G[0][0] += rd0 * wyz0;
-          G[0][1] += rx0 * dw[0][1][jj] * a0;
-          G[0][2] += rx0 * w[0][1][jj] * ad0;
-          G[1][0] += rd1 * wyz1;
-          G[1][1] += rx1 * dw[1][1][jj] * a1;
-          G[1][2] += rx1 * w[1][1][jj] * ad1;
         }
-      }
     }
-  } else {
-#pragma unroll
-    for (int m = 0; m < 2; ++m)
-      adj_scatter_one<T, INTERP>(rp, box, lam, Y, Z, act[m], g[m], b[m], w[m], dw[m], ox, oy, oz, G[m]);
-  }
-#pragma unroll
-  for (int m = 0; m < 2; ++m) {
-    if (act[m]) {
-      T* hs = hsum + 3 * (int64_t)((k0 + m) * Z + j * Y + i);
+    if (act) {
 #pragma unroll
       for (int a = 0; a < 3; ++a)
-        if (!(flags[m] & (1 << a))) hs[a] = h0[m][a] + (T)(-P.c[a]) * g[m] * G[m][a];
+        if (!(q.w & (1 << a))) hs[a] = h0[a] + (T)(-P.c[a]) * g * G[a];
     }
   }
-  __syncthreads();
-  for (int t = tid; t < SLBZ * LBY * LBX; t += 256) {
+  for (int t = lane; t < WBOX; t += 64) {
     const T v = box[t];
     if (v == T(0)) continue;
-    const int lx = t % LBX, ly = (t / LBX) % LBY, lz = t / (LBX * LBY);
-    const int gx = ox - AR + lx, gy = oy - AR + ly, gz = oz - AR + lz;
+    const int lx = t % WBX, ly = (t / WBX) % WBY, lz = t / (WBX * WBY);
+    const int gx = ox - WR + lx, gy = oy - WR + ly, gz = oz - WR + lz;
     if (gx < 0 || gx >= nx || gy < 0 || gy >= ny || gz < 0 || gz >= nz) continue;
     atomic_add(&lam[gz * Z + gy * Y + gx], v);
   }
@@ -868,14 +777,14 @@ static int adj_step_t(int interp, const void* phi0, const void* record, const vo
                       void* hsum, const AdjP& P, hipStream_t s) {
   const int64_t n = (int64_t)P.w.nx * P.w.ny * P.w.nz;
   const int4* rb = static_cast<const int4*>(record);
-  dim3 grid((P.w.nx + ABX - 1) / ABX, (P.w.ny + ABY - 1) / ABY, (P.w.nz + SBZ - 1) / SBZ);
+  dim3 grid((P.w.nx + ABX - 1) / ABX, (P.w.ny + ABY - 1) / ABY, (P.w.nz + AWZ - 1) / AWZ);
   const T* p0 = static_cast<const T*>(phi0);
   const T* ln = static_cast<const T*>(lam_next);
   if (interp == DCL_CUBIC)
-    hipLaunchKernelGGL((adj_step_k<T, DCL_CUBIC>), grid, dim3(ABX, ABY, ABZ), 0, s, p0, rb,
+    hipLaunchKernelGGL((adj_step_k<T, DCL_CUBIC>), grid, dim3(64), 0, s, p0, rb,
                        static_cast<const void*>(rb + n), ln, static_cast<T*>(lam), static_cast<T*>(hsum), P);
   else
-    hipLaunchKernelGGL((adj_step_k<T, DCL_LINEAR>), grid, dim3(ABX, ABY, ABZ), 0, s, p0, rb,
+    hipLaunchKernelGGL((adj_step_k<T, DCL_LINEAR>), grid, dim3(64), 0, s, p0, rb,
                        static_cast<const void*>(rb + n), ln, static_cast<T*>(lam), static_cast<T*>(hsum), P);
   DCL_LAUNCHED("adj_step");
   return 0;
