@@ -163,202 +163,226 @@ __device__ __forceinline__ void mac_scatter_lanes(T* __restrict__ gvel, const KV
   }
 }
 
-// LDS box of lam for one 16x4x4 block of nodes plus the stencil reach R (a
-// block's 16-lane DPP rows are its x rows)
+// A wave's lanes are a 16 x 4 plane of nodes (its 16-lane DPP rows are x
+// rows, lanes ABX apart are y neighbours); adj_finish_k's blocks stack ABZ
+// such planes.
 constexpr int ABX = 16, ABY = 4, ABZ = 4;
 static_assert(ABX * ABY == 64, "a wave is one z plane of the block: lanes ABX apart are y neighbours");
-constexpr int AR = 5;  // reach: |departure| <= AR - 2 cells lands in the box (cubic stencil -1..+2)
-constexpr int LBX = ABX + 2 * AR, LBY = ABY + 2 * AR, LBZ = ABZ + 2 * AR;
+
+// One wave per block: its 16 x 4 lanes own a 16 x 4 column of nodes and walk
+// AWZ planes of it, scattering lam into a wave-private LDS box (the column
+// plus the reach WR) with plain read-add-write, lanes that share a stencil
+// corner found with a tag table and given LDS atomics, stencils beyond the
+// box added into lam with HBM atomics (as adj_step_k below); the velocity
+// gradient goes to HBM with atomics, x- and y-neighbour lanes combining the
+// MAC corners and node faces they share.
+constexpr int AWZ = 4, WR = 3;  // node planes per wave, box reach
+constexpr int WBX = ABX + 2 * WR, WBY = ABY + 2 * WR, WBZ = AWZ + 2 * WR, WBOX = WBX * WBY * WBZ;
+constexpr int NTAG = 1024;
 
 template <typename T, int INTERP, int BT>
-__global__ __launch_bounds__(256) void sl_adjoint_k(const T* __restrict__ phi0, const T* __restrict__ vel,
-                                                    const T* __restrict__ lam_next, T* __restrict__ lam,
-                                                    T* __restrict__ gvel, AdjP P) {
-  __shared__ T box[LBZ * LBY * LBX];
-  const int tid = threadIdx.x + ABX * (threadIdx.y + ABY * threadIdx.z);
-  for (int i = tid; i < LBZ * LBY * LBX; i += 256) box[i] = T(0);
-  __syncthreads();
+__global__ __launch_bounds__(64) void sl_adjoint_k(const T* __restrict__ phi0, const T* __restrict__ vel,
+                                                   const T* __restrict__ lam_next, T* __restrict__ lam,
+                                                   T* __restrict__ gvel, AdjP P) {
+  __shared__ T box_s[WBOX];
+  __shared__ int tag_s[NTAG];
+  typedef __attribute__((address_space(3))) T lds_T;
+  typedef __attribute__((address_space(3))) int lds_int;
+  volatile lds_T* box = (volatile lds_T*)box_s;
+  volatile lds_int* tag = (volatile lds_int*)tag_s;
+  const int lane = threadIdx.x;
+  for (int t = lane; t < WBOX; t += 64) box[t] = T(0);
 
-  const int ox = blockIdx.x * ABX, oy = blockIdx.y * ABY, oz = blockIdx.z * ABZ;
-  const int i = ox + threadIdx.x, j = oy + threadIdx.y, k = oz + threadIdx.z;
+  const int ox = blockIdx.x * ABX, oy = blockIdx.y * ABY, oz = blockIdx.z * AWZ;
+  const int i = ox + (lane & (ABX - 1)), j = oy + lane / ABX;
   const int nx = P.w.nx, ny = P.w.ny, nz = P.w.nz;
   const int Y = nx, Z = nx * ny;
-  const bool valid = i < nx && j < ny && k < nz;
-  const bool inner = valid && i >= 2 && i <= nx - 3 && j >= 2 && j <= ny - 3 && k >= 2 && k <= nz - 3;
-  if (valid && !inner) {
-    const int id = k * Z + j * Y + i;
-    const T g = lam_next[id];
-    if (g != T(0)) atomic_add(&lam[id], g);
-  }
-  // The stencil loop runs with every lane of the wave (lanes without work
-  // carry g = 0 and no box key), so that x-neighbour lanes can combine their
-  // contributions to the same LDS word before the atomic.
-  T g = T(0);
-  if (inner) g = lam_next[k * Z + j * Y + i];
-  const bool act = inner && g != T(0);
   const int NP = INTERP == DCL_CUBIC ? 4 : 2;
   const int OFF = INTERP == DCL_CUBIC ? -1 : 0;
-  double Xm[3] = {0.0, 0.0, 0.0};
-  int b[3] = {0, 0, 0};
-  bool cs[3] = {false, false, false};
-  T w[3][4], dw[3][4];
-  int bx0 = 0, by0 = 0, bz0 = 0;
-  bool in_box = false;
-  if (act) {
-    const int X = P.w.px + i, Yg = P.w.py + j, Zg = P.w.pz + k;
-    const double Xd[3] = {(double)X, (double)Yg, (double)Zg};
-    T V1[3];
-    node_vel<T, 3>(vel, P.v, X, Yg, Zg, V1);
-    // the midpoint value now (the forward's arithmetic); its derivative
-    // state only after the stencil loop, to keep it out of the loop's
-    // registers (more waves in flight hide the dependent gathers)
-    double S[3];
-    if (BT == DCL_RK2) {
+#pragma unroll 1
+  for (int kz = 0; kz < AWZ; ++kz) {
+    const int k = oz + kz;
+    const bool valid = i < nx && j < ny && k < nz;
+    const bool inner = valid && i >= 2 && i <= nx - 3 && j >= 2 && j <= ny - 3 && k >= 2 && k <= nz - 3;
+    if (valid && !inner) {
+      const int id = k * Z + j * Y + i;
+      const T g = lam_next[id];
+      if (g != T(0)) atomic_add(&lam[id], g);
+    }
+    // The stencil loop runs with every lane of the wave (lanes without work
+    // carry g = 0 and no box key), so that x-neighbour lanes can combine
+    // their contributions to the same LDS word before writing it.
+    T g = T(0);
+    if (inner) g = lam_next[k * Z + j * Y + i];
+    const bool act = inner && g != T(0);
+    double Xm[3] = {0.0, 0.0, 0.0};
+    int b[3] = {0, 0, 0};
+    bool cs[3] = {false, false, false};
+    T w[3][4], dw[3][4];
+    int bx0 = 0, by0 = 0, bz0 = 0;
+    bool in_box = false;
+    if (act) {
+      const int X = P.w.px + i, Yg = P.w.py + j, Zg = P.w.pz + k;
+      const double Xd[3] = {(double)X, (double)Yg, (double)Zg};
+      T V1[3];
+      node_vel<T, 3>(vel, P.v, X, Yg, Zg, V1);
+      // the midpoint value now (the forward's arithmetic); its derivative
+      // state only after the stencil loop, to keep it out of the loop's
+      // registers
+      double S[3];
+      if (BT == DCL_RK2) {
 #pragma unroll
-      for (int a = 0; a < 3; ++a) Xm[a] = Xd[a] - (double)V1[a] * P.ch[a];
-      T V2[3];
-      mac_vel<T, 3>(vel, P.v, Xm, V2);
+        for (int a = 0; a < 3; ++a) Xm[a] = Xd[a] - (double)V1[a] * P.ch[a];
+        T V2[3];
+        mac_vel<T, 3>(vel, P.v, Xm, V2);
 #pragma unroll
-      for (int a = 0; a < 3; ++a) S[a] = Xd[a] - (double)V2[a] * P.c[a];
+        for (int a = 0; a < 3; ++a) S[a] = Xd[a] - (double)V2[a] * P.c[a];
+      } else {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) S[a] = Xd[a] - (double)V1[a] * P.c[a];
+      }
+      // split + weights
+      const int pp[3] = {P.w.px, P.w.py, P.w.pz}, nn[3] = {nx, ny, nz};
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        double f;
+        if (INTERP == DCL_CUBIC) {
+          const double lo = pp[a] + 1, hi = pp[a] + nn[a] - 2;
+          cs[a] = !(S[a] >= lo && S[a] <= hi);
+          b[a] = cub_split(S[a], pp[a], nn[a], f);
+          cr_weights<T>((T)f, w[a]);
+          cr_dweights<T>((T)f, dw[a]);
+        } else {
+          const double lo = pp[a], hi = pp[a] + nn[a] - 1;
+          cs[a] = !(S[a] >= lo && S[a] <= hi);
+          b[a] = lin_split(S[a], pp[a], nn[a], f);
+          w[a][0] = (T)(1.0 - f);
+          w[a][1] = (T)f;
+          dw[a][0] = T(-1);
+          dw[a][1] = T(1);
+        }
+      }
+      bx0 = b[0] + OFF - (ox - WR);
+      by0 = b[1] + OFF - (oy - WR);
+      bz0 = b[2] + OFF - (oz - WR);
+      in_box = bx0 >= 0 && bx0 + NP <= WBX && by0 >= 0 && by0 + NP <= WBY && bz0 >= 0 && bz0 + NP <= WBZ;
     } else {
 #pragma unroll
-      for (int a = 0; a < 3; ++a) S[a] = Xd[a] - (double)V1[a] * P.c[a];
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) w[a][q] = dw[a][q] = T(0);
     }
-    // split + weights
-    const int pp[3] = {P.w.px, P.w.py, P.w.pz}, nn[3] = {nx, ny, nz};
+    // the box word of the lane's stencil corner, plus one (0: no box word).
+    // Lanes d = 1 .. NP-1 apart in a 16-lane row whose keys differ by d
+    // write the same word in every stencil row, the lower lane with its
+    // entry d and the upper lane with its entry 0: the lower one leaves it
+    // to the upper one, which adds both.
+    const int key = in_box ? (bz0 * WBY + by0) * WBX + bx0 + 1 : 0;
+    const int slot = key & (NTAG - 1);
+    if (in_box) tag[slot] = lane;
+    const bool lost = in_box && tag[slot] != lane;
+    if (lost) tag[slot] = -1;
+    const bool shared = in_box && tag[slot] == -1;
+    const bool plain = in_box && !shared;
+    bool lo_m[4] = {false, false, false, false}, up_m[4] = {false, false, false, false};
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      double f;
-      if (INTERP == DCL_CUBIC) {
-        const double lo = pp[a] + 1, hi = pp[a] + nn[a] - 2;
-        cs[a] = !(S[a] >= lo && S[a] <= hi);
-        b[a] = cub_split(S[a], pp[a], nn[a], f);
-        cr_weights<T>((T)f, w[a]);
-        cr_dweights<T>((T)f, dw[a]);
-      } else {
-        const double lo = pp[a], hi = pp[a] + nn[a] - 1;
-        cs[a] = !(S[a] >= lo && S[a] <= hi);
-        b[a] = lin_split(S[a], pp[a], nn[a], f);
-        w[a][0] = (T)(1.0 - f);
-        w[a][1] = (T)f;
-        dw[a][0] = T(-1);
-        dw[a][1] = T(1);
-      }
+    for (int d = 1; d < NP; ++d) {
+      const int kl = lane_from_lower(key, d), ku = lane_from_upper(key, d);
+      lo_m[d] = key != 0 && kl != 0 && kl + d == key;
+      up_m[d] = key != 0 && ku != 0 && key + d == ku;
     }
-    bx0 = b[0] + OFF - (ox - AR);
-    by0 = b[1] + OFF - (oy - AR);
-    bz0 = b[2] + OFF - (oz - AR);
-    in_box = bx0 >= 0 && bx0 + NP <= LBX && by0 >= 0 && by0 + NP <= LBY && bz0 >= 0 && bz0 + NP <= LBZ;
-  } else {
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) w[a][q] = dw[a][q] = T(0);
-  }
-  // the box word of the lane's stencil corner, plus one (0: no box word).
-  // Lanes d = 1 .. NP-1 apart in a 16-lane row whose keys differ by d write
-  // the same word in every stencil row, the lower lane with its entry d and
-  // the upper lane with its entry 0: the lower one leaves it to the upper
-  // one, which adds both.
-  const int key = in_box ? (bz0 * LBY + by0) * LBX + bx0 + 1 : 0;
-  bool lo_m[4] = {false, false, false, false}, up_m[4] = {false, false, false, false};
-#pragma unroll
-  for (int d = 1; d < NP; ++d) {
-    const int kl = lane_from_lower(key, d), ku = lane_from_upper(key, d);
-    lo_m[d] = key != 0 && kl != 0 && kl + d == key;
-    up_m[d] = key != 0 && ku != 0 && key + d == ku;
-  }
-  // transposed gather into the LDS box (or HBM outside it) + gradient gather
-  T G[3] = {T(0), T(0), T(0)};
-  const T* src = phi0 + (b[2] + OFF) * Z + (b[1] + OFF) * Y + (b[0] + OFF);
-  // one stencil plane at a time: bounds the loads in flight (registers)
+    // transposed gather into the box (or HBM outside it) + gradient gather
+    T G[3] = {T(0), T(0), T(0)};
+    const T* src = phi0 + (b[2] + OFF) * Z + (b[1] + OFF) * Y + (b[0] + OFF);
+    // one stencil plane at a time: bounds the loads in flight (registers)
 #pragma unroll 1
-  for (int kk = 0; kk < NP; ++kk)
+    for (int kk = 0; kk < NP; ++kk)
 #pragma unroll
-    for (int jj = 0; jj < NP; ++jj) {
-      const T wyz = w[1][jj] * w[2][kk];
-      T add[4];
+      for (int jj = 0; jj < NP; ++jj) {
+        const T wyz = w[1][jj] * w[2][kk];
+        T add[4];
 #pragma unroll
-      for (int ii = 0; ii < NP; ++ii) add[ii] = g * w[0][ii] * wyz;
-      T word0 = add[0];
+        for (int ii = 0; ii < NP; ++ii) add[ii] = g * w[0][ii] * wyz;
+        T word0 = add[0];
 #pragma unroll
-      for (int d = 1; d < NP; ++d) {
-        const T o = lane_from_lower(add[d], d);
-        if (lo_m[d]) word0 += o;
-      }
-      if (act) {
-        const T* r = src + kk * Z + jj * Y;
-        T rx = T(0), rd = T(0);
+        for (int d = 1; d < NP; ++d) {
+          const T o = lane_from_lower(add[d], d);
+          if (lo_m[d]) word0 += o;
+        }
+        if (act) {
+          const T* r = src + kk * Z + jj * Y;
+          T rx = T(0), rd = T(0);
+#pragma unroll
+          for (int ii = 0; ii < NP; ++ii) {
+            const T f = r[ii];
+            rx += w[0][ii] * f;
+            rd += dw[0][ii] * f;
+          }
+          G[0] += rd * wyz;
+          G[1] += rx * dw[1][jj] * w[2][kk];
+          G[2] += rx * w[1][jj] * dw[2][kk];
+        }
+        const int wo = ((bz0 + kk) * WBY + (by0 + jj)) * WBX + bx0;
 #pragma unroll
         for (int ii = 0; ii < NP; ++ii) {
-          const T f = r[ii];
-          rx += w[0][ii] * f;
-          rd += dw[0][ii] * f;
-          if (in_box) {
-            T* bw = &box[((bz0 + kk) * LBY + (by0 + jj)) * LBX + (bx0 + ii)];
-            if (ii == 0)
-              atomicAdd(bw, word0);
-            else if (!up_m[ii])
-              atomicAdd(bw, add[ii]);
-          } else {
+          const T v = ii == 0 ? word0 : add[ii];
+          const bool mine = ii == 0 || !up_m[ii];
+          if (plain && mine) box[wo + ii] = box[wo + ii] + v;
+          if (shared && mine) atomicAdd(&box_s[wo + ii], v);
+          if (act && !in_box)
             atomic_add(&lam[(b[2] + OFF + kk) * Z + (b[1] + OFF + jj) * Y + (b[0] + OFF + ii)], add[ii]);
-          }
         }
-        G[0] += rd * wyz;
-        G[1] += rx * dw[1][jj] * w[2][kk];
-        G[2] += rx * w[1][jj] * dw[2][kk];
+      }
+    T H[3] = {T(0), T(0), T(0)}, H1[3] = {T(0), T(0), T(0)};
+    MacState<T> m2;
+    if (act) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) H[a] = cs[a] ? T(0) : (T)(-P.c[a]) * g * G[a];
+      if (BT == DCL_RK2) {
+        bool cm[3] = {false, false, false};
+        mac_state<T>(vel, P.v, Xm, m2, cm);
+#pragma unroll
+        for (int bb = 0; bb < 3; ++bb) {
+          const T K = H[0] * m2.dV[0][bb] + H[1] * m2.dV[1][bb] + H[2] * m2.dV[2][bb];
+          H1[bb] = (T)(-P.ch[bb]) * K;
+        }
+      } else {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) H1[a] = H[a];
       }
     }
-  T H[3] = {T(0), T(0), T(0)}, H1[3] = {T(0), T(0), T(0)};
-  MacState<T> m2;
-  if (act) {
-#pragma unroll
-    for (int a = 0; a < 3; ++a) H[a] = cs[a] ? T(0) : (T)(-P.c[a]) * g * G[a];
-    if (BT == DCL_RK2) {
-      bool cm[3] = {false, false, false};
-      mac_state<T>(vel, P.v, Xm, m2, cm);
-#pragma unroll
-      for (int bb = 0; bb < 3; ++bb) {
-        const T K = H[0] * m2.dV[0][bb] + H[1] * m2.dV[1][bb] + H[2] * m2.dV[2][bb];
-        H1[bb] = (T)(-P.ch[bb]) * K;
-      }
-    } else {
-#pragma unroll
-      for (int a = 0; a < 3; ++a) H1[a] = H[a];
+    if (BT == DCL_RK2) mac_scatter_lanes<T>(gvel, P.v, m2, H, act);
+    // node velocity: V_a = (f(X) + f(X + e_a)) / 2.  The x face at X + e_x
+    // is the next x lane's face at its node, and the y face at X + e_y that
+    // of the lane ABX above (the node at y + 1 in the same wave): that lane
+    // adds both.
+    const int sx = P.v.dof, sy = P.w.nx * P.v.dof, sz = P.w.ny * sy;
+    const int gofs = (k * P.w.ny + j) * sy + i * sx + P.v.udof;
+    const T h0 = T(0.5) * H1[0], h1 = T(0.5) * H1[1], h2 = T(0.5) * H1[2];
+    const int nkey = act ? gofs + 1 : 0;
+    const int nkl = lane_from_lower(nkey, 1), nku = lane_from_upper(nkey, 1);
+    const bool nlo = act && nkl != 0 && nkl + sx == nkey, nup = act && nku != 0 && nkey + sx == nku;
+    const T h0l = lane_from_lower(h0, 1);
+    const int ykl = __shfl_up(nkey, ABX), yku = __shfl_down(nkey, ABX);
+    const bool ylo = act && lane >= ABX && ykl != 0 && ykl + sy == nkey;
+    const bool yup = act && lane < 64 - ABX && yku != 0 && nkey + sy == yku;
+    const T h1l = __shfl_up(h1, ABX);
+    if (act) {
+      T* gb = gvel + gofs;
+      atomic_add(gb, nlo ? h0 + h0l : h0);
+      if (!nup) atomic_add(gb + sx, h0);
+      atomic_add(gb + 1, ylo ? h1 + h1l : h1);
+      if (!yup) atomic_add(gb + sy + 1, h1);
+      atomic_add(gb + 2, h2);
+      atomic_add(gb + sz + 2, h2);
     }
   }
-  if (BT == DCL_RK2) mac_scatter_lanes<T>(gvel, P.v, m2, H, act);
-  // node velocity: V_a = (f(X) + f(X + e_a)) / 2.  The x face at X + e_x is
-  // the next x lane's face at its node, and the y face at X + e_y that of the
-  // lane ABX above (the node at y + 1 in the same wave): that lane adds both.
-  const int sx = P.v.dof, sy = P.w.nx * P.v.dof, sz = P.w.ny * sy;
-  const int gofs = (k * P.w.ny + j) * sy + i * sx + P.v.udof;
-  const T h0 = T(0.5) * H1[0], h1 = T(0.5) * H1[1], h2 = T(0.5) * H1[2];
-  const int nkey = act ? gofs + 1 : 0;
-  const int nkl = lane_from_lower(nkey, 1), nku = lane_from_upper(nkey, 1);
-  const bool nlo = act && nkl != 0 && nkl + sx == nkey, nup = act && nku != 0 && nkey + sx == nku;
-  const T h0l = lane_from_lower(h0, 1);
-  const int lane = tid & 63;  // the linear thread index: x + ABX (y + ABY z)
-  const int ykl = __shfl_up(nkey, ABX), yku = __shfl_down(nkey, ABX);
-  const bool ylo = act && lane >= ABX && ykl != 0 && ykl + sy == nkey;
-  const bool yup = act && lane < 64 - ABX && yku != 0 && nkey + sy == yku;
-  const T h1l = __shfl_up(h1, ABX);
-  if (act) {
-    T* gb = gvel + gofs;
-    atomic_add(gb, nlo ? h0 + h0l : h0);
-    if (!nup) atomic_add(gb + sx, h0);
-    atomic_add(gb + 1, ylo ? h1 + h1l : h1);
-    if (!yup) atomic_add(gb + sy + 1, h1);
-    atomic_add(gb + 2, h2);
-    atomic_add(gb + sz + 2, h2);
-  }
-  __syncthreads();
   // flush the box: one HBM atomic per nonzero word inside the window
-  for (int t = tid; t < LBZ * LBY * LBX; t += 256) {
+  for (int t = lane; t < WBOX; t += 64) {
     const T v = box[t];
     if (v == T(0)) continue;
-    const int lx = t % LBX, ly = (t / LBX) % LBY, lz = t / (LBX * LBY);
-    const int gx = ox - AR + lx, gy = oy - AR + ly, gz = oz - AR + lz;
+    const int lx = t % WBX, ly = (t / WBX) % WBY, lz = t / (WBX * WBY);
+    const int gx = ox - WR + lx, gy = oy - WR + ly, gz = oz - WR + lz;
     if (gx < 0 || gx >= nx || gy < 0 || gy >= ny || gz < 0 || gz >= nz) continue;
     atomic_add(&lam[gz * Z + gy * Y + gx], v);
   }
@@ -459,10 +483,6 @@ __global__ __launch_bounds__(256) void adj_prepare_k(const T* __restrict__ vel, 
 // (departures of 2 cells or more) add into lam with HBM atomics.  The box and
 // tag accesses are volatile: the lanes read words other lanes wrote, which
 // the compiler must not forward from a lane's own earlier store.
-constexpr int AWZ = 4, WR = 3;  // node planes per wave, box reach
-constexpr int WBX = ABX + 2 * WR, WBY = ABY + 2 * WR, WBZ = AWZ + 2 * WR, WBOX = WBX * WBY * WBZ;
-constexpr int NTAG = 1024;
-
 template <typename T, int INTERP>
 __global__ __launch_bounds__(64) void adj_step_k(const T* __restrict__ phi0, const int4* __restrict__ rb,
                                                  const void* __restrict__ rfv,
@@ -485,28 +505,45 @@ __global__ __launch_bounds__(64) void adj_step_k(const T* __restrict__ phi0, con
   const int nx = P.w.nx, ny = P.w.ny, nz = P.w.nz;
   const int Y = nx, Z = nx * ny;
   const rsrc_t rp = make_rsrc(phi0, (unsigned)((int64_t)nx * ny * nz * (int64_t)sizeof(T)));
+  // a node's record, lam_next and sensitivity sum so far, loaded together,
+  // the next plane's while this plane's stencils are worked
+  auto inner_at = [&](int k) {
+    return i >= 2 && i <= nx - 3 && j >= 2 && j <= ny - 3 && k >= 2 && k <= nz - 3;
+  };
+  T ng = T(0), nh[3] = {T(0), T(0), T(0)};
+  int4 nq = make_int4(0, 0, 0, 0);
+  Frac4<T, INTERP> nf = {0, 0, 0, 0};
+  auto load_node = [&](int k) {
+    ng = T(0);
+    nh[0] = nh[1] = nh[2] = T(0);
+    nq = make_int4(0, 0, 0, 0);
+    nf = Frac4<T, INTERP>{0, 0, 0, 0};
+    if (inner_at(k)) {
+      const int id = k * Z + j * Y + i;
+      ng = lam_next[id];
+      nq = rb[id];
+      nf = rf[id];
+      const T* hs = hsum + 3 * (int64_t)id;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) nh[a] = hs[a];
+    }
+  };
+  load_node(oz);
 #pragma unroll 1
   for (int kz = 0; kz < AWZ; ++kz) {
     const int k = oz + kz;
     const bool valid = i < nx && j < ny && k < nz;
-    const bool inner = valid && i >= 2 && i <= nx - 3 && j >= 2 && j <= ny - 3 && k >= 2 && k <= nz - 3;
+    const bool inner = valid && inner_at(k);
     const int id = k * Z + j * Y + i;
     if (valid && !inner) {
       const T gb = lam_next[id];
       if (gb != T(0)) atomic_add(&lam[id], gb);
     }
-    // the node's record, lam_next and sensitivity sum so far, loaded together
-    T g = T(0), h0[3] = {T(0), T(0), T(0)};
-    int4 q = make_int4(0, 0, 0, 0);
-    Frac4<T, INTERP> f4 = {0, 0, 0, 0};
+    const T g = ng, h0[3] = {nh[0], nh[1], nh[2]};
+    const int4 q = nq;
+    const Frac4<T, INTERP> f4 = nf;
+    if (kz + 1 < AWZ) load_node(k + 1);
     T* hs = hsum + 3 * (int64_t)id;
-    if (inner) {
-      g = lam_next[id];
-      q = rb[id];
-      f4 = rf[id];
-#pragma unroll
-      for (int a = 0; a < 3; ++a) h0[a] = hs[a];
-    }
     const bool act = inner && g != T(0);
     const int b[3] = {q.x, q.y, q.z};
     T w[3][4], dw[3][4];
@@ -550,9 +587,9 @@ __global__ __launch_bounds__(64) void adj_step_k(const T* __restrict__ phi0, con
     T G[3] = {T(0), T(0), T(0)};
     // the stencil rows with buffer loads (a row of 4 values is one 16-byte
     // load in FP32, two in FP64; inactive lanes read the window's first row),
-    // two stencil planes in flight at a time in FP32, one in FP64
+    // all stencil planes in flight at once in FP32, one at a time in FP64
     const int vo = act ? ((b[2] + OFF) * Z + (b[1] + OFF) * Y + (b[0] + OFF)) * (int)sizeof(T) : 0;
-    constexpr int KP = sizeof(T) == 4 ? 2 : 1;
+    constexpr int KP = sizeof(T) == 4 ? NP : 1;
 #pragma unroll 1
     for (int k0 = 0; k0 < NP; k0 += KP) {
       T rows[KP][4][4];
@@ -689,8 +726,8 @@ __global__ __launch_bounds__(256) void adj_finish_k(const T* __restrict__ vel, c
 template <typename T, int INTERP, int BT>
 static int launch_adj(const void* phi0, const void* vel, const void* lam_next, void* lam, void* gvel,
                       const AdjP& P, hipStream_t s) {
-  dim3 grid((P.w.nx + ABX - 1) / ABX, (P.w.ny + ABY - 1) / ABY, (P.w.nz + ABZ - 1) / ABZ);
-  hipLaunchKernelGGL((sl_adjoint_k<T, INTERP, BT>), grid, dim3(ABX, ABY, ABZ), 0, s,
+  dim3 grid((P.w.nx + ABX - 1) / ABX, (P.w.ny + ABY - 1) / ABY, (P.w.nz + AWZ - 1) / AWZ);
+  hipLaunchKernelGGL((sl_adjoint_k<T, INTERP, BT>), grid, dim3(64), 0, s,
                      static_cast<const T*>(phi0), static_cast<const T*>(vel),
                      static_cast<const T*>(lam_next), static_cast<T*>(lam), static_cast<T*>(gvel), P);
   DCL_LAUNCHED("sl_adjoint");
