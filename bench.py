@@ -6,7 +6,11 @@ LeVeque deformation velocity with its cos(pi t/3) factor folded into dt
 tricubic SL followed by the 19-point curvature each step.  One step = one pass
 over the grid; one grid-point update = one phi value written by the SL step
 and one curvature value at the same node, counted over the (N-4)^3 interior
-nodes the SL step advects.  Inputs are resident in HBM when timing starts.
+nodes the SL step advects.  Inputs are resident in HBM when timing starts;
+the velocity is held as component planes (one n^3 plane per component, as
+dcl_get_velocity_planes writes it from the fluid), the layout the FP64 SL
+kernel gathers fastest from; the form fp64_config_scheme_256_per_node_velocity
+times the same step with the reference's per-node (u, v, w) layout.
 
 Beside the headline, "forms" reports the same metric for the reference's own
 scheme (Euler + trilinear), the FP32 form at 384^3 and the other workloads,
@@ -63,10 +67,21 @@ def parse():
     return ap.parse_args()
 
 
-def dense_run(n, dtype, interp, bt, steps, warmup, deform=True):
+def deformation_velocity_dev(n, dtype, planes):
+    """The deformation velocity on the device, per node (n, n, n, 3) or as
+    component planes (3, n, n, n), the layout dcl_get_velocity_planes writes."""
+    import torch
+    from dcell_amd import configs
+    v = configs.deformation_velocity(n)
+    if planes:
+        v = np.ascontiguousarray(np.moveaxis(v, -1, 0))
+    return torch.from_numpy(v).to("cuda:0", dtype)
+
+
+def dense_run(n, dtype, interp, bt, steps, warmup, deform=True, planes=False):
     """Time `steps` dense updates (SL kernel then curvature kernel) of an n^3
     sphere under the deformation field; per-kernel times from HIP events on
-    the stream the kernels run on."""
+    the stream the kernels run on.  ``planes``: velocity as component planes."""
     import torch
     import dcell_amd as D
     from dcell_amd import configs
@@ -74,7 +89,7 @@ def dense_run(n, dtype, interp, bt, steps, warmup, deform=True):
     win = D.GridWindow.unit_box(n)
     dev = torch.device("cuda:0")
     phi = torch.from_numpy(configs.sphere_sdf(n)).to(dev, dtype)
-    vel = torch.from_numpy(configs.deformation_velocity(n)).to(dev, dtype)
+    vel = deformation_velocity_dev(n, dtype, planes)
     bufs = [phi, torch.empty_like(phi)]
     kappa = torch.empty_like(phi)
     dts = [configs.deformation_dt(n, s) for s in range(warmup + steps)]
@@ -83,7 +98,7 @@ def dense_run(n, dtype, interp, bt, steps, warmup, deform=True):
         src, dst = bufs[s % 2], bufs[(s + 1) % 2]
         if ev is not None:
             ev[0].record()
-        D.advect_sl(src, vel, win, dts[s], interp, bt, out=dst)
+        D.advect_sl(src, vel, win, dts[s], interp, bt, out=dst, planes=planes)
         if ev is not None:
             ev[1].record()
         D.curvature(dst, win, out=kappa)
@@ -104,7 +119,7 @@ def dense_run(n, dtype, interp, bt, steps, warmup, deform=True):
     interior = (n - 4) ** 3
     B = b_alg(torch.tensor([], dtype=dtype).element_size())
     ups = interior * steps / wall
-    out = dict(n=n, steps=steps, wall_s=wall, updates_per_s=ups, ms_per_step=wall / steps * 1e3,
+    out = dict(n=n, velocity_layout="component planes" if planes else "per node", steps=steps, wall_s=wall, updates_per_s=ups, ms_per_step=wall / steps * 1e3,
                step_roofline_frac=ups * B["step"] / HBM_PEAK,
                sl_ms=sl_s * 1e3, curv_ms=cv_s * 1e3,
                sl_gbs=interior * B["sl"] / sl_s / 1e9, curv_gbs=interior * B["curv"] / cv_s / 1e9,
@@ -116,7 +131,8 @@ def dense_run(n, dtype, interp, bt, steps, warmup, deform=True):
 
 def probe(args):
     """Short programs for the counter runs (rocprofv3 --pmc), one per probe:
-    sl64: 5 launches of the headline SL kernel on the headline workload, then
+    sl64: 5 launches of the headline SL kernel on the headline workload
+    (velocity as component planes), then
     5 launches of a plain copy kernel (torch's elementwise neg over COPY_BYTES
     of FP32: reads and writes COPY_BYTES), whose counter reading per byte
     scales the SL kernel's; the others run the kernels named in PROBES."""
@@ -131,10 +147,10 @@ def probe(args):
     win = D.GridWindow.unit_box(n)
     dev = torch.device("cuda:0")
     phi = torch.from_numpy(configs.sphere_sdf(n)).to(dev, torch.float64)
-    vel = torch.from_numpy(configs.deformation_velocity(n)).to(dev, torch.float64)
+    vel = deformation_velocity_dev(n, torch.float64, True)
     out = torch.empty_like(phi)
     for s in range(5):
-        D.advect_sl(phi, vel, win, configs.deformation_dt(n, s), "cubic", "rk2", out=out)
+        D.advect_sl(phi, vel, win, configs.deformation_dt(n, s), "cubic", "rk2", out=out, planes=True)
     src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device=dev)
     dst = torch.empty_like(src)
     for _ in range(5):
@@ -837,26 +853,33 @@ def irregular_run(n=256, steps=20, width=6.0):
 def get_velocity_run(n=256, steps=20):
     """LevelSetGetVelocity from a device-resident fluid field: the (u, v, w)
     window of an n^3 level set out of an n^3 FP64 fluid array of (p, u, v, w)
-    records.  Kernel time from HIP events on the current stream; 48 B per node
-    (24 read, 24 written) against the HBM peak."""
+    records, per node and as component planes.  Kernel time from HIP events on
+    the current stream; 48 B per node (24 read, 24 written) against the HBM
+    peak."""
     import torch
     import dcell_amd as D
 
     fluid = torch.rand((n, n, n, 4), dtype=torch.float64, device="cuda:0")
     win = D.GridWindow((0, 0, 0), (n, n, n), (1.0 / (n - 1),) * 3)
-    out = torch.empty((n, n, n, 3), dtype=torch.float64, device="cuda:0")
-    D.get_velocity(fluid, win, out=out)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(steps):
-        D.get_velocity(fluid, win, out=out)
-    e1.record()
-    torch.cuda.synchronize()
-    t = e0.elapsed_time(e1) * 1e-3 / steps
-    gbs = n ** 3 * 48 / t / 1e9
-    return dict(n=n, ms_per_call=t * 1e3, nodes_per_s=n ** 3 / t, achieved_gbs=gbs,
-                roofline_frac=gbs * 1e9 / HBM_PEAK, bytes_per_node=48)
+    res = dict(n=n, bytes_per_node=48)
+    for planes in (False, True):
+        out = torch.empty((3, n, n, n) if planes else (n, n, n, 3), dtype=torch.float64, device="cuda:0")
+        D.get_velocity(fluid, win, out=out, planes=planes)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            D.get_velocity(fluid, win, out=out, planes=planes)
+        e1.record()
+        torch.cuda.synchronize()
+        t = e0.elapsed_time(e1) * 1e-3 / steps
+        gbs = n ** 3 * 48 / t / 1e9
+        r = dict(ms_per_call=t * 1e3, nodes_per_s=n ** 3 / t, achieved_gbs=gbs, roofline_frac=gbs * 1e9 / HBM_PEAK)
+        if planes:
+            res["planes"] = r
+        else:
+            res.update(r)
+    return res
 
 
 def band_mode_run(n=256, steps=30, warmup=3):
@@ -1140,6 +1163,7 @@ FORMS = {
     "precision_pls_384": lambda: pls_run(384, 150),
     "adjoint_128": lambda: adjoint_run(128, 50, 10),
     "fp64_reference_scheme_256": lambda: dense_run(256, _f64(), "linear", "euler", 50, 2),
+    "fp64_config_scheme_256_per_node_velocity": lambda: dense_run(256, _f64(), "cubic", "rk2", 50, 2),
     "fp32_config_scheme_384": lambda: dense_run(384, _f32(), "cubic", "rk2", 50, 2),
     "fp32_reference_scheme_384": lambda: dense_run(384, _f32(), "linear", "euler", 50, 2),
     "zalesak_2d_128_fp64_config_scheme": lambda: zalesak_run(128, 628, "cubic", "rk2"),
@@ -1183,7 +1207,7 @@ def main():
         print(json.dumps({args.form: FORMS[args.form]()}), flush=True)
         return
     import torch
-    r = dense_run(args.n, torch.float64, "cubic", "rk2", args.steps, args.warmup)
+    r = dense_run(args.n, torch.float64, "cubic", "rk2", args.steps, args.warmup, planes=True)
     interior = (args.n - 4) ** 3
     B = b_alg(8)
     achieved = interior * B["sl"] / (r["sl_ms"] * 1e-3)
@@ -1204,6 +1228,7 @@ def main():
         "config": {"workload": f"3D deformation, {args.n}^3 FP64 sphere r=0.15, RK2 backtrace + "
                                "Catmull-Rom tricubic SL + 19-point curvature, dt=h",
                    "grid": [args.n] * 3, "interior_updates_per_step": interior,
+                   "velocity_layout": "component planes [c][z][y][x] (dcl_get_velocity_planes)",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": kname or "dense SL (RK2 + tricubic, FP64)",
                      "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",

@@ -26,7 +26,7 @@ EXPORTS = (
     "dcl_sl_advect", "dcl_curvature", "dcl_levelset_step",
     "dcl_pls_advect", "dcl_pls_ws_create", "dcl_pls_ws_destroy", "dcl_pls_correct",
     "dcl_pls_adjust_radii", "dcl_max_velocity", "dcl_band_compact", "dcl_band_compact_static", "dcl_irregular_nodes",
-    "dcl_surface_quantities", "dcl_get_velocity", "dcl_reinitialize", "dcl_reinitialize_parallel",
+    "dcl_surface_quantities", "dcl_get_velocity", "dcl_get_velocity_planes", "dcl_reinitialize", "dcl_reinitialize_parallel",
     "dcl_real64_pack", "dcl_irregular_node_rows",
     "dcl_interface_metrics", "dcl_error_norms", "dcl_sl_adjoint",
     "dcl_sl_adjoint_record_bytes", "dcl_sl_adjoint_prepare", "dcl_sl_adjoint_step", "dcl_sl_adjoint_finish",
@@ -104,6 +104,7 @@ def _declare(L):
         "dcl_real64_pack": ([vp, i32, i64, vp, vp], i32),
         "dcl_irregular_node_rows": ([vp, vp, vp, i64, i32, vp, i64, C.POINTER(i64), vp], i32),
         "dcl_get_velocity": ([vp, C.POINTER(ICoor), W, i32, vp, vp], i32),
+        "dcl_get_velocity_planes": ([vp, C.POINTER(ICoor), W, i32, vp, vp], i32),
         "dcl_reinitialize": ([C.POINTER(ReinitCell), i32, i64, vp], i32),
         "dcl_reinitialize_parallel": ([C.POINTER(ReinitCell), i32, i32, C.POINTER(C.c_int), vp], i32),
         "dcl_interface_metrics": ([vp, vp, W, dbl, i32, C.POINTER(dbl), vp], i32),

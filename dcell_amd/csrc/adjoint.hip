@@ -756,9 +756,7 @@ static int adj_params(const dcl_window* win, const dcl_velocity* vel, double dt,
       vw.n.y != win->n.y || vw.n.z != win->n.z)
     return set_error(DCL_ERR_SUP, "the adjoint needs the velocity window to equal phi's");
   P.w = kwin(*win);
-  P.v.w = kwin(vw);
-  P.v.dof = vel->dof;
-  P.v.udof = vel->udof;
+  P.v = kvel(*vel);
   const double d[3] = {win->d.x, win->d.y, win->d.z};
   for (int a = 0; a < 3; ++a) {
     P.c[a] = dt / d[a];
@@ -889,9 +887,7 @@ extern "C" int dcl_sl_adjoint(const void* phi0, const dcl_window* win, const dcl
   if (lam == lam_next || lam == phi0) return set_error(DCL_ERR_ARG_WRONG, "lam must not alias inputs");
   AdjP P;
   P.w = kwin(*win);
-  P.v.w = kwin(vw);
-  P.v.dof = vel->dof;
-  P.v.udof = vel->udof;
+  P.v = kvel(*vel);
   const double d[3] = {win->d.x, win->d.y, win->d.z};
   for (int a = 0; a < 3; ++a) {
     P.c[a] = dt / d[a];

@@ -122,6 +122,7 @@ __device__ __forceinline__ SLP cell_slp(const BCell& c) {
   P.v.w = P.w;
   P.v.dof = DIM;
   P.v.udof = 0;
+  P.v.cs = 1;
   for (int a = 0; a < 3; ++a) {
     P.c[a] = c.c[a];
     P.ch[a] = c.ch[a];

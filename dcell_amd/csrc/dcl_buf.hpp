@@ -70,8 +70,10 @@ struct BL<double> {
 };
 
 // node velocity (see node_vel): the two faces around the node per component.
-// With 3 FP32 components per node, (u, v, w) of the node and u of its +x
-// neighbour are one 16-byte load.
+// With 3 components per node, (u, v, w) of the node and u of its +x
+// neighbour are one 16-byte load in FP32, two in FP64 (the texture path's
+// cost is per load instruction, not per byte); with component planes, u of
+// the node and of its +x neighbour are one load.
 template <typename T>
 __device__ __forceinline__ void node_vel_buf(rsrc_t rv, const KVel& v, int X, int Y, int Z, T V[3]) {
   const int x = clampi(X, v.w.px, v.w.px + v.w.nx - 2) - v.w.px;
@@ -80,7 +82,7 @@ __device__ __forceinline__ void node_vel_buf(rsrc_t rv, const KVel& v, int X, in
   const int sx = v.dof, sy = v.w.nx * v.dof, sz = v.w.ny * sy;
   const int e = (int)sizeof(T);
   const int vo = (((z * v.w.ny + y) * v.w.nx + x) * v.dof + v.udof) * e;
-  if (sizeof(T) == 4 && v.dof == 3) {
+  if (v.dof == 3) {
     T q[4];
     BL<T>::four(rv, vo, 0, q);
     V[0] = vmid(q[0], q[3]);
@@ -88,27 +90,46 @@ __device__ __forceinline__ void node_vel_buf(rsrc_t rv, const KVel& v, int X, in
     V[2] = vmid(q[2], BL<T>::one(rv, vo + 2 * e, sz * e));
     return;
   }
-  V[0] = vmid(BL<T>::one(rv, vo, 0), BL<T>::one(rv, vo, sx * e));
-  V[1] = vmid(BL<T>::one(rv, vo + e, 0), BL<T>::one(rv, vo + e, sy * e));
-  V[2] = vmid(BL<T>::one(rv, vo + 2 * e, 0), BL<T>::one(rv, vo + 2 * e, sz * e));
+  const int ce = v.cs * e;  // bytes between components
+  if (v.dof == 1) {
+    T p[2];
+    BL<T>::two(rv, vo, 0, p);
+    V[0] = vmid(p[0], p[1]);
+  } else {
+    V[0] = vmid(BL<T>::one(rv, vo, 0), BL<T>::one(rv, vo, sx * e));
+  }
+  V[1] = vmid(BL<T>::one(rv, vo + ce, 0), BL<T>::one(rv, vo + ce, sy * e));
+  V[2] = vmid(BL<T>::one(rv, vo + 2 * ce, 0), BL<T>::one(rv, vo + 2 * ce, sz * e));
 }
 
 // MAC trilinear velocity of the split (b0, b1, f0, f1) (see mac_vel).  With 3
 // FP32 components per node, each x pair of faces of one component is the
-// first and last value of one 16-byte load.
+// first and last value of one 16-byte load; with component planes it is two
+// neighbouring values, one load (16 bytes in FP64).
 template <typename T>
 __device__ __forceinline__ void mac_vel_buf_split(rsrc_t rv, const KVel& v, const int b0[3], const int b1[3],
                                                   const T f0[3], const T f1[3], T V[3]) {
   const int sx = v.dof, sy = v.w.nx * v.dof, sz = v.w.ny * sy;
   const int e = (int)sizeof(T);
   const bool quad = sizeof(T) == 4 && v.dof == 3;
+  const bool planes = v.dof == 1;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const int ix = c == 0 ? b1[0] : b0[0], iy = c == 1 ? b1[1] : b0[1], iz = c == 2 ? b1[2] : b0[2];
     const T x1 = c == 0 ? f1[0] : f0[0], y1 = c == 1 ? f1[1] : f0[1], z1 = c == 2 ? f1[2] : f0[2];
-    const int vo = (iz * sz + iy * sy + ix * sx + v.udof + c) * e;
+    const int vo = (iz * sz + iy * sy + ix * sx + v.udof + c * v.cs) * e;
     T a000, a100, a010, a110, a001, a101, a011, a111;
-    if (quad) {
+    if (planes) {
+      T p[2];
+      BL<T>::two(rv, vo, 0, p);
+      a000 = p[0], a100 = p[1];
+      BL<T>::two(rv, vo, sy * e, p);
+      a010 = p[0], a110 = p[1];
+      BL<T>::two(rv, vo, sz * e, p);
+      a001 = p[0], a101 = p[1];
+      BL<T>::two(rv, vo, (sz + sy) * e, p);
+      a011 = p[0], a111 = p[1];
+    } else if (quad) {
       T q[4];
       BL<T>::four(rv, vo, 0, q);
       a000 = q[0], a100 = q[3];

@@ -19,15 +19,27 @@ struct KWin {
   int nx, ny, nz;
 };
 
+// velocity as the kernels see it: component c of node (x, y, z) (local) is
+// element ((z ny + y) nx + x) dof + udof + c cs; cs = 1 for components
+// stored per node, the window's node count for component planes (dof 1)
 struct KVel {
   KWin w;
-  int dof, udof;
+  int dof, udof, cs;
 };
 
 __host__ inline KWin kwin(const dcl_window& w) {
   KWin k;
   k.px = w.p.x; k.py = w.p.y; k.pz = w.p.z;
   k.nx = w.n.x; k.ny = w.n.y; k.nz = w.n.z < 1 ? 1 : w.n.z;
+  return k;
+}
+
+__host__ inline KVel kvel(const dcl_velocity& v) {
+  KVel k;
+  k.w = kwin(v.win);
+  k.dof = v.dof;
+  k.udof = v.udof;
+  k.cs = v.dof == 1 ? k.w.nx * k.w.ny * k.w.nz : 1;
   return k;
 }
 
@@ -146,12 +158,12 @@ __device__ __forceinline__ void node_vel(const T* __restrict__ vel, const KVel& 
     const int sz = v.w.ny * sy;
     const T* b = vel + (z * v.w.ny + y) * sy + x * sx + v.udof;
     V[0] = vmid(b[0], b[sx]);
-    V[1] = vmid(b[1], b[sy + 1]);
-    V[2] = vmid(b[2], b[sz + 2]);
+    V[1] = vmid(b[v.cs], b[sy + v.cs]);
+    V[2] = vmid(b[2 * v.cs], b[sz + 2 * v.cs]);
   } else {
     const T* b = vel + y * sy + x * sx + v.udof;
     V[0] = vmid(b[0], b[sx]);
-    V[1] = vmid(b[1], b[sy + 1]);
+    V[1] = vmid(b[v.cs], b[sy + v.cs]);
   }
 }
 
@@ -192,10 +204,10 @@ __device__ __forceinline__ void mac_vel(const T* __restrict__ vel, const KVel& v
     if (DIM == 3) {
       const int iz = c == 2 ? b1[2] : b0[2];
       const T z1 = c == 2 ? f1[2] : f0[2];
-      const T* b = vel + iz * sz + iy * sy + ix * sx + v.udof + c;
+      const T* b = vel + iz * sz + iy * sy + ix * sx + v.udof + c * v.cs;
       V[c] = vtri(b[0], b[sx], b[sy], b[sy + sx], b[sz], b[sz + sx], b[sz + sy], b[sz + sy + sx], x1, y1, z1);
     } else {
-      const T* b = vel + iy * sy + ix * sx + v.udof + c;
+      const T* b = vel + iy * sy + ix * sx + v.udof + c * v.cs;
       V[c] = vlerp(vlerp(b[0], b[sx], x1), vlerp(b[sy], b[sy + sx], x1), y1);
     }
   }

@@ -24,7 +24,8 @@ inline int64_t win_nodes(const dcl_window& w) {
 }
 
 int check_window(const dcl_window* w, const char* name, int min_n, int dof = 1);
-int check_velocity(const dcl_velocity* v, int dim);
+// planes: the entry point also takes component planes (dof 1)
+int check_velocity(const dcl_velocity* v, int dim, bool planes = false);
 int check_precision(int precision);
 
 inline unsigned grid1d(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }

@@ -198,8 +198,10 @@ __device__ __forceinline__ void vring_fetch(rsrc_t rv, const int xo[V::PF], int 
         pf[q][1] = (T)x.y;
         pf[q][2] = (T)x.z;
       } else {
-        pf[q][0] = BL<T>::one(rv, xo[q], zo);
-        pf[q][1] = BL<T>::one(rv, xo[q] + (int)sizeof(T), zo);
+        T a[2];
+        BL<T>::two(rv, xo[q], zo, a);
+        pf[q][0] = a[0];
+        pf[q][1] = a[1];
         pf[q][2] = BL<T>::one(rv, xo[q] + 2 * (int)sizeof(T), zo);
       }
     }

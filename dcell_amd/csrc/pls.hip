@@ -235,10 +235,7 @@ extern "C" int dcl_pls_advect(void* particles, int64_t count, const dcl_velocity
   if (!(dh.x > 0 && dh.y > 0 && (dim == 2 || dh.z > 0)))
     return set_error(DCL_ERR_ARG_WRONG, "dh must be positive");
   if (count == 0) return 0;
-  KVel v;
-  v.w = kwin(vel->win);
-  v.dof = vel->dof;
-  v.udof = vel->udof;
+  const KVel v = kvel(*vel);
   hipStream_t s = as_stream(stream);
   const unsigned g = grid1d(count, 256);
   const long long vb = win_nodes(vel->win) * vel->dof * (long long)(precision == DCL_F64 ? 8 : 4);

@@ -215,10 +215,7 @@ extern "C" int dcl_max_velocity(const dcl_velocity* vel, const void* nodes, int6
                      (long long)stride_bytes);
   *max_out = 0.0;
   if (count == 0) return 0;
-  KVel v;
-  v.w = kwin(vel->win);
-  v.dof = vel->dof;
-  v.udof = vel->udof;
+  const KVel v = kvel(*vel);
   const int nb = (int)(grid1d(count, RB) < (unsigned)RGRID ? grid1d(count, RB) : RGRID);
   void* sc = nullptr;
   DCL_TRY(scratch(sizeof(double) * (nb + 1), &sc));

@@ -36,12 +36,17 @@ int check_window(const dcl_window* w, const char* name, int min_n, int dof) {
   return 0;
 }
 
-int check_velocity(const dcl_velocity* v, int dim) {
+int check_velocity(const dcl_velocity* v, int dim, bool planes) {
   if (!v) return set_error(DCL_ERR_ARG_NULL, "velocity is NULL");
   if (!v->data) return set_error(DCL_ERR_ARG_NULL, "velocity data is NULL");
   if ((v->win.n.z < 2 ? 2 : 3) != dim)
     return set_error(DCL_ERR_ARG_WRONG, "velocity window is %dD, phi is %dD",
                      v->win.n.z < 2 ? 2 : 3, dim);
+  if (v->dof == 1) {
+    if (!planes) return set_error(DCL_ERR_SUP, "this entry point takes velocity components stored per node");
+    if (v->udof != 0) return set_error(DCL_ERR_ARG_WRONG, "component planes (dof 1) take udof 0, not %d", v->udof);
+    return check_window(&v->win, "velocity", 2, dim);
+  }
   if (v->udof < 0 || v->udof + dim > v->dof)
     return set_error(DCL_ERR_ARG_WRONG, "velocity dof %d cannot hold %d components from udof %d",
                      v->dof, dim, v->udof);

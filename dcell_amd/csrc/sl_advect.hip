@@ -76,17 +76,19 @@ __global__ __launch_bounds__(256) void sl_band(const T* __restrict__ phi0, T* __
 
 // Dense 3D windows: phi0 staged in an LDS ring (sl_ring3d) or gathered from
 // global memory per node (sl_dense).  Measured on 256^3 / 384^3 windows, the
-// ring is faster for the 64-value tricubic stencil and for the RK2 midpoint,
-// the gather for trilinear Euler.  DCL_SL_KERNEL=ring|gather overrides the
+// ring is faster for the 64-value tricubic stencil and, with the velocity
+// per node, for the trilinear RK2 midpoint; the gather for trilinear Euler,
+// and for trilinear RK2 with component planes (FP64 256^3: 0.25 ms against
+// the ring's 0.29).  DCL_SL_KERNEL=ring|gather overrides the
 // choice (read at every call); DCL_SL_KERNEL=plain selects the pointer-based
 // gather that windows of 4 GiB and more take.
 static const char* sl_kernel_env() { return getenv("DCL_SL_KERNEL"); }
 
-static bool sl_ring_selected(int interp, int bt) {
+static bool sl_ring_selected(int interp, int bt, bool planes) {
   const char* e = sl_kernel_env();
   if (e && (strcmp(e, "gather") == 0 || strcmp(e, "plain") == 0)) return false;
   if (e && strcmp(e, "ring") == 0) return true;
-  return interp == DCL_CUBIC || bt == DCL_RK2;
+  return interp == DCL_CUBIC || (bt == DCL_RK2 && !planes);
 }
 
 template <typename T, int DIM, int INTERP, int BT, bool BUF>
@@ -95,7 +97,7 @@ static int launch_sl_b(const void* phi0, void* phi, const void* vel, const SLP& 
                        unsigned vb, hipStream_t s) {
   const T* p0 = static_cast<const T*>(phi0);
   const T* v = static_cast<const T*>(vel);
-  if (!band && BUF && DIM == 3 && sl_ring_selected(INTERP, BT)) {
+  if (!band && BUF && DIM == 3 && sl_ring_selected(INTERP, BT, P.v.dof == 1)) {
     // 32 x 8 columns (DCL_RING_TILE=0|1|2 selects 16 x 16, 32 x 8, 64 x 4);
     // velocity staged too where it can be (same window, 3 components) and
     // measured faster: FP32, or FP64 with the trilinear stencil (the FP64
@@ -154,7 +156,8 @@ static int launch_sl(const void* phi0, void* phi, const void* vel, const SLP& P,
                      const dcl_icoor* band, int64_t band_len, void* band_values,
                      hipStream_t s) {
   const long long pb = (long long)P.w.nx * P.w.ny * P.w.nz * (long long)sizeof(T);
-  const long long vb = (long long)P.v.w.nx * P.v.w.ny * P.v.w.nz * P.v.dof * (long long)sizeof(T);
+  const long long vb =
+      (long long)P.v.w.nx * P.v.w.ny * P.v.w.nz * (P.v.dof == 1 ? DIM : P.v.dof) * (long long)sizeof(T);
   const char* e = sl_kernel_env();
   if (DIM == 3 && buf_ok(pb, vb) && !(e && strcmp(e, "plain") == 0))
     return launch_sl_b<T, DIM, INTERP, BT, true>(phi0, phi, vel, P, band, band_len, band_values,
@@ -181,9 +184,7 @@ static int dispatch_sl(int dim, int interp, int bt, const void* phi0, void* phi,
 
 int sl_params(const dcl_window* win, const dcl_velocity* vel, double dt, SLP& P) {
   P.w = kwin(*win);
-  P.v.w = kwin(vel->win);
-  P.v.dof = vel->dof;
-  P.v.udof = vel->udof;
+  P.v = kvel(*vel);
   const double d[3] = {win->d.x, win->d.y, win->d.z};
   for (int a = 0; a < 3; ++a) {
     P.c[a] = d[a] > 0 ? dt / d[a] : 0.0;
@@ -213,7 +214,7 @@ int check_sl_args(const void* phi0, void* phi, const dcl_window* win, const dcl_
   const int min_n = band ? 2 : 5;
   DCL_TRY(check_window(win, "phi", min_n));
   const int dim = win->n.z < 2 ? 2 : 3;
-  DCL_TRY(check_velocity(vel, dim));
+  DCL_TRY(check_velocity(vel, dim, true));
   if (interp != DCL_LINEAR && interp != DCL_CUBIC)
     return set_error(DCL_ERR_ARG_WRONG, "interp %d is neither DCL_LINEAR nor DCL_CUBIC", interp);
   if (backtrace != DCL_EULER && backtrace != DCL_RK2)

@@ -6,7 +6,9 @@
 // Global Arrays get; here one thread per node of that box copies the
 // velocity components (U_FACE..W_FACE).  An HBM-bound strided copy: each
 // 3D node reads one 32-byte (16-byte in FP32) fluid record, of which the
-// velocity is 24 (12) bytes, and writes 24 (12) bytes.
+// velocity is 24 (12) bytes, and writes 24 (12) bytes, per node
+// (u, v, w) or, for dcl_get_velocity_planes, one plane per component (the
+// layout the FP64 SL kernels gather fastest from).
 #include "dcl_host.hpp"
 
 namespace dcl {
@@ -20,7 +22,7 @@ struct GvBox {
   KWin w;          // the velocity window
 };
 
-template <typename T, int DIM>
+template <typename T, int DIM, bool PL>
 __global__ __launch_bounds__(GVB) void get_velocity_k(const T* __restrict__ fluid, GvBox g, T* __restrict__ vel) {
   const int64_t t = (int64_t)blockIdx.x * GVB + threadIdx.x;
   const int64_t plane = (int64_t)g.bx * g.by;
@@ -29,28 +31,31 @@ __global__ __launch_bounds__(GVB) void get_velocity_k(const T* __restrict__ flui
   const int y = r / g.bx, x = r - y * g.bx;
   const int X = g.x0 + x, Y = g.y0 + y, Z = g.z0 + z;
   const size_t src = (((size_t)Z * g.fy + Y) * g.fx + X) * (DIM + 1);
-  const size_t dst = (((size_t)(Z - g.w.pz) * g.w.ny + (Y - g.w.py)) * g.w.nx + (X - g.w.px)) * DIM;
+  const size_t node = ((size_t)(Z - g.w.pz) * g.w.ny + (Y - g.w.py)) * g.w.nx + (X - g.w.px);
+  // component c of the node at dst + c * cs
+  const size_t dst = PL ? node : node * DIM;
+  const size_t cs = PL ? (size_t)g.w.nx * g.w.ny * g.w.nz : 1;
   if (DIM == 3 && sizeof(T) == 8) {
     // a node's record is 32 bytes, 32-byte aligned: two 16-byte loads
     typedef double d2 __attribute__((ext_vector_type(2)));
     const d2 a = *reinterpret_cast<const d2*>(fluid + src);
     const d2 b = *reinterpret_cast<const d2*>(fluid + src + 2);
     vel[dst] = (T)a.y;
-    vel[dst + 1] = (T)b.x;
-    vel[dst + 2] = (T)b.y;
+    vel[dst + cs] = (T)b.x;
+    vel[dst + 2 * cs] = (T)b.y;
   } else if (DIM == 3) {
     typedef float f4 __attribute__((ext_vector_type(4)));
     const f4 a = *reinterpret_cast<const f4*>(fluid + src);
     vel[dst] = (T)a.y;
-    vel[dst + 1] = (T)a.z;
-    vel[dst + 2] = (T)a.w;
+    vel[dst + cs] = (T)a.z;
+    vel[dst + 2 * cs] = (T)a.w;
   } else {
     vel[dst] = fluid[src + 1];
-    vel[dst + 1] = fluid[src + 2];
+    vel[dst + cs] = fluid[src + 2];
   }
 }
 
-template <typename T>
+template <typename T, bool PL>
 static int get_velocity_t(const void* fluid, const dcl_icoor& fn, const dcl_window* win, void* vel, hipStream_t s) {
   const bool d3 = fn.z >= 2;
   const int nz = d3 ? win->n.z : 1;
@@ -69,13 +74,16 @@ static int get_velocity_t(const void* fluid, const dcl_icoor& fn, const dcl_wind
   g.fx = fn.x;
   g.fy = fn.y;
   g.w = kwin(*win);
-  if (!d3) g.w.pz = 0;
+  if (!d3) {
+    g.w.pz = 0;
+    g.w.nz = 1;
+  }
   const int64_t n = (int64_t)g.bx * g.by * g.bz;
   if (d3)
-    hipLaunchKernelGGL((get_velocity_k<T, 3>), dim3(grid1d(n, GVB)), dim3(GVB), 0, s, static_cast<const T*>(fluid),
+    hipLaunchKernelGGL((get_velocity_k<T, 3, PL>), dim3(grid1d(n, GVB)), dim3(GVB), 0, s, static_cast<const T*>(fluid),
                        g, static_cast<T*>(vel));
   else
-    hipLaunchKernelGGL((get_velocity_k<T, 2>), dim3(grid1d(n, GVB)), dim3(GVB), 0, s, static_cast<const T*>(fluid),
+    hipLaunchKernelGGL((get_velocity_k<T, 2, PL>), dim3(grid1d(n, GVB)), dim3(GVB), 0, s, static_cast<const T*>(fluid),
                        g, static_cast<T*>(vel));
   DCL_LAUNCHED("get_velocity");
   return 0;
@@ -85,8 +93,8 @@ static int get_velocity_t(const void* fluid, const dcl_icoor& fn, const dcl_wind
 
 using namespace dcl;
 
-extern "C" int dcl_get_velocity(const void* fluid, const dcl_icoor* fluid_n, const dcl_window* win, int precision,
-                                void* vel, void* stream) {
+static int get_velocity(const void* fluid, const dcl_icoor* fluid_n, const dcl_window* win, int precision, void* vel,
+                        void* stream, bool planes) {
   DCL_TRY(check_precision(precision));
   if (!fluid || !vel) return set_error(DCL_ERR_ARG_NULL, "fluid or vel is NULL");
   if (!fluid_n) return set_error(DCL_ERR_ARG_NULL, "fluid_n is NULL");
@@ -104,6 +112,19 @@ extern "C" int dcl_get_velocity(const void* fluid, const dcl_icoor* fluid_n, con
     return set_error(DCL_ERR_ARG_WRONG, "fluid must be aligned to one node record (%d bytes)",
                      precision == DCL_F64 ? 32 : 16);
   hipStream_t s = as_stream(stream);
-  if (precision == DCL_F64) return get_velocity_t<double>(fluid, *fluid_n, win, vel, s);
-  return get_velocity_t<float>(fluid, *fluid_n, win, vel, s);
+  if (precision == DCL_F64)
+    return planes ? get_velocity_t<double, true>(fluid, *fluid_n, win, vel, s)
+                  : get_velocity_t<double, false>(fluid, *fluid_n, win, vel, s);
+  return planes ? get_velocity_t<float, true>(fluid, *fluid_n, win, vel, s)
+                : get_velocity_t<float, false>(fluid, *fluid_n, win, vel, s);
+}
+
+extern "C" int dcl_get_velocity(const void* fluid, const dcl_icoor* fluid_n, const dcl_window* win, int precision,
+                                void* vel, void* stream) {
+  return get_velocity(fluid, fluid_n, win, precision, vel, stream, false);
+}
+
+extern "C" int dcl_get_velocity_planes(const void* fluid, const dcl_icoor* fluid_n, const dcl_window* win,
+                                       int precision, void* vel, void* stream) {
+  return get_velocity(fluid, fluid_n, win, precision, vel, stream, true);
 }

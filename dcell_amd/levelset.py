@@ -83,8 +83,17 @@ def _p(t):
     return None if t is None else C.c_void_p(t.data_ptr())
 
 
-def velocity(vel: torch.Tensor, win: GridWindow, udof: int = 0) -> L.Velocity:
+def velocity(vel: torch.Tensor, win: GridWindow, udof: int = 0, planes: bool = False) -> L.Velocity:
+    """The C ABI's view of a velocity tensor: components per node, shaped
+    (..., dof), or with ``planes`` one plane per component, shaped
+    (dim, nz, ny, nx) (2D: (2, ny, nx)), passed as dof 1."""
     _dev(vel, "vel")
+    if planes:
+        dim = 3 if win.n[2] >= 2 else 2
+        if udof != 0 or vel.shape[0] != dim or vel.numel() != win.nodes * dim:
+            raise ValueError(f"component planes must be shaped ({dim}, window) with udof 0, "
+                             f"got {tuple(vel.shape)}")
+        return L.Velocity(C.c_void_p(vel.data_ptr()), win.c(), 1, 0)
     dof = vel.shape[-1]
     if vel.numel() != win.nodes * dof:
         raise ValueError(f"vel has {vel.numel()} elements, window {win.n} x dof {dof} needs "
@@ -103,19 +112,21 @@ def _band(band):
 
 def advect_sl(phi0, vel, win: GridWindow, dt: float, interp="linear", backtrace="euler",
               band=None, vel_win: GridWindow | None = None, udof=0, out=None,
-              band_values=False):
+              band_values=False, planes=False):
     """One SL step (LevelSetAdvectSL, LevelSetMethod/AdvectSL.c:6-89).
 
     Dense (band None): returns the new phi window.  Band: writes the band
     nodes of ``out`` (a copy of phi0 if not given) and returns it, or with
-    ``band_values=True`` returns just the band values."""
+    ``band_values=True`` returns just the band values.  ``planes``: vel holds
+    one plane per component, shaped (dim, nz, ny, nx), instead of the
+    components per node (..., dof)."""
     _dev(phi0, "phi0")
     prec = precision_of(phi0)
     if vel.dtype != phi0.dtype:
         raise TypeError("phi0 and vel must share a dtype")
     if phi0.numel() != win.nodes:
         raise ValueError(f"phi0 has {phi0.numel()} elements, window needs {win.nodes}")
-    V = velocity(vel, vel_win or win, udof)
+    V = velocity(vel, vel_win or win, udof, planes)
     b, n = _band(band)
     if b is not None and n == 0:      # an empty tensor has a NULL data pointer: NULL means dense
         return phi0.new_empty(0) if band_values else (phi0.clone() if out is None else out)
@@ -155,11 +166,14 @@ def curvature(phi, win: GridWindow, stencil="std", band=None, out=None, band_val
 
 
 def levelset_step(phi0, vel, win: GridWindow, dt: float, interp="linear", backtrace="euler",
-                  out_phi=None, out_kappa=None, udof=0):
-    """One dense update: SL step + curvature of the result.  Returns (phi, kappa)."""
+                  out_phi=None, out_kappa=None, udof=0, planes=False):
+    """One dense update: SL step + curvature of the result.  Returns (phi,
+    kappa).  ``planes`` as in advect_sl."""
     _dev(phi0, "phi0")
     prec = precision_of(phi0)
-    V = velocity(vel, win, udof)
+    if vel.dtype != phi0.dtype:
+        raise TypeError("phi0 and vel must share a dtype")
+    V = velocity(vel, win, udof, planes)
     out_phi = torch.empty_like(phi0) if out_phi is None else out_phi
     out_kappa = torch.empty_like(phi0) if out_kappa is None else out_kappa
     L.call("dcl_levelset_step", _p(phi0), _p(out_phi), _p(out_kappa), C.byref(win.c()),
@@ -167,25 +181,27 @@ def levelset_step(phi0, vel, win: GridWindow, dt: float, interp="linear", backtr
     return out_phi, out_kappa
 
 
-def get_velocity(fluid, win: GridWindow, out=None):
+def get_velocity(fluid, win: GridWindow, out=None, planes=False):
     """LevelSetGetVelocity (LevelSetMethod/Advect.c:145-192) from a device
     fluid field: `fluid` is the fluid's (Nz, Ny, Nx, 4) array of (p, u, v, w)
     (2D: (Ny, Nx, 3) of (p, u, v)) over its domain with node (0, 0, 0) at the
     global origin.  Returns the velocity window (win's shape + (3,) or (2,)):
     (u, v, w) at the nodes of win inside the domain; the other nodes of `out`
-    keep their values (zeros for a new tensor)."""
+    keep their values (zeros for a new tensor).  ``planes``: the window as
+    component planes, shaped (3,) + win's shape ((2,) + in 2D), the layout
+    advect_sl and levelset_step take with planes=True."""
     _dev(fluid, "fluid")
     d3 = fluid.dim() == 4
     if fluid.shape[-1] != (4 if d3 else 3):
         raise ValueError("fluid must be (Nz, Ny, Nx, 4) or (Ny, Nx, 3)")
     dim = 3 if d3 else 2
-    shape = win.shape + (dim,)
+    shape = (dim,) + win.shape if planes else win.shape + (dim,)
     if out is None:
         out = torch.zeros(shape, dtype=fluid.dtype, device=fluid.device)
     _dev(out, "out", fluid.dtype)
     if tuple(out.shape) != shape:
         raise ValueError(f"out has shape {tuple(out.shape)}, expected {shape}")
     fn = L.ICoor(fluid.shape[-2], fluid.shape[-3], fluid.shape[0] if d3 else 1)
-    L.call("dcl_get_velocity", _p(fluid), C.byref(fn), C.byref(win.c()), precision_of(fluid), _p(out),
+    L.call("dcl_get_velocity_planes" if planes else "dcl_get_velocity", _p(fluid), C.byref(fn), C.byref(win.c()), precision_of(fluid), _p(out),
            stream_handle())
     return out

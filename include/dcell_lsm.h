@@ -72,11 +72,16 @@ typedef struct { double x, y, z, radius; } dcl_particle;
 /* the DCL_F32 form of a particle; 16 bytes */
 typedef struct { float x, y, z, radius; } dcl_particle_f32;
 
-/* a velocity grid: the reference's velgrid (LevelSetMethod/Advect.c:12-22) */
+/* a velocity grid: the reference's velgrid (LevelSetMethod/Advect.c:12-22).
+ * dof 1 stores the components in planes instead, [c][z][y][x] (u, v, w, each
+ * plane the window's node count), the layout of a solver that keeps its MAC
+ * components apart; dcl_sl_advect and dcl_levelset_step take it (an x pair
+ * of one component's faces is then one load in the gathers), the other entry
+ * points return DCL_ERR_SUP for it. */
 typedef struct {
-  const void *data; /* device, [z][y][x][dof] of the precision in use */
+  const void *data; /* device, [z][y][x][dof] (or [c][z][y][x] for dof 1) of the precision in use */
   dcl_window win;
-  int32_t dof;      /* components stored per node: 2 (2D), 3 (3D), 4 fluid [p,u,v,w] */
+  int32_t dof;      /* components stored per node: 2 (2D), 3 (3D), 4 fluid [p,u,v,w]; 1: planes */
   int32_t udof;     /* index of u within a node: 0 for a velgrid, 1 for the fluid (U_FACE) */
 } dcl_velocity;
 
@@ -305,6 +310,14 @@ int dcl_reinitialize_parallel(dcl_reinit_cell *cells, int ncells, int max_sweeps
  * Advect.c:160-186).  Asynchronous on stream. */
 int dcl_get_velocity(const void *fluid, const dcl_icoor *fluid_n, const dcl_window *win, int precision,
                      void *vel, void *stream);
+
+/* dcl_get_velocity writing vel as component planes, the dof-1 layout of
+ * dcl_velocity ([c][z][y][x], 2D [c][y][x], each plane over win), which
+ * dcl_sl_advect and dcl_levelset_step take; same copy, same clipping.  The
+ * reference's velocity grid is per node (dof 3); this layout is the FP64 SL
+ * kernels' fastest (one load per pair of x-neighbouring values). */
+int dcl_get_velocity_planes(const void *fluid, const dcl_icoor *fluid_n, const dcl_window *win, int precision,
+                            void *vel, void *stream);
 
 /* The surface quantities of one irregular-node record: the fields that
  * IIMUpdateSurfaceQuantities_3D / _2D (IIM/SurfaceQuantities.c:31-118) set

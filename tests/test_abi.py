@@ -53,6 +53,17 @@ def test_abi_version_and_argument_errors():
     assert rc == 62  # unknown precision
     rc = lib.dcl_curvature(dummy, C.c_void_p(32), C.byref(w), None, 0, None, 0, 1, None)
     assert rc == 56  # LSS stencil is 2D only
+    # component planes (dof 1): the SL entry points take them (these calls fail
+    # later, on the window size), the others answer DCL_ERR_SUP
+    w5 = L.Window(L.ICoor(0, 0, 0), L.ICoor(5, 5, 5), L.Coor(1, 1, 1))
+    P = L.Velocity(dummy, w5, 1, 0)
+    rc = lib.dcl_sl_advect(dummy, C.c_void_p(32), C.byref(w), C.byref(P), None, 0, None, 0.1, 0, 0, 0, None)
+    assert rc == 60, lib.dcl_last_error()
+    rc = lib.dcl_max_velocity(C.byref(P), dummy, 1, 12, 3, 0, C.byref(C.c_double()), None)
+    assert rc == 56 and b"per node" in lib.dcl_last_error()
+    rc = lib.dcl_sl_advect(dummy, C.c_void_p(32), C.byref(w5), C.byref(L.Velocity(dummy, w5, 1, 1)), None, 0,
+                           None, 0.1, 0, 0, 0, None)
+    assert rc == 62 and b"udof 0" in lib.dcl_last_error()
 
 
 def test_missing_library_fails_loudly(monkeypatch):

@@ -115,6 +115,40 @@ def test_sl_ring_equals_gather(dev, monkeypatch, dtype, interp, shape, amp, tile
     assert rel_max(outs["ring"], ref, np.max(np.abs(phi0))) <= TOL[dtype] * 4
 
 
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("interp", ["linear", "cubic"])
+@pytest.mark.parametrize("bt", ["euler", "rk2"])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_sl_component_planes_equal_per_node(dev, dtype, interp, bt, dim, sl_kernel):
+    """Velocity as component planes (dof 1, [c][z][y][x]) gives the values of
+    the same velocity stored per node, bit for bit (same arithmetic, other
+    loads), dense and in band mode with a velocity window offset from phi's;
+    and both equal the oracle."""
+    n = 24 if dim == 3 else 64
+    win, phi0, _, rng = _case(dim, n, seed=300 + dim)
+    m = n + 6
+    vwin = D.GridWindow(tuple(c - 3 for c in win.p), (m, m, m) if dim == 3 else (m, m, 1), win.d)
+    vel = rounded(random_velocity(m, dim, rng, 1.2), dtype)
+    phi0 = rounded(phi0, dtype)
+    planes = np.ascontiguousarray(np.moveaxis(vel, -1, 0))
+    dt = 1.3 / (n - 1)
+    p_dev, v_dev, pl_dev = to_dev(phi0, dtype), to_dev(vel, dtype), to_dev(planes, dtype)
+    a = to_np(D.advect_sl(p_dev, v_dev, win, dt, interp, bt, vel_win=vwin))
+    b = to_np(D.advect_sl(p_dev, pl_dev, win, dt, interp, bt, vel_win=vwin, planes=True))
+    assert np.array_equal(a, b)
+    ref = O.sl_advect(phi0, vel, owin(win), dt, IM[interp], BM[bt], vwin=owin(vwin))
+    assert rel_max(b, ref, np.max(np.abs(phi0))) <= TOL[dtype]
+    nodes = np.argwhere(np.abs(phi0) < 4.0)[:, ::-1]
+    band = np.zeros((len(nodes), 3), dtype=np.int32)
+    band[:, :dim] = nodes + np.array(win.p[:dim])
+    band[:, 2] = band[:, 2] if dim == 3 else win.p[2]
+    bd = to_dev(band, torch.int32)
+    va = to_np(D.advect_sl(p_dev, v_dev, win, dt, interp, bt, band=bd, vel_win=vwin, band_values=True))
+    vb = to_np(D.advect_sl(p_dev, pl_dev, win, dt, interp, bt, band=bd, vel_win=vwin, band_values=True,
+                           planes=True))
+    assert np.array_equal(va, vb)
+
+
 def test_sl_fluid_layout_udof(dev):
     """The fluid's [p, u, v, w] velocity (udof = 1, FluidField.c:146-151)."""
     win, phi0, vel, rng = _case(3, 20, seed=5)
@@ -188,6 +222,26 @@ def test_levelset_step_deformation_c2_shape(dev, sl_kernel):
     assert rel_max(got, ref, np.max(np.abs(phi))) <= 1e-12
     # size-independent property: the border layers are phi0 bit for bit
     assert np.array_equal(out[:2], phi[:2]) and np.array_equal(out[:, -2:], phi[:, -2:])
+    # the velocity as component planes: the same step bit for bit
+    planes = to_dev(np.ascontiguousarray(np.moveaxis(vel, -1, 0)))
+    outp = to_np(D.advect_sl(to_dev(phi), planes, win, dt, "cubic", "rk2", planes=True))
+    assert np.array_equal(outp, out)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_levelset_step_planes_equal_per_node(dev, dtype, dim):
+    """dcl_levelset_step with component planes: phi and curvature equal the
+    per-node velocity's bit for bit."""
+    n = 48 if dim == 3 else 120
+    win, phi0, vel, _ = _case(dim, n, seed=40 + dim, p=(0, 0, 0))
+    phi0, vel = rounded(phi0, dtype), rounded(vel, dtype)
+    pd = to_dev(phi0, dtype)
+    planes = to_dev(np.ascontiguousarray(np.moveaxis(vel, -1, 0)), dtype)
+    for interp, bt in (("cubic", "rk2"), ("linear", "euler"), ("linear", "rk2")):
+        a, ka = D.levelset_step(pd, to_dev(vel, dtype), win, 1.5 / n, interp, bt)
+        b, kb = D.levelset_step(pd, planes, win, 1.5 / n, interp, bt, planes=True)
+        assert torch.equal(a, b) and torch.equal(ka, kb)
 
 
 def test_zalesak_c1_revolution_fp64(dev):
