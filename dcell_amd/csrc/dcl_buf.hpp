@@ -157,11 +157,39 @@ __device__ __forceinline__ void mac_vel_buf(rsrc_t rv, const KVel& v, const doub
   mac_vel_buf_split<T>(rv, v, b0, b1, f0, f1, V);
 }
 
-// departure point of node (X, Y, Z) (see sl_departure) with buffer loads
+// node velocity (see node_vel) with component planes for a thread that
+// walks +z through the nodes of one (X, Y): the w face above the node at one
+// plane is the face below it at the next.  carry (uniform): wc holds this
+// node's lower w face, loaded as the upper one at the previous plane; wc is
+// set to this node's upper face.
+template <typename T>
+__device__ __forceinline__ void node_vel_planes_z(rsrc_t rv, const KVel& v, int X, int Y, int Z, bool carry, T& wc,
+                                                  T V[3]) {
+  const int x = clampi(X, v.w.px, v.w.px + v.w.nx - 2) - v.w.px;
+  const int y = clampi(Y, v.w.py, v.w.py + v.w.ny - 2) - v.w.py;
+  const int z = clampi(Z, v.w.pz, v.w.pz + v.w.nz - 2) - v.w.pz;
+  const int e = (int)sizeof(T);
+  const int sy = v.w.nx, sz = v.w.ny * sy;
+  const int vo = ((z * v.w.ny + y) * v.w.nx + x) * e;
+  const int ce = v.cs * e;
+  T p[2];
+  BL<T>::two(rv, vo, 0, p);
+  V[0] = vmid(p[0], p[1]);
+  V[1] = vmid(BL<T>::one(rv, vo + ce, 0), BL<T>::one(rv, vo + ce, sy * e));
+  T lo;
+  if (carry)
+    lo = wc;
+  else
+    lo = BL<T>::one(rv, vo + 2 * ce, 0);
+  wc = BL<T>::one(rv, vo + 2 * ce, sz * e);
+  V[2] = vmid(lo, wc);
+}
+
+// departure point of node (X, Y, Z) (see sl_departure) from its node
+// velocity V, the RK2 midpoint velocity read with buffer loads
 template <typename T, int BT>
-__device__ __forceinline__ void sl_dep3_buf(rsrc_t rv, const SLP& P, int X, int Y, int Z, double S[3]) {
-  T V[3];
-  node_vel_buf<T>(rv, P.v, X, Y, Z, V);
+__device__ __forceinline__ void sl_dep3_from(rsrc_t rv, const SLP& P, int X, int Y, int Z, const T V[3],
+                                             double S[3]) {
   const double Xd[3] = {(double)X, (double)Y, (double)Z};
   if (BT == DCL_RK2) {
     double Xm[3];
@@ -175,6 +203,14 @@ __device__ __forceinline__ void sl_dep3_buf(rsrc_t rv, const SLP& P, int X, int 
 #pragma unroll
     for (int a = 0; a < 3; ++a) S[a] = Xd[a] - (double)V[a] * P.c[a];
   }
+}
+
+// departure point of node (X, Y, Z) (see sl_departure) with buffer loads
+template <typename T, int BT>
+__device__ __forceinline__ void sl_dep3_buf(rsrc_t rv, const SLP& P, int X, int Y, int Z, double S[3]) {
+  T V[3];
+  node_vel_buf<T>(rv, P.v, X, Y, Z, V);
+  sl_dep3_from<T, BT>(rv, P, X, Y, Z, V, S);
 }
 
 // Bilinear3D (Common/Grid3D.c:17-28) sum in the order i, j, k over the

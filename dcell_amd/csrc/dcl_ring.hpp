@@ -307,6 +307,11 @@ __device__ __forceinline__ void ring_column(const T* __restrict__ phi0, T* __res
   const int i = x0 + threadIdx.x, j = y0 + threadIdx.y;
   const bool in_w = i < nx && j < ny;
   const bool in_xy = i >= 2 && i <= nx - 3 && j >= 2 && j <= ny - 3;
+  // component planes: the node's upper w face carries to the next plane
+  // (node_vel_planes_z) when the previous plane was computed here and the
+  // velocity window's clamp did not hold z there
+  const bool planes = P.v.dof == 1;
+  T wc = T(0);
   for (int k = z0; k < z1; ++k) {
     const int zn = k + RG_H + 1;
     const bool pre = zn < nz && k + 1 < z1;
@@ -325,6 +330,13 @@ __device__ __forceinline__ void ring_column(const T* __restrict__ phi0, T* __res
           const int b = (k - z0) % VR_R;
           const int sl[4] = {b, (b + 1) % VR_R, (b + 2) % VR_R, (b + 3) % VR_R};
           sl_dep3_ring<T, BT, V>(rv, vr, sl, P, i, j, k, x0, y0, S);
+        } else if (planes) {
+          const int vz0 = P.v.w.pz, vz1 = P.v.w.pz + P.v.w.nz - 2;
+          const bool carry = k - 1 >= max(z0, 2) &&
+                             clampi(P.w.pz + k, vz0, vz1) == clampi(P.w.pz + k - 1, vz0, vz1) + 1;
+          T V[3];
+          node_vel_planes_z<T>(rv, P.v, P.w.px + i, P.w.py + j, P.w.pz + k, carry, wc, V);
+          sl_dep3_from<T, BT>(rv, P, P.w.px + i, P.w.py + j, P.w.pz + k, V, S);
         } else {
           sl_dep3_buf<T, BT>(rv, P, P.w.px + i, P.w.py + j, P.w.pz + k, S);
         }

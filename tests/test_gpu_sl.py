@@ -149,6 +149,28 @@ def test_sl_component_planes_equal_per_node(dev, dtype, interp, bt, dim, sl_kern
     assert np.array_equal(va, vb)
 
 
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("interp,bt", [("cubic", "rk2"), ("linear", "euler")])
+def test_sl_component_planes_clamped_velocity_window(dev, dtype, interp, bt, sl_kernel):
+    """A velocity window that covers only the middle planes of phi's (every
+    node above or below it clamps to its end planes, so a thread walking z
+    meets planes whose velocity plane does not advance) and ends inside phi's
+    in x and y: planes equal the per-node layout and the oracle."""
+    n = 28
+    win, phi0, _, rng = _case(3, n, seed=77)
+    vwin = D.GridWindow((win.p[0] + 2, win.p[1] - 1, win.p[2] + 6), (n - 4, n + 2, n - 13), win.d)
+    vel = rounded(rng.uniform(-1.0, 1.0, size=(vwin.n[2], vwin.n[1], vwin.n[0], 3)), dtype)
+    phi0 = rounded(phi0, dtype)
+    planes = np.ascontiguousarray(np.moveaxis(vel, -1, 0))
+    dt = 1.4 / (n - 1)
+    a = to_np(D.advect_sl(to_dev(phi0, dtype), to_dev(vel, dtype), win, dt, interp, bt, vel_win=vwin))
+    b = to_np(D.advect_sl(to_dev(phi0, dtype), to_dev(planes, dtype), win, dt, interp, bt, vel_win=vwin,
+                          planes=True))
+    assert np.array_equal(a, b)
+    ref = O.sl_advect(phi0, vel, owin(win), dt, IM[interp], BM[bt], vwin=owin(vwin))
+    assert rel_max(b, ref, np.max(np.abs(phi0))) <= TOL[dtype]
+
+
 def test_sl_fluid_layout_udof(dev):
     """The fluid's [p, u, v, w] velocity (udof = 1, FluidField.c:146-151)."""
     win, phi0, vel, rng = _case(3, 20, seed=5)
