@@ -1183,7 +1183,14 @@ FORM_ONLY = {"reinitialize_parallel_fp64"}
 
 
 def forms(args):
-    return {name: run() for name, run in FORMS.items() if name not in FORM_ONLY}
+    out = {}
+    for name, run in FORMS.items():
+        if name in FORM_ONLY:
+            continue
+        t0 = time.perf_counter()
+        out[name] = run()
+        print(f"form {name}: {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    return out
 
 
 def main():
