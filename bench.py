@@ -1116,7 +1116,10 @@ def dropin_run(n=256, calls=10):
 def copy_ceiling():
     """The measured HBM ceiling: dcl_stream_copy (the library's streaming copy
     kernel) of 1 GiB, best of 10, read + write bytes over its time, with the
-    same for torch's device copy beside it."""
+    same for torch's device copy beside it; and the streaming copy's rate
+    over working sets of 64 MiB to 1 GiB copied again and again (a copy whose
+    source and destination fit the 256 MB last-level cache between HBM and
+    the XCDs runs faster than HBM allows)."""
     import ctypes as C
     import torch
     from dcell_amd import _lib as L
@@ -1124,7 +1127,7 @@ def copy_ceiling():
     src = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")
     dst = torch.empty_like(src)
 
-    def best_of(fn, reps=10):
+    def best_of(fn, nbytes, reps=10):
         fn()
         best = 1e30
         for _ in range(reps):
@@ -1134,14 +1137,18 @@ def copy_ceiling():
             e1.record()
             torch.cuda.synchronize()
             best = min(best, e0.elapsed_time(e1) * 1e-3)
-        return 2 * (1 << 30) / best
+        return 2 * nbytes / best
 
-    own = best_of(lambda: L.call("dcl_stream_copy", C.c_void_p(dst.data_ptr()), C.c_void_p(src.data_ptr()),
-                                 1 << 30, stream_handle()))
-    torch_copy = best_of(lambda: dst.copy_(src))
+    def own_copy(nbytes):
+        return lambda: L.call("dcl_stream_copy", C.c_void_p(dst.data_ptr()), C.c_void_p(src.data_ptr()),
+                              nbytes, stream_handle())
+
+    own = best_of(own_copy(1 << 30), 1 << 30)
+    torch_copy = best_of(lambda: dst.copy_(src), 1 << 30)
+    by_size = {f"{m} MiB": round(best_of(own_copy(m << 20), m << 20) / 1e9, 1) for m in (64, 128, 256, 512, 1024)}
     del src, dst
     torch.cuda.empty_cache()
-    return own, torch_copy
+    return own, torch_copy, by_size
 
 
 def _f64():
@@ -1247,9 +1254,10 @@ def main():
         "kernels_ms": {"sl_advect": r["sl_ms"], "curvature": r["curv_ms"]},
         "curvature_gbs": r["curv_gbs"],
     }
-    own, torch_copy = copy_ceiling()
+    own, torch_copy, by_size = copy_ceiling()
     line["measured_copy_gbs"] = own / 1e9
     line["torch_copy_gbs"] = torch_copy / 1e9
+    line["copy_gbs_by_working_set"] = by_size
     line["roofline"]["frac_of_measured_copy"] = achieved / own
     line["roofline"]["limiter"] = read_limiter()
     if not args.no_forms:
