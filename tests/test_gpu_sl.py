@@ -116,6 +116,32 @@ def test_sl_ring_equals_gather(dev, monkeypatch, dtype, interp, shape, amp, tile
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("interp,bt", [("cubic", "rk2"), ("linear", "rk2"), ("cubic", "euler")])
+@pytest.mark.parametrize("shape,amp", [((37, 23, 50), 1.0), ((70, 9, 33), 3.0), ((5, 6, 7), 1.0)])
+@pytest.mark.parametrize("tile", ["0", "1", "2"])
+def test_sl_ring_planes_equals_gather(dev, monkeypatch, dtype, interp, bt, shape, amp, tile):
+    """Component planes through the LDS-ring kernel (every column shape)
+    give the per-node gather kernel's values bit for bit, on windows that
+    are not multiples of the column, a flow slow enough to stay in the ring
+    and one fast enough to leave it."""
+    rng = np.random.default_rng(sum(shape) + 1)
+    nx, ny, nz = shape
+    h = 1.0 / (max(shape) - 1)
+    win = D.GridWindow((1, -4, 2), (nx, ny, nz), (h, h, h))
+    phi0 = rounded(rng.normal(size=(nz, ny, nx)), dtype)
+    vel = rounded(rng.uniform(-amp, amp, size=(nz, ny, nx, 3)), dtype)
+    planes = to_dev(np.ascontiguousarray(np.moveaxis(vel, -1, 0)), dtype)
+    monkeypatch.setenv("DCL_SL_KERNEL", "gather")
+    ref_gpu = to_np(D.advect_sl(to_dev(phi0, dtype), to_dev(vel, dtype), win, h, interp, bt))
+    monkeypatch.setenv("DCL_SL_KERNEL", "ring")
+    monkeypatch.setenv("DCL_RING_TILE", tile)
+    got = to_np(D.advect_sl(to_dev(phi0, dtype), planes, win, h, interp, bt, planes=True))
+    assert np.array_equal(got, ref_gpu)
+    ref = O.sl_advect(phi0, vel, owin(win), h, IM[interp], BM[bt])
+    assert rel_max(got, ref, np.max(np.abs(phi0))) <= TOL[dtype] * 4
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 @pytest.mark.parametrize("interp", ["linear", "cubic"])
 @pytest.mark.parametrize("bt", ["euler", "rk2"])
 @pytest.mark.parametrize("dim", [2, 3])
