@@ -833,6 +833,9 @@ extern "C" int dcl_sl_adjoint_step(const void* phi0, const dcl_window* win, cons
   if (interp != DCL_LINEAR && interp != DCL_CUBIC) return set_error(DCL_ERR_ARG_WRONG, "bad interp");
   DCL_TRY(check_window(win, "phi", 5));
   if (win->n.z < 2) return set_error(DCL_ERR_SUP, "the adjoint is 3D only");
+  // phi0's stencils are read with buffer loads at signed 32-bit byte offsets
+  if (win_nodes(*win) * (precision == DCL_F64 ? 8 : 4) >= ((int64_t)1 << 31))
+    return set_error(DCL_ERR_ARG_SIZ, "dcl_sl_adjoint_step takes phi windows below 2 GiB");
   if (lam == lam_next || lam == phi0 || hsum == lam || hsum == lam_next)
     return set_error(DCL_ERR_ARG_WRONG, "lam and hsum must not alias inputs or each other");
   AdjP P;

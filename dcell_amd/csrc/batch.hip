@@ -523,10 +523,10 @@ extern "C" int dcl_batch_create(int32_t ncells, const dcl_window* wins, int prec
       return set_error(DCL_ERR_SUP, "cell %d is %dD, cell 0 %dD: a batch holds cells of one dimension", c,
                        wins[c].n.z < 2 ? 2 : 3, dim);
     // the 3D kernels address a cell's velocity through a buffer descriptor
-    // whose byte size is 32-bit
+    // with signed 32-bit byte offsets
     const int64_t vbytes = dim * win_nodes(wins[c]) * (precision == DCL_F64 ? 8 : 4);
-    if (vbytes >= ((int64_t)1 << 32))
-      return set_error(DCL_ERR_ARG_SIZ, "cell %d: its velocity takes %lld bytes; the limit is 2^32 - 1", c,
+    if (vbytes >= ((int64_t)1 << 31))
+      return set_error(DCL_ERR_ARG_SIZ, "cell %d: its velocity takes %lld bytes; the limit is 2^31 - 1", c,
                        (long long)vbytes);
   }
   dcl_batch b = new (std::nothrow) dcl_batch_s();

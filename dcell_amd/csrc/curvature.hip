@@ -20,7 +20,7 @@ struct CurvP {
 };
 
 // dense 3D windows use the 2.5D LDS kernel; DCL_DENSE_KERNEL=gather selects
-// the plain per-node kernel that windows of 4 GiB and more take (read at
+// the plain per-node kernel that windows of 2 GiB and more take (read at
 // every call)
 static bool use_tiled_curv() {
   const char* e = getenv("DCL_DENSE_KERNEL");

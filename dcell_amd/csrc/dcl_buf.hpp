@@ -268,9 +268,11 @@ __device__ __forceinline__ T sl_point3_buf(rsrc_t rp, rsrc_t rv, const SLP& P, i
   return gather3_buf<T, INTERP>(rp, P.w, S);
 }
 
-// windows whose bytes a buffer descriptor can span
+// windows the buffer-load kernels take: their per-lane byte offsets are
+// signed 32-bit values, so every buffer stays below 2 GiB (larger windows
+// take the pointer-based kernels)
 __host__ inline bool buf_ok(long long phi_bytes, long long vel_bytes) {
-  return phi_bytes < 0xFFFFFFFFll && vel_bytes < 0xFFFFFFFFll;
+  return phi_bytes < 0x80000000ll && vel_bytes < 0x80000000ll;
 }
 
 }  // namespace dcl

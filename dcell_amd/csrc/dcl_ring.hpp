@@ -286,7 +286,7 @@ __device__ __forceinline__ void ring_column(const T* __restrict__ phi0, T* __res
   const int z1 = min(nz, z0 + zc);
   const int tid = threadIdx.x + G::TX * threadIdx.y;
   // bytes per plane of phi0 (a third of velocity's); products are taken
-  // unsigned, the buffer offsets being 32-bit (windows below 4 GiB)
+  // unsigned, the buffer offsets being 32-bit (windows below 2 GiB)
   const unsigned pz = (unsigned)(nx * ny) * (unsigned)sizeof(T);
   int xo[G::PF], so[G::PF], vxo[V::PF], vso[V::PF];
   ring_offsets<T, G>(P.w, x0, y0, tid, xo, so);

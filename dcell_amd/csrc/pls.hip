@@ -33,7 +33,7 @@ __device__ __forceinline__ double dist2(double dx, double dy, double dz, bool th
 }
 
 // --------------------------------------------------------------- advect
-// BUF (3D, velocity window below 4 GiB): the MAC interpolations read with
+// BUF (3D, velocity window below 2 GiB): the MAC interpolations read with
 // buffer loads (32-bit offsets; with 3 FP32 components an x pair of faces is
 // one 16-byte load), the same arithmetic as the pointer path
 template <typename T, int DIM, bool BUF>

@@ -81,7 +81,7 @@ __global__ __launch_bounds__(256) void sl_band(const T* __restrict__ phi0, T* __
 // and for trilinear RK2 with component planes (FP64 256^3: 0.25 ms against
 // the ring's 0.29).  DCL_SL_KERNEL=ring|gather overrides the
 // choice (read at every call); DCL_SL_KERNEL=plain selects the pointer-based
-// gather that windows of 4 GiB and more take.
+// gather that windows of 2 GiB and more take.
 static const char* sl_kernel_env() { return getenv("DCL_SL_KERNEL"); }
 
 static bool sl_ring_selected(int interp, int bt, bool planes) {

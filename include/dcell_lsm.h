@@ -478,7 +478,8 @@ int dcl_sl_adjoint(const void *phi0, const dcl_window *win, const dcl_velocity *
  *     with the velocity scatter of dcl_sl_adjoint (MAC corners at the RK2
  *     midpoint and the node's faces).
  * The sum of dcl_sl_adjoint over the steps equals these to rounding.  No
- * counterpart in the reference; the velocity window must equal phi's. */
+ * counterpart in the reference; the velocity window must equal phi's, and
+ * dcl_sl_adjoint_step takes phi windows below 2 GiB (DCL_ERR_ARG_SIZ). */
 int dcl_sl_adjoint_record_bytes(const dcl_window *win, int64_t *bytes);
 int dcl_sl_adjoint_prepare(const dcl_window *win, const dcl_velocity *vel, double dt, int precision,
                            int interp, int backtrace, void *record, void *stream);

@@ -66,6 +66,22 @@ def test_abi_version_and_argument_errors():
     assert rc == 62 and b"udof 0" in lib.dcl_last_error()
 
 
+def test_buffer_load_paths_reject_windows_of_2_gib():
+    """Kernels that read through buffer descriptors use signed 32-bit byte
+    offsets: the adjoint step and the batch refuse windows of 2 GiB and more
+    with DCL_ERR_ARG_SIZ before any device work."""
+    lib = L.lib()
+    dummy = C.c_void_p(16)
+    big = L.Window(L.ICoor(0, 0, 0), L.ICoor(1024, 1024, 300), L.Coor(1, 1, 1))  # 2.5 GB of FP64
+    rc = lib.dcl_sl_adjoint_step(dummy, C.byref(big), C.c_void_p(32), C.c_void_p(48), C.c_void_p(64),
+                                 C.c_void_p(80), 0.1, L.DCL_F64, L.DCL_CUBIC, None)
+    assert rc == 60 and b"2 GiB" in lib.dcl_last_error()
+    cell = L.Window(L.ICoor(0, 0, 0), L.ICoor(600, 600, 600), L.Coor(1, 1, 1))   # velocity 2.6 GB in FP32
+    out = C.c_void_p()
+    rc = lib.dcl_batch_create(1, C.byref(cell), L.DCL_F32, L.DCL_CUBIC, L.DCL_RK2, C.byref(out))
+    assert rc == 60 and b"2^31" in lib.dcl_last_error()
+
+
 def test_missing_library_fails_loudly(monkeypatch):
     monkeypatch.setattr(L, "_lib", None)
     monkeypatch.setattr(L, "LIB_PATH", "/nonexistent/libdcell_lsm.so")

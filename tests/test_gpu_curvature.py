@@ -22,7 +22,7 @@ pytestmark = pytest.mark.gpu
                                                 (2, "lss", "tiled")])
 def test_curvature_dense_and_band(dev, monkeypatch, dtype, tol, dim, stencil, kernel):
     # kernel: the 3D plane-ring kernel, or (DCL_DENSE_KERNEL=gather) the per-node
-    # kernel that windows of 4 GiB and more take
+    # kernel that windows of 2 GiB and more take
     monkeypatch.setenv("DCL_DENSE_KERNEL", kernel)
     rng = np.random.default_rng(31 + dim)
     n = 36 if dim == 3 else 80

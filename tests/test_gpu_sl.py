@@ -34,7 +34,7 @@ def _case(dim, n, seed, p=(3, -2, 5), amp=1.0):
 @pytest.fixture(params=["gather", "ring", "plain"])
 def sl_kernel(request, monkeypatch):
     """Every dense 3D SL kernel: per-node buffer-load gathers, phi0 staged in an
-    LDS ring, and the pointer-based gather that windows of 4 GiB and more take
+    LDS ring, and the pointer-based gather that windows of 2 GiB and more take
     (selected by DCL_SL_KERNEL at every call)."""
     monkeypatch.setenv("DCL_SL_KERNEL", request.param)
     return request.param
