@@ -261,10 +261,9 @@ def counter_summary(args):
     with open(out, "w") as fo:
         fo.write(f"# rocprofv3 counters: {name}\n# one pass per line of counters (from the repository root):\n")
         for q, c in enumerate(COUNTER_PASSES):
-            fo.write(f"#   rocprofv3 --pmc {c} -d gpurun_out/pmc_{probe_name}_{q} --output-format csv -- "
-                     f"python3 bench.py --probe {probe_name}\n")
-        fo.write(f"#   python3 bench.py --counter-summary {name} {probe_name} "
-                 + " ".join(f"gpurun_out/pmc_{probe_name}_{q}" for q in range(len(COUNTER_PASSES))) + "\n")
+            d = dirs[q] if q < len(dirs) else f"DIR{q}"
+            fo.write(f"#   rocprofv3 --pmc {c} -d {d} --output-format csv -- python3 bench.py --probe {probe_name}\n")
+        fo.write(f"#   python3 bench.py --counter-summary {name} {probe_name} " + " ".join(dirs) + "\n")
         fo.write("# values: mean per dispatch; '/wave' = per wavefront of 64 lanes;\n"
                  "# FETCH_SIZE / WRITE_SIZE in KiB (raw: the copy kernel reads 2x its FETCH_SIZE, see the\n"
                  f"# traffic summary); GRBM_GUI_ACTIVE sums the {XCDS} XCDs' busy cycles, so the kernel's\n"
