@@ -1194,7 +1194,10 @@ def forms(args):
         if name in FORM_ONLY:
             continue
         t0 = time.perf_counter()
-        out[name] = run()
+        try:
+            out[name] = run()
+        except Exception as e:  # a failing form is reported, the headline line still prints
+            out[name] = {"error": f"{type(e).__name__}: {e}"}
         print(f"form {name}: {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     return out
 
