@@ -1265,7 +1265,10 @@ def main():
     if not args.no_forms:
         line["forms"] = forms(args)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(args)
+        try:
+            line["cpu_baseline"] = cpu_baseline(args)
+        except Exception as e:  # reported in the line; the GPU figures still print
+            line["cpu_baseline"] = {"error": f"{type(e).__name__}: {e}"}
     print(json.dumps(line), flush=True)
 
 
