@@ -185,6 +185,40 @@ __device__ __forceinline__ void node_vel_planes_z(rsrc_t rv, const KVel& v, int 
   V[2] = vmid(lo, wc);
 }
 
+// node_vel_planes_z split for a thread that loads the next plane's node
+// faces while it works the current one: the faces of node (X, Y, Z) other
+// than its lower w face (the u pair, the v pair, the upper w face), and the
+// lower w face alone
+template <typename T>
+struct NodeFaces {
+  T u0, u1, v0, v1, wh;
+};
+template <typename T>
+__device__ __forceinline__ int node_face_offset(const KVel& v, int X, int Y, int Z) {
+  const int x = clampi(X, v.w.px, v.w.px + v.w.nx - 2) - v.w.px;
+  const int y = clampi(Y, v.w.py, v.w.py + v.w.ny - 2) - v.w.py;
+  const int z = clampi(Z, v.w.pz, v.w.pz + v.w.nz - 2) - v.w.pz;
+  return ((z * v.w.ny + y) * v.w.nx + x) * (int)sizeof(T);
+}
+template <typename T>
+__device__ __forceinline__ NodeFaces<T> node_faces_planes(rsrc_t rv, const KVel& v, int X, int Y, int Z) {
+  const int e = (int)sizeof(T);
+  const int vo = node_face_offset<T>(v, X, Y, Z), ce = v.cs * e;
+  NodeFaces<T> f;
+  T p[2];
+  BL<T>::two(rv, vo, 0, p);
+  f.u0 = p[0];
+  f.u1 = p[1];
+  f.v0 = BL<T>::one(rv, vo + ce, 0);
+  f.v1 = BL<T>::one(rv, vo + ce, v.w.nx * e);
+  f.wh = BL<T>::one(rv, vo + 2 * ce, v.w.nx * v.w.ny * e);
+  return f;
+}
+template <typename T>
+__device__ __forceinline__ T node_wlow_planes(rsrc_t rv, const KVel& v, int X, int Y, int Z) {
+  return BL<T>::one(rv, node_face_offset<T>(v, X, Y, Z) + 2 * v.cs * (int)sizeof(T), 0);
+}
+
 // departure point of node (X, Y, Z) (see sl_departure) from its node
 // velocity V, the RK2 midpoint velocity read with buffer loads
 template <typename T, int BT>
